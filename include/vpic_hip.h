@@ -1,0 +1,297 @@
+/*
+ * vpic_hip.h -- C-ABI of libvpichip, the MI355X (gfx950) implementation of
+ * VPIC's per-step particle-in-cell hot path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an int status (0 = ok, < 0 = error; vpic_hip_last_error() gives
+ * the message).  Array arguments are DEVICE pointers (HBM) unless the name
+ * says `host_`.  `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  Calls are stream-ordered; a call returns a host-visible
+ * scalar (nm, energies) only after synchronising its stream.
+ *
+ * The data types below are byte-for-byte the reference's, so a reference
+ * array can be copied to HBM and handed over without repacking.
+ * Citations are paths in lanl/vpic (legacy), relative to its root.
+ */
+#ifndef VPIC_HIP_H
+#define VPIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Data types (layouts fixed by the reference)                              */
+/* ------------------------------------------------------------------------ */
+
+/* src/species_advance/species_advance_aos.h:21-32 (32 B).  dx,dy,dz on
+ * [-1,1] in cell coordinates, i = local voxel index (8*voxel+face while a
+ * particle waits for boundary handling). */
+typedef struct vpic_particle {
+  float dx, dy, dz;
+  int32_t i;
+  float ux, uy, uz;
+  float w;
+} vpic_particle_t;
+
+/* src/species_advance/species_advance_aos.h:37-40 (16 B).  Remaining
+ * displacement in half-cell units and the index of the particle. */
+typedef struct vpic_particle_mover {
+  float dispx, dispy, dispz;
+  int32_t i;
+} vpic_particle_mover_t;
+
+/* src/sf_interface/sf_interface.h:62-74, scalar build (PAD_SIZE 2, 80 B). */
+typedef struct vpic_interpolator {
+  float ex, dexdy, dexdz, d2exdydz;
+  float ey, deydz, deydx, d2eydzdx;
+  float ez, dezdx, dezdy, d2ezdxdy;
+  float cbx, dcbxdx;
+  float cby, dcbydy;
+  float cbz, dcbzdz;
+  float _pad[2];
+} vpic_interpolator_t;
+
+/* src/sf_interface/sf_interface.h:115-123 (48 B): 4x the charge through
+ * each quarter face per step. */
+typedef struct vpic_accumulator {
+  float jx[4];
+  float jy[4];
+  float jz[4];
+} vpic_accumulator_t;
+
+/* src/field_advance/field_advance.h:152-160 (80 B). */
+typedef struct vpic_field {
+  float ex, ey, ez, div_e_err;
+  float cbx, cby, cbz, div_b_err;
+  float tcax, tcay, tcaz, rhob;
+  float jfx, jfy, jfz, rhof;
+  int16_t ematx, ematy, ematz, nmat;
+  int16_t fmatx, fmaty, fmatz, cmat;
+} vpic_field_t;
+
+/* Boundary codes, src/grid/grid.h:14-38. */
+enum {
+  VPIC_BC_ANTI_SYMMETRIC_FIELDS = -1, /* pec / metal: E_tang = 0 */
+  VPIC_BC_SYMMETRIC_FIELDS      = -2, /* B_tang = 0, B_norm = 0 */
+  VPIC_BC_PMC_FIELDS            = -3, /* B_tang = 0 */
+  VPIC_BC_ABSORB_FIELDS         = -4, /* not supported by the device path */
+  VPIC_BC_REFLECT_PARTICLES     = -1,
+  VPIC_BC_ABSORB_PARTICLES      = -2,
+  /* Face partner is this same domain (one-rank periodic wrap). */
+  VPIC_BC_PERIODIC              = 0
+};
+
+/* The local grid, restating the fields of grid_t (src/grid/grid.h:73-131)
+ * that the path reads.  `neighbor` is an optional DEVICE copy of the
+ * reference's 6*nv int64 neighbor table (src/grid/ops.cc:45-76); when it
+ * is NULL the kernels derive neighbours from the voxel coordinates and
+ * `pbc` (particle BC per domain face, order -x,-y,-z,+x,+y,+z), which
+ * reproduces size_grid + join_grid for one rank (src/grid/ops.cc:19-165,
+ * src/grid/partition.cc:35-90).  `fbc` gives the field BC of each domain
+ * face in the same order (VPIC_BC_PERIODIC or a local field code). */
+typedef struct vpic_grid {
+  float dt, cvac, eps0;
+  float dx, dy, dz, dV;
+  float rdx, rdy, rdz, r8V;
+  int32_t nx, ny, nz;
+  int32_t sx, sy, sz, nv;
+  int64_t rangel, rangeh;
+  const int64_t *neighbor;  /* device, 6*nv, or NULL */
+  int32_t pbc[6];
+  int32_t fbc[6];
+} vpic_grid_t;
+
+/* Field-solve material coefficients for the one vacuum material and the
+ * TCA damping, src/field_advance/standard/sfa_private.h:14-30 and
+ * src/field_advance/standard/sfa.cc:100-136. */
+typedef struct vpic_field_params {
+  float decayx, drivex, decayy, drivey, decayz, drivez;
+  float rmux, rmuy, rmuz;
+  float epsx, epsy, epsz;
+  float damp;
+} vpic_field_params_t;
+
+/* ------------------------------------------------------------------------ */
+/* Library state                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Select and initialise device `device` for this thread.  Idempotent. */
+int vpic_hip_init(int device);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *vpic_hip_last_error(void);
+
+/* Library version string. */
+const char *vpic_hip_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Species advance (src/species_advance/species_advance.h:64-157)          */
+/* ------------------------------------------------------------------------ */
+
+/* advance_p: relativistic Boris push with energy-conserving interpolation,
+ * in-cell Villasenor-Buneman deposit and move_p for cell crossers.
+ * Replaces advance_p(species_t*, accumulator_array_t*, const
+ * interpolator_array_t*) (species_advance.h:74-76; advance_p.cc:12-19 ->
+ * advance_p_pipeline.cc:253-340).  q and m are the species charge and
+ * mass; the coupling constants are formed in float exactly as
+ * advance_p_pipeline.cc:279-283.  Currents are added into `a` (one
+ * accumulator copy, nv entries).  Particles that hit a face the grid
+ * cannot handle are returned as movers in `pm` in particle-index order
+ * (up to max_nm, the rest rolled back as the reference does); `*host_nm`
+ * receives their count and `*host_n_ignored` (may be NULL) the number
+ * that did not fit.  Synchronous on return. */
+int vpic_hip_advance_p(vpic_particle_t *p, int32_t np, float q, float m,
+                       vpic_particle_mover_t *pm, int32_t max_nm,
+                       int32_t *host_nm, int32_t *host_n_ignored,
+                       vpic_accumulator_t *a, const vpic_interpolator_t *f,
+                       const vpic_grid_t *g, void *stream);
+
+/* Stream-ordered form of vpic_hip_advance_p: writes {nm, n_ignored} to the
+ * device int pair `dev_nm` instead of the host and does not synchronise. */
+int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
+                             float m, vpic_particle_mover_t *pm,
+                             int32_t max_nm, int32_t *dev_nm,
+                             vpic_accumulator_t *a,
+                             const vpic_interpolator_t *f,
+                             const vpic_grid_t *g, void *stream);
+
+/* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),
+ * applied to the nm movers pm[0..nm) in order on the device (one lane, as
+ * the reference's callers do, e.g. vpic/misc.cc:95).  `dev_ret` receives
+ * one int per mover: 1 if the mover is still in use.  q = species charge. */
+int vpic_hip_move_p(vpic_particle_t *p, vpic_particle_mover_t *pm,
+                    int32_t nm, vpic_accumulator_t *a, const vpic_grid_t *g,
+                    float q, int32_t *dev_ret, void *stream);
+
+/* sort_p: stable counting sort of p[0..np) by voxel and the voxel
+ * partition (species_advance.h:66; sort_p.cc:154-165 ->
+ * sort_p_pipeline.cc:221-371).  `partition` receives nv+1 ints.  `aux`
+ * is np particles of scratch; when `in_place` is 0 the sorted particles
+ * are left in `aux` (the caller swaps the two arrays) and p is scratch. */
+int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int32_t np,
+                    int32_t *partition, const vpic_grid_t *g, int in_place,
+                    void *stream);
+
+/* energy_p: kinetic energy of a species in FP64 (species_advance.h:119-120;
+ * energy_p_pipeline.cc:18-115).  Result in *host_en.  Synchronous. */
+int vpic_hip_energy_p(const vpic_particle_t *p, int32_t np, float q, float m,
+                      const vpic_interpolator_t *f, const vpic_grid_t *g,
+                      double *host_en, void *stream);
+
+/* accumulate_rho_p: trilinear charge deposit into f.rhof
+ * (species_advance.h; rho_p.cc:23-113). */
+int vpic_hip_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p,
+                              int32_t np, float q, const vpic_grid_t *g,
+                              void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Field/particle interface (src/sf_interface/sf_interface.h:82-176)        */
+/* ------------------------------------------------------------------------ */
+
+/* load_interpolator_array (sf_interface.h:100-101;
+ * interpolator_array_pipeline.cc:21-135). */
+int vpic_hip_load_interpolator_array(vpic_interpolator_t *fi,
+                                     const vpic_field_t *f,
+                                     const vpic_grid_t *g, void *stream);
+
+/* clear_accumulator_array (sf_interface.h:148; clear_array_pipeline.cc:40-67):
+ * zeroes all nv accumulators (ghosts included). */
+int vpic_hip_clear_accumulator_array(vpic_accumulator_t *a,
+                                     const vpic_grid_t *g, void *stream);
+
+/* reduce_accumulator_array (sf_interface.h:159): the device keeps one
+ * accumulator copy, so this is a checked no-op. */
+int vpic_hip_reduce_accumulator_array(vpic_accumulator_t *a,
+                                      const vpic_grid_t *g, void *stream);
+
+/* unload_accumulator_array (sf_interface.h:173-174;
+ * unload_accumulator_pipeline.cc:18-82,130-141): jf += c*(sum of quarter
+ * face currents) for voxels 1..n+1. */
+int vpic_hip_unload_accumulator_array(vpic_field_t *f,
+                                      const vpic_accumulator_t *a,
+                                      const vpic_grid_t *g, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Field advance (src/field_advance/field_advance.h:170-218)               */
+/* ------------------------------------------------------------------------ */
+
+/* clear_jf (standard/sfa.cc:232-237). */
+int vpic_hip_clear_jf(vpic_field_t *f, const vpic_grid_t *g, void *stream);
+
+/* clear_rhof. */
+int vpic_hip_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, void *stream);
+
+/* synchronize_jf: local_adjust_jf then the shared-face fold
+ * (standard/remote.cc:418-507, standard/local.cc:336-368). */
+int vpic_hip_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
+                            void *stream);
+
+/* advance_b: cB -= frac*c*dt*curl E on all faces, then local_adjust_norm_b
+ * (advance_b_pipeline.h:17-59; advance_b_pipeline.cc:20-125). */
+int vpic_hip_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac,
+                       void *stream);
+
+/* vacuum_advance_e: tangential-B ghosts, E update on every edge, then
+ * local_adjust_tang_e (vacuum_advance_e_pipeline.h:17-69;
+ * vacuum_advance_e_pipeline.cc:20-332).  frac must be 1. */
+int vpic_hip_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
+                              const vpic_field_params_t *mp, float frac,
+                              void *stream);
+
+/* energy_f: six FP64 field energies {ex,ey,ez,bx,by,bz}
+ * (vacuum_energy_f_pipeline.h:17-75; vacuum_energy_f_pipeline.cc:13-97).
+ * Synchronous. */
+int vpic_hip_energy_f(double host_en[6], const vpic_field_t *f,
+                      const vpic_grid_t *g, const vpic_field_params_t *mp,
+                      void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Set-up helpers (no reference counterpart on the device; they fill the   */
+/* same arrays the reference's set-up code fills on the host)               */
+/* ------------------------------------------------------------------------ */
+
+/* Fill g as partition_periodic_box does for one rank
+ * (src/grid/partition.cc:35-90 + src/grid/ops.cc:19-165); neighbor = NULL,
+ * pbc and fbc = periodic. */
+int vpic_hip_grid_periodic(vpic_grid_t *g, double gx0, double gy0,
+                           double gz0, double gx1, double gy1, double gz1,
+                           int32_t gnx, int32_t gny, int32_t gnz, float dt,
+                           float cvac, float eps0);
+
+/* Vacuum material coefficients as create_sfa_params computes them for
+ * eps = mu = 1, sigma = 0 (standard/sfa.cc:100-136). */
+int vpic_hip_field_params_vacuum(vpic_field_params_t *mp, float damp);
+
+/* Fill the host neighbor table (6*nv int64) that size_grid + join_grid
+ * build for this one-rank grid (src/grid/ops.cc:45-165). */
+int vpic_hip_grid_neighbor_table(const vpic_grid_t *g, int64_t *host_nb);
+
+/* Load a uniform thermal plasma: `ppc` particles per interior cell,
+ * positions uniform in the cell, momenta N(drift, uth) per axis, weight w;
+ * counter-based RNG keyed by `seed`.  Particle k belongs to cell
+ * k / ppc in x-fastest order.  Writes np = ppc*nx*ny*nz particles. */
+int vpic_hip_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
+                                 int32_t ppc, float uth_x, float uth_y,
+                                 float uth_z, float ud_x, float ud_y,
+                                 float ud_z, float w, uint64_t seed,
+                                 void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Timing of the push kernel (hipEvents on the launch stream)               */
+/* ------------------------------------------------------------------------ */
+
+/* When enabled, every advance_p brackets its in-cell push kernel with
+ * hipEvents; vpic_hip_push_timing returns the launches and total ms since
+ * the last reset (synchronises the recorded events). */
+int vpic_hip_push_timing_enable(int on);
+int vpic_hip_push_timing(int64_t *host_launches, double *host_total_ms,
+                         int64_t *host_particles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPIC_HIP_H */

@@ -1,0 +1,125 @@
+/*
+ * oracle.h -- CPU restatement of VPIC's per-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the device path in
+ * vpic_amd/ is compared against; only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and never as the thing that is
+ * measured or shipped.  Arrays are host arrays with the reference's
+ * layouts (include/vpic_hip.h).  Each function cites the reference
+ * (lanl/vpic legacy) file:line whose arithmetic order it follows; FP32
+ * expressions are evaluated exactly as written there (build with
+ * -ffp-contract=off, no fast-math).
+ */
+#ifndef VPIC_ORACLE_H
+#define VPIC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/vpic_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* partition_periodic_box for one rank (src/grid/partition.cc:35-90) plus
+ * size_grid strides (src/grid/ops.cc:25-30). */
+void orc_grid_periodic(vpic_grid_t *g, double gx0, double gy0, double gz0,
+                       double gx1, double gy1, double gz1, int gnx, int gny,
+                       int gnz, float dt, float cvac, float eps0);
+
+/* size_grid + join_grid / set_pbc neighbour table for one rank
+ * (src/grid/ops.cc:45-76,119-165): 6*nv int64 into nb. */
+void orc_neighbor_table(const vpic_grid_t *g, int64_t *nb);
+
+/* move_p, scalar branch (src/species_advance/standard/move_p.cc:215-378).
+ * Uses g->neighbor (host table). */
+int orc_move_p(vpic_particle_t *p0, vpic_particle_mover_t *pm,
+               vpic_accumulator_t *a0, const vpic_grid_t *g, float qsp);
+
+/* advance_p with one pipeline plus the host straggler pipeline, as
+ * EXEC_PIPELINES runs it with N_PIPELINE = 1
+ * (advance_p_pipeline.cc:21-245,253-340; pipelines_exec_pth.h:56-65):
+ * particles [0, np&~15) accumulate into copy 1, the tail into copy 0, then
+ * reduce_accumulator_array adds copy 1 into copy 0
+ * (reduce_array_pipeline.cc:21-60).  `a` is the single accumulator copy
+ * the caller sees (nv entries, added to, not overwritten).  Movers are
+ * compacted as advance_p_pipeline.cc:310-339 does.  Returns nm. */
+int orc_advance_p(vpic_particle_t *p, int np, float q, float m,
+                  vpic_particle_mover_t *pm, int max_nm, int *n_ignored,
+                  vpic_accumulator_t *a, const vpic_interpolator_t *f,
+                  const vpic_grid_t *g);
+
+/* The per-particle coupling constants of advance_p_pipeline.cc:279-283. */
+void orc_push_coefficients(float q, float m, const vpic_grid_t *g,
+                           float out[4]);
+
+/* load_interpolator_pipeline_scalar over voxels 1..n
+ * (interpolator_array_pipeline.cc:21-135). */
+void orc_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
+                           const vpic_grid_t *g);
+
+/* unload_accumulator over voxels 1..n+1 with the coupling constants of
+ * unload_accumulator_pipeline.cc:137-139 (unload_accumulator_pipeline.cc:
+ * 18-82). */
+void orc_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
+                            const vpic_grid_t *g);
+
+/* clear_jf (standard/sfa.cc:232-237) and clear_rhof. */
+void orc_clear_jf(vpic_field_t *f, const vpic_grid_t *g);
+void orc_clear_rhof(vpic_field_t *f, const vpic_grid_t *g);
+
+/* synchronize_jf: local_adjust_jf then the x, y, z face folds in that
+ * order (standard/remote.cc:418-507, standard/local.cc:336-368). */
+void orc_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g);
+
+/* advance_b: bulk + host leftovers + local_adjust_norm_b
+ * (advance_b_pipeline.h:17-59, advance_b_pipeline.cc:20-125,
+ * standard/local.cc:266-296). */
+void orc_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac);
+
+/* vacuum_advance_e: remote/local tangential-B ghosts, every E edge, then
+ * local_adjust_tang_e (vacuum_advance_e_pipeline.h:17-69,
+ * vacuum_advance_e_pipeline.cc:49-332, standard/local.cc:50-122,224-264,
+ * standard/remote.cc:61-134). */
+void orc_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
+                          const vpic_field_params_t *mp);
+
+/* create_sfa_params for the vacuum material eps=mu=1, sigma=0
+ * (standard/sfa.cc:100-136). */
+void orc_field_params_vacuum(vpic_field_params_t *mp, float damp);
+
+/* sort_p with one pipeline: stable counting sort by voxel and the
+ * partition array (sort_p_pipeline.cc:143-214,221-371).  partition has
+ * nv+1 entries; aux holds np particles of scratch. */
+void orc_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                int *partition, const vpic_grid_t *g);
+
+/* energy_p with one pipeline + host tail (energy_p_pipeline.cc:18-115). */
+double orc_energy_p(const vpic_particle_t *p, int np, float q, float m,
+                    const vpic_interpolator_t *f, const vpic_grid_t *g);
+
+/* vacuum energy_f with one pipeline + host tail
+ * (vacuum_energy_f_pipeline.h:17-75, vacuum_energy_f_pipeline.cc:13-97). */
+void orc_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
+                  const vpic_field_params_t *mp);
+
+/* accumulate_rho_p (src/species_advance/standard/rho_p.cc:23-113). */
+void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
+                          float q, const vpic_grid_t *g);
+
+/* The reference's CPU parallel scheme for advance_p, used only as the
+ * timed CPU baseline: n_thread pthreads each run one DISTRIBUTE(np,16)
+ * block into a private accumulator copy, the caller runs the np%16 tail,
+ * then the copies are summed (pipelines_exec_pth.h:56-65,
+ * advance_p_pipeline.cc:55-85, accumulator_array.cc:65-87).  `acc` must
+ * hold (n_thread+1)*nv accumulators; copy 0 receives the reduced sum. */
+int orc_advance_p_threaded(vpic_particle_t *p, int np, float q, float m,
+                           vpic_particle_mover_t *pm, int max_nm,
+                           vpic_accumulator_t *acc,
+                           const vpic_interpolator_t *f, const vpic_grid_t *g,
+                           int n_thread);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

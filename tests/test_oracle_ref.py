@@ -1,0 +1,257 @@
+"""Pins the oracle (oracle/oracle.c) to the reference's own scalar kernels
+compiled from the lanl/vpic sources (oracle/_ref), bit for bit.
+
+What each reference function is pinned by:
+  advance_p_pipeline_scalar + move_p    -> test_advance_p_* (bit-exact,
+                                           particles, accumulators, movers)
+  move_p (single mover)                 -> test_move_p_single
+  load_interpolator_pipeline_scalar     -> test_load_interpolator
+  unload_accumulator_pipeline_scalar    -> test_unload_accumulator
+  advance_b_pipeline_scalar (bulk)      -> test_advance_b (+ translation
+                                           invariance for the face leftovers)
+  vacuum_advance_e_pipeline_scalar      -> test_vacuum_advance_e (same)
+  subsort_pipeline_scalar               -> test_sort_p
+  energy_p_pipeline_scalar              -> test_energy_p
+  vacuum_energy_f_pipeline_scalar       -> test_energy_f
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import host_grid
+from vpic_amd import types as T
+
+import helpers as H
+
+pytestmark = pytest.mark.ref
+
+
+def _box(nx, ny, nz, cell=1.0, frac=0.95):
+    L = (nx * cell, ny * cell, nz * cell)
+    dt = frac * T.courant_length(L[0], L[1], L[2], nx, ny, nz)
+    return T.grid_periodic(0, 0, 0, L[0], L[1], L[2], nx, ny, nz, dt)
+
+
+def _push_inputs(g, ppc, uth, seed, e_amp=0.05, b_amp=0.05):
+    from oracle.oracle import Oracle
+    f = H.random_fields(g, seed, e_amp, b_amp)
+    fi = H.interp_array(g)
+    Oracle.get().load_interpolator(fi, f, g)
+    p = H.thermal_particles(g, ppc, uth, seed + 1)
+    return p, fi
+
+
+@pytest.mark.parametrize("q,m,uth,ppc", [(-1.0, 1.0, 0.1, 8),
+                                          (1.0, 25.0, 0.02, 8),
+                                          (-1.0, 1.0, 0.5, 13)])
+def test_advance_p_bitexact(orc, ref, q, m, uth, ppc):
+    g = _box(8, 8, 8)
+    p, fi = _push_inputs(g, ppc, uth, 11)
+    hg = host_grid(g)
+    pa, pb = p.copy(), p.copy()
+    aa, ab = H.acc_array(g), H.acc_array(g)
+    nma, iga, pma = orc.advance_p(pa, q, m, aa, fi, hg)
+    nmb, igb, pmb = ref.advance_p(pb, q, m, ab, fi, hg)
+    assert H.same_bits(pa, pb)
+    assert H.same_bits(aa, ab)
+    assert (nma, iga) == (nmb, igb) == (0, 0)
+    # crossers were exercised
+    assert np.any(pa["i"] != p["i"])
+
+
+def test_advance_p_walls_and_movers(orc, ref):
+    """outbndj-style 4x5x6 box, fast particles, reflecting x walls and
+    absorbing z walls: exercises reflection, returned movers, the mover
+    segment split and its overflow rollback (advance_p_pipeline.cc:221-237)."""
+    g = _box(4, 5, 6)
+    g.pbc[0] = g.pbc[3] = T.BC_REFLECT_PARTICLES
+    g.pbc[2] = g.pbc[5] = T.BC_ABSORB_PARTICLES
+    p, fi = _push_inputs(g, 7, 0.9, 5, e_amp=0.2, b_amp=0.2)
+    hg = host_grid(g)
+    for max_nm in (10000, 40, 3):
+        pa, pb = p.copy(), p.copy()
+        aa, ab = H.acc_array(g), H.acc_array(g)
+        ra = orc.advance_p(pa, -1.0, 1.0, aa, fi, hg, max_nm=max_nm)
+        rb = ref.advance_p(pb, -1.0, 1.0, ab, fi, hg, max_nm=max_nm)
+        assert H.same_bits(pa, pb)
+        assert H.same_bits(aa, ab)
+        assert ra[0] == rb[0] and ra[1] == rb[1]
+        assert H.same_bits(ra[2], rb[2])
+        if max_nm == 10000:
+            assert ra[0] > 0 and ra[1] == 0
+        if max_nm == 3:
+            assert ra[1] > 0
+
+
+def test_advance_p_threaded_particles_match(orc, ref):
+    """Particle results do not depend on the pipeline count; only the
+    deposit summation order does."""
+    g = _box(8, 8, 8)
+    p, fi = _push_inputs(g, 9, 0.1, 3)
+    hg = host_grid(g)
+    pa, pb = p.copy(), p.copy()
+    aa, ab = H.acc_array(g), H.acc_array(g)
+    orc.advance_p(pa, -1.0, 1.0, aa, fi, hg)
+    ref.advance_p(pb, -1.0, 1.0, ab, fi, hg, n_pipeline=4)
+    assert H.same_bits(pa, pb)
+    assert H.acc_rel_err(aa, ab) < 1e-6
+
+
+def test_move_p_single(orc, ref):
+    g = _box(4, 5, 6)
+    g.pbc[1] = g.pbc[4] = T.BC_REFLECT_PARTICLES
+    hg = host_grid(g)
+    rng = np.random.default_rng(7)
+    p = H.thermal_particles(g, 2, 0.1, 8)
+    for trial in range(200):
+        k = int(rng.integers(len(p)))
+        mv = np.zeros(1, dtype=T.MOVER)
+        mv["dispx"], mv["dispy"], mv["dispz"] = rng.uniform(-3, 3, 3)
+        mv["i"] = k
+        pa, pb = p.copy(), p.copy()
+        ma, mb = mv.copy(), mv.copy()
+        aa, ab = H.acc_array(g), H.acc_array(g)
+        ra = orc.move_p(pa, ma, aa, hg, -1.0)
+        rb = ref.move_p(pb, mb, ab, hg, -1.0)
+        assert ra == rb
+        assert H.same_bits(pa, pb) and H.same_bits(ma, mb)
+        assert H.same_bits(aa, ab)
+
+
+def test_load_interpolator(orc, ref):
+    g = _box(7, 5, 6)
+    f = H.random_full_fields(g, 2)
+    a, b = H.interp_array(g), H.interp_array(g)
+    orc.load_interpolator(a, f, g)
+    ref.load_interpolator(b, f, g)
+    assert H.same_bits(a, b)
+
+
+def test_unload_accumulator(orc, ref):
+    g = _box(7, 5, 6)
+    rng = np.random.default_rng(4)
+    acc = rng.standard_normal((g.nv, 12)).astype(np.float32)
+    # the reference assumes zero ghost accumulators
+    x, y, z = np.meshgrid(np.arange(g.nx + 2), np.arange(g.ny + 2),
+                          np.arange(g.nz + 2), indexing="ij")
+    ghost = ((x == 0) | (x == g.nx + 1) | (y == 0) | (y == g.ny + 1) |
+             (z == 0) | (z == g.nz + 1))
+    acc[g.voxel(x[ghost], y[ghost], z[ghost])] = 0
+    fa = H.random_full_fields(g, 3)
+    fb = fa.copy()
+    orc.unload_accumulator(fa, acc, g)
+    ref.unload_accumulator(fb, acc, g)
+    assert H.same_bits(fa, fb)
+
+
+def _interior_mask(g, lo):
+    v = np.arange(g.nv)
+    x = v % (g.nx + 2)
+    y = (v // (g.nx + 2)) % (g.ny + 2)
+    z = v // ((g.nx + 2) * (g.ny + 2))
+    return ((x >= lo) & (x <= g.nx) & (y >= lo) & (y <= g.ny) & (z >= lo) &
+            (z <= g.nz))
+
+
+def test_advance_b(orc, ref):
+    g = _box(7, 5, 6)
+    f = H.random_full_fields(g, 5)
+    fa, fb = f.copy(), f.copy()
+    orc.advance_b(fa, g, 0.5)
+    ref.advance_b_bulk(fb, g, 0.5)
+    m = _interior_mask(g, 1)
+    for c in ("cbx", "cby", "cbz"):
+        assert H.same_bits(fa[c][m], fb[c][m])
+
+
+def _roll_field(g, f, sx, sy, sz):
+    """Periodic translation of the interior+face data by (sx,sy,sz) cells:
+    interior voxel (x,y,z) <- (x-sx,y-sy,z-sz) wrapped on 1..n."""
+    a = f.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+    out = f.copy().reshape(a.shape)
+    inner = a[1:g.nz + 1, 1:g.ny + 1, 1:g.nx + 1]
+    out[1:g.nz + 1, 1:g.ny + 1, 1:g.nx + 1] = np.roll(
+        inner, shift=(sz, sy, sx), axis=(0, 1, 2))
+    return out.reshape(-1)
+
+
+def _periodize(g, f):
+    """Make the +faces (x=nx+1 etc.) copies of the x=1 faces, as a
+    periodic field is after a step."""
+    a = f.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+    a[:, :, g.nx + 1] = a[:, :, 1]
+    a[:, g.ny + 1, :] = a[:, 1, :]
+    a[g.nz + 1, :, :] = a[1, :, :]
+    a[:, :, 0] = a[:, :, g.nx]
+    a[:, 0, :] = a[:, g.ny, :]
+    a[0, :, :] = a[g.nz, :, :]
+    return f
+
+
+def test_fdtd_translation_invariance(orc):
+    """On a periodic box the face/leftover updates must equal the bulk
+    update of the translated field; with the bulk pinned to the reference
+    this pins every voxel."""
+    g = _box(6, 5, 4)
+    mp = T.vacuum_params()
+    f = _periodize(g, H.random_full_fields(g, 6))
+    for op in ("b", "e"):
+        fa = f.copy()
+        fr = _periodize(g, _roll_field(g, f, 2, 3, 1))
+        if op == "b":
+            orc.advance_b(fa, g, 0.5)
+            orc.advance_b(fr, g, 0.5)
+            comps = ("cbx", "cby", "cbz")
+        else:
+            orc.vacuum_advance_e(fa, g, mp)
+            orc.vacuum_advance_e(fr, g, mp)
+            comps = ("ex", "ey", "ez", "tcax", "tcay", "tcaz")
+        ra = _roll_field(g, fa, 2, 3, 1)
+        m = _interior_mask(g, 1)
+        for c in comps:
+            assert H.same_bits(ra[c][m], fr[c][m]), (op, c)
+        # the components living on the x = nx+1 face stay copies of x = 1
+        a = fa.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+        face = ("cbx",) if op == "b" else ("ey", "ez", "tcay", "tcaz")
+        for c in face:
+            assert H.same_bits(a[c][1:g.nz + 1, 1:g.ny + 1, g.nx + 1],
+                               a[c][1:g.nz + 1, 1:g.ny + 1, 1])
+
+
+def test_vacuum_advance_e(orc, ref):
+    g = _box(7, 5, 6)
+    mp = T.vacuum_params(damp=0.01)
+    f = H.random_full_fields(g, 8)
+    fa, fb = f.copy(), f.copy()
+    orc.vacuum_advance_e(fa, g, mp)
+    ref.vacuum_advance_e_bulk(fb, g, mp)
+    m = _interior_mask(g, 2)
+    for c in ("ex", "ey", "ez", "tcax", "tcay", "tcaz"):
+        assert H.same_bits(fa[c][m], fb[c][m]), c
+
+
+def test_sort_p(orc, ref):
+    g = _box(6, 5, 4)
+    p = H.thermal_particles(g, 5, 0.1, 9, shuffle=True)
+    pa, pb = p.copy(), p.copy()
+    qa = orc.sort_p(pa, g)
+    qb = ref.sort_p(pb, g)
+    assert H.same_bits(pa, pb)
+    assert H.same_bits(qa[:g.nv], qb[:g.nv])
+    # stable: equals numpy's stable sort by voxel
+    order = np.argsort(p["i"], kind="stable")
+    assert H.same_bits(pa, p[order])
+
+
+def test_energy_p(orc, ref):
+    g = _box(6, 5, 4)
+    p, fi = _push_inputs(g, 7, 0.1, 12)
+    ea = orc.energy_p(p, -1.0, 1.0, fi, g)
+    eb = ref.energy_p(p, -1.0, 1.0, fi, g)
+    assert ea == eb
+
+
+def test_energy_f(orc, ref):
+    g = _box(6, 5, 4)
+    f = H.random_full_fields(g, 13)
+    mp = T.vacuum_params()
+    assert np.array_equal(orc.energy_f(f, g, mp), ref.energy_f(f, g, mp))
