@@ -149,8 +149,11 @@ int vpic_hip_advance_p(vpic_particle_t *p, int32_t np, float q, float m,
                        vpic_accumulator_t *a, const vpic_interpolator_t *f,
                        const vpic_grid_t *g, void *stream);
 
-/* Stream-ordered form of vpic_hip_advance_p: writes {nm, n_ignored} to the
- * device int pair `dev_nm` instead of the host and does not synchronise. */
+/* Stream-ordered form of vpic_hip_advance_p: writes {nm, n_ignored,
+ * n_bad} to the three device ints at `dev_nm` instead of the host and does
+ * not synchronise.  n_bad counts particles whose voxel index lies outside
+ * [0,nv); they are left untouched (the synchronous form reports them as an
+ * error). */
 int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
                              float m, vpic_particle_mover_t *pm,
                              int32_t max_nm, int32_t *dev_nm,

@@ -1,0 +1,329 @@
+"""GPU parity of the HIP path (through the C-ABI) against the oracle.
+
+Bars (from the path's specification):
+  * cell indices, cell-crosser counts, movers: bit-exact
+  * FP32 positions and momenta: <= 2 ulp per step (the kernels reproduce the
+    reference's operation order, so the observed difference is 0 ulp and
+    the tests assert 0)
+  * accumulated currents: <= 1e-5 relative (max-norm, floor = max |ref|):
+    only the summation order of the deposits differs
+  * field stencils, interpolator load, jf unload/fold, sort permutation and
+    partition: bit-exact
+  * FP64 energies: <= 1e-12 relative (summation order)
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import host_grid
+from vpic_amd import _lib
+from vpic_amd import types as T
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+ACC_TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    _lib.call("vpic_hip_init", 0)
+    from vpic_amd import device
+    return device
+
+
+def _box(nx, ny, nz, cell=1.0, frac=0.95):
+    L = (nx * cell, ny * cell, nz * cell)
+    dt = frac * T.courant_length(L[0], L[1], L[2], nx, ny, nz)
+    return T.grid_periodic(0, 0, 0, L[0], L[1], L[2], nx, ny, nz, dt)
+
+
+def _inputs(orc, g, ppc, uth, seed, e_amp=0.05, b_amp=0.05, shuffle=False):
+    f = H.random_fields(g, seed, e_amp, b_amp)
+    fi = H.interp_array(g)
+    orc.load_interpolator(fi, f, g)
+    p = H.thermal_particles(g, ppc, uth, seed + 1, shuffle=shuffle)
+    return p, fi
+
+
+def _gpu_advance_p(dev, p, q, m, fi, g, max_nm=None, acc=None):
+    if max_nm is None:
+        max_nm = max(len(p), 16)
+    dp = dev.DeviceArray.from_host(p)
+    dfi = dev.DeviceArray.from_host(fi)
+    da = dev.DeviceArray.from_host(acc if acc is not None else H.acc_array(g))
+    dpm = dev.movers(max_nm)
+    nm = ctypes.c_int32()
+    ign = ctypes.c_int32()
+    _lib.call("vpic_hip_advance_p", dp.ptr, len(p), q, m, dpm.ptr, max_nm,
+              ctypes.byref(nm), ctypes.byref(ign), da.ptr, dfi.ptr,
+              ctypes.byref(g), None)
+    pm = dpm.download()[:nm.value].copy()
+    return dp.download(), da.download(), nm.value, ign.value, pm
+
+
+CASES = [
+    ("electrons uth=0.1", -1.0, 1.0, 0.1, 8, (8, 8, 8)),
+    ("ions mi/me=25 uth=0.02", 1.0, 25.0, 0.02, 8, (8, 8, 8)),
+    ("hot uth=0.5, heavy crossing", -1.0, 1.0, 0.5, 13, (8, 8, 8)),
+    ("ragged np, 1-cell-thick box", -1.0, 1.0, 0.3, 7, (5, 1, 3)),
+]
+
+
+@pytest.mark.parametrize("name,q,m,uth,ppc,n", CASES, ids=[c[0] for c in CASES])
+def test_advance_p_parity(dev, orc, name, q, m, uth, ppc, n):
+    g = _box(*n)
+    p, fi = _inputs(orc, g, ppc, uth, 21)
+    hg = host_grid(g)
+    po, ao = p.copy(), H.acc_array(g)
+    nmo, igo, pmo = orc.advance_p(po, q, m, ao, fi, hg)
+    say("advance_p case:", name, "np =", len(p))
+    pg, ag, nmg, igg, pmg = _gpu_advance_p(dev, p, q, m, fi, g)
+    assert np.array_equal(pg["i"], po["i"])
+    assert H.particle_ulp_diff(pg, po) == 0
+    assert (nmg, igg) == (nmo, igo) == (0, 0)
+    assert H.acc_rel_err(ag, ao) <= ACC_TOL
+    assert np.any(po["i"] != p["i"])  # crossers exercised
+
+
+def test_advance_p_unsorted_and_np_edges(dev, orc):
+    g = _box(6, 5, 4)
+    p, fi = _inputs(orc, g, 9, 0.2, 5, shuffle=True)
+    hg = host_grid(g)
+    for npart in (0, 1, 63, 64, 65, 257, len(p)):
+        sub = p[:npart].copy()
+        po, ao = sub.copy(), H.acc_array(g)
+        orc.advance_p(po, -1.0, 1.0, ao, fi, hg)
+        say("advance_p np =", npart)
+        pg, ag, nmg, _, _ = _gpu_advance_p(dev, sub, -1.0, 1.0, fi, g)
+        assert H.same_bits(pg, po)
+        assert nmg == 0
+        assert H.acc_rel_err(ag, ao) <= ACC_TOL or npart == 0
+
+
+def test_advance_p_walls_movers(dev, orc):
+    """Reflecting x walls, absorbing z walls (movers returned in particle
+    order), and mover-capacity overflow with the p->i >>= 3 rollback."""
+    g = _box(4, 5, 6)
+    g.pbc[0] = g.pbc[3] = T.BC_REFLECT_PARTICLES
+    g.pbc[2] = g.pbc[5] = T.BC_ABSORB_PARTICLES
+    p, fi = _inputs(orc, g, 7, 0.9, 5, e_amp=0.2, b_amp=0.2)
+    hg = host_grid(g)
+    po, ao = p.copy(), H.acc_array(g)
+    nmo, igo, pmo = orc.advance_p(po, -1.0, 1.0, ao, fi, hg, max_nm=len(p))
+    assert nmo > 0
+    say("advance_p walls/movers, nm =", nmo)
+    pg, ag, nmg, igg, pmg = _gpu_advance_p(dev, p, -1.0, 1.0, fi, g,
+                                           max_nm=len(p))
+    assert (nmg, igg) == (nmo, 0)
+    assert H.same_bits(pmg, pmo)
+    assert H.same_bits(pg, po)
+    assert H.acc_rel_err(ag, ao) <= ACC_TOL
+    # overflow: keep the first max_nm movers in particle order, roll back
+    # the voxel index of the rest
+    cap = nmo // 3
+    say("advance_p mover overflow, max_nm =", cap)
+    pg, ag, nmg, igg, pmg = _gpu_advance_p(dev, p, -1.0, 1.0, fi, g,
+                                           max_nm=cap)
+    assert (nmg, igg) == (cap, nmo - cap)
+    assert H.same_bits(pmg, pmo[:cap])
+    expect = po.copy()
+    lost = pmo["i"][cap:]
+    expect["i"][lost] = expect["i"][lost] >> 3
+    assert H.same_bits(pg, expect)
+
+
+def test_advance_p_bad_voxel_reported(dev, orc):
+    g = _box(4, 4, 4)
+    p, fi = _inputs(orc, g, 4, 0.1, 8)
+    p["i"][10] = g.nv + 5
+    p["i"][11] = -3
+    say("advance_p with out-of-range voxel indices")
+    with pytest.raises(_lib.VpicError, match="outside"):
+        _gpu_advance_p(dev, p, -1.0, 1.0, fi, g)
+
+
+def test_move_p_api(dev, orc):
+    g = _box(4, 5, 6)
+    g.pbc[1] = g.pbc[4] = T.BC_REFLECT_PARTICLES
+    hg = host_grid(g)
+    p = H.thermal_particles(g, 2, 0.1, 8)
+    rng = np.random.default_rng(3)
+    mv = np.zeros(50, dtype=T.MOVER)
+    mv["i"] = rng.choice(len(p), 50, replace=False)
+    for c in ("dispx", "dispy", "dispz"):
+        mv[c] = rng.uniform(-3, 3, 50).astype(np.float32)
+    po, ao, mo = p.copy(), H.acc_array(g), mv.copy()
+    ret_o = []
+    for k in range(len(mo)):
+        one = mo[k:k + 1].copy()
+        ret_o.append(orc.move_p(po, one, ao, hg, -1.0))
+        mo[k] = one[0]
+    dp = dev.DeviceArray.from_host(p)
+    dm = dev.DeviceArray.from_host(mv)
+    da = dev.DeviceArray.from_host(H.acc_array(g))
+    dr = dev.DeviceArray((len(mv),), np.int32)
+    say("move_p API on", len(mv), "movers")
+    _lib.call("vpic_hip_move_p", dp.ptr, dm.ptr, len(mv), da.ptr,
+              ctypes.byref(g), -1.0, dr.ptr, None)
+    assert np.array_equal(dr.download(), np.array(ret_o, dtype=np.int32))
+    assert H.same_bits(dp.download(), po)
+    assert H.same_bits(dm.download(), mo)
+    assert H.acc_rel_err(da.download(), ao) <= ACC_TOL
+
+
+def test_load_interpolator_bitexact(dev, orc):
+    g = _box(7, 5, 6)
+    f = H.random_full_fields(g, 2)
+    fo = H.interp_array(g)
+    orc.load_interpolator(fo, f, g)
+    df = dev.DeviceArray.from_host(f)
+    di = dev.DeviceArray.from_host(H.interp_array(g))
+    say("load_interpolator")
+    _lib.call("vpic_hip_load_interpolator_array", di.ptr, df.ptr,
+              ctypes.byref(g), None)
+    got = di.download()
+    for c in T.INTERP_FIELDS[:18]:
+        assert H.same_bits(got[c], fo[c]), c
+
+
+def _field_bc_grids():
+    g = _box(7, 5, 6)
+    yield "periodic", g
+    g2 = g.copy()
+    g2.fbc[0] = g2.fbc[3] = T.BC_PEC_FIELDS
+    g2.fbc[2] = g2.fbc[5] = T.BC_SYMMETRIC_FIELDS
+    yield "pec-x sym-z", g2
+    g3 = g.copy()
+    g3.fbc[1] = g3.fbc[4] = T.BC_PMC_FIELDS
+    yield "pmc-y", g3
+
+
+def test_field_kernels_bitexact(dev, orc):
+    mp = T.vacuum_params(damp=0.0)
+    mp2 = T.vacuum_params(damp=0.02)
+    for name, g in _field_bc_grids():
+        f = H.random_full_fields(g, 3)
+        acc = np.random.default_rng(1).standard_normal((g.nv, 12)).astype(
+            np.float32)
+        # the reference assumes zero ghost accumulators
+        v = np.arange(g.nv)
+        x, y, z = v % (g.nx + 2), (v // (g.nx + 2)) % (g.ny + 2), v // (
+            (g.nx + 2) * (g.ny + 2))
+        ghost = ((x == 0) | (x == g.nx + 1) | (y == 0) | (y == g.ny + 1) |
+                 (z == 0) | (z == g.nz + 1))
+        acc[ghost] = 0
+        fo = f.copy()
+        orc.clear_jf(fo, g)
+        orc.unload_accumulator(fo, acc, g)
+        orc.synchronize_jf(fo, g)
+        orc.advance_b(fo, g, 0.5)
+        orc.vacuum_advance_e(fo, g, mp)
+        orc.advance_b(fo, g, 0.5)
+        orc.vacuum_advance_e(fo, g, mp2)
+        df = dev.DeviceArray.from_host(f)
+        da = dev.DeviceArray.from_host(acc)
+        say("field kernels,", name)
+        G = ctypes.byref(g)
+        _lib.call("vpic_hip_clear_jf", df.ptr, G, None)
+        _lib.call("vpic_hip_unload_accumulator_array", df.ptr, da.ptr, G, None)
+        _lib.call("vpic_hip_synchronize_jf", df.ptr, G, None)
+        _lib.call("vpic_hip_advance_b", df.ptr, G, 0.5, None)
+        _lib.call("vpic_hip_vacuum_advance_e", df.ptr, G, ctypes.byref(mp),
+                  1.0, None)
+        _lib.call("vpic_hip_advance_b", df.ptr, G, 0.5, None)
+        _lib.call("vpic_hip_vacuum_advance_e", df.ptr, G, ctypes.byref(mp2),
+                  1.0, None)
+        got = df.download()
+        for c in T.FIELD_FLOATS:
+            assert H.same_bits(got[c], fo[c]), (name, c)
+        # energy_f
+        en = (ctypes.c_double * 6)()
+        _lib.call("vpic_hip_energy_f", en, df.ptr, G, ctypes.byref(mp), None)
+        eo = orc.energy_f(fo, g, mp)
+        assert np.allclose(np.array(en[:]), eo, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode", ["shuffled", "drifted", "one-hot-voxel"])
+def test_sort_p_bitexact(dev, orc, mode):
+    g = _box(6, 5, 4)
+    if mode == "shuffled":
+        p = H.thermal_particles(g, 5, 0.1, 9, shuffle=True)
+    elif mode == "drifted":
+        # sorted, then advanced a few steps so cells drift
+        p, fi = _inputs(orc, g, 40, 0.3, 4)
+        hg = host_grid(g)
+        for _ in range(3):
+            orc.advance_p(p, -1.0, 1.0, H.acc_array(g), fi, hg)
+    else:
+        # one voxel with more particles than one LDS segment holds
+        p = H.thermal_particles(g, 3, 0.1, 9, shuffle=True)
+        extra = H.thermal_particles(g, 1, 0.1, 10)[:1].repeat(3000)
+        extra["dx"] = np.linspace(-1, 1, 3000, dtype=np.float32)
+        p = np.concatenate([p, extra])
+        p = p[np.random.default_rng(2).permutation(len(p))]
+    po = p.copy()
+    part_o = orc.sort_p(po, g)
+    dp = dev.DeviceArray.from_host(p)
+    dx = dev.particles(len(p))
+    dpart = dev.DeviceArray((g.nv + 1,), np.int32)
+    say("sort_p", mode, "np =", len(p))
+    _lib.call("vpic_hip_sort_p", dp.ptr, dx.ptr, len(p), dpart.ptr,
+              ctypes.byref(g), 1, None)
+    assert H.same_bits(dp.download(), po)
+    assert H.same_bits(dpart.download(), part_o)
+
+
+def test_energy_p_and_rho(dev, orc):
+    g = _box(6, 5, 4)
+    p, fi = _inputs(orc, g, 7, 0.1, 12)
+    eo = orc.energy_p(p, -1.0, 1.0, fi, g)
+    dp = dev.DeviceArray.from_host(p)
+    dfi = dev.DeviceArray.from_host(fi)
+    en = ctypes.c_double()
+    say("energy_p")
+    _lib.call("vpic_hip_energy_p", dp.ptr, len(p), -1.0, 1.0, dfi.ptr,
+              ctypes.byref(g), ctypes.byref(en), None)
+    assert abs(en.value - eo) <= 1e-12 * abs(eo)
+    f = np.zeros(g.nv, dtype=T.FIELD)
+    fo = f.copy()
+    orc.accumulate_rho_p(fo, p, -1.0, g)
+    df = dev.DeviceArray.from_host(f)
+    say("accumulate_rho_p")
+    _lib.call("vpic_hip_accumulate_rho_p", df.ptr, dp.ptr, len(p), -1.0,
+              ctypes.byref(g), None)
+    got = df.download()["rhof"]
+    assert H.acc_rel_err(got, fo["rhof"]) <= 1e-6
+
+
+def test_loader_statistics(dev):
+    g = _box(8, 8, 8)
+    ppc = 64
+    dp = dev.particles(ppc * 512)
+    say("load_uniform_plasma")
+    _lib.call("vpic_hip_load_uniform_plasma", dp.ptr, ctypes.byref(g), ppc,
+              0.1, 0.1, 0.1, 0.0, 0.0, 0.05, 2.0, 1234, None)
+    p = dp.download()
+    assert np.all(np.abs(p["dx"]) <= 1) and np.all(np.abs(p["dz"]) <= 1)
+    cells = np.bincount(p["i"], minlength=g.nv)
+    x, y, z = H.voxel_coords(g, 512)
+    vox = x + (g.nx + 2) * (y + (g.ny + 2) * z)
+    assert np.all(cells[vox] == ppc)
+    assert abs(np.std(p["ux"]) - 0.1) < 0.003
+    assert abs(np.mean(p["uz"]) - 0.05) < 0.003
+    assert np.all(p["w"] == 2.0)
+    # reproducible
+    dq = dev.particles(ppc * 512)
+    _lib.call("vpic_hip_load_uniform_plasma", dq.ptr, ctypes.byref(g), ppc,
+              0.1, 0.1, 0.1, 0.0, 0.0, 0.05, 2.0, 1234, None)
+    assert H.same_bits(dq.download(), p)

@@ -1,0 +1,184 @@
+// common.h -- shared device/host definitions of libvpichip (gfx950).
+//
+// Compiled with -ffp-contract=off and IEEE-correct division and sqrt (the
+// HIP defaults), no fast-math: every FP32 expression of the particle push
+// and the field stencils rounds exactly as the reference's scalar build.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/vpic_hip.h"
+
+#define VPIC_WAVE 64
+
+// Voxel index, src/grid/grid.h:136.
+__host__ __device__ inline int vox(int x, int y, int z, int nx, int ny) {
+  return x + (nx + 2) * (y + (ny + 2) * z);
+}
+
+// Grid data every kernel needs, passed by value.
+struct GridDev {
+  int nx, ny, nz, sy, sz, nv;
+  int64_t rangel, rangeh;
+  const int64_t *neighbor;
+  int pbc[6];
+  int fbc[6];
+};
+
+inline GridDev make_grid_dev(const vpic_grid_t *g) {
+  GridDev d;
+  d.nx = g->nx;
+  d.ny = g->ny;
+  d.nz = g->nz;
+  d.sy = g->sy;
+  d.sz = g->sz;
+  d.nv = g->nv;
+  d.rangel = g->rangel;
+  d.rangeh = g->rangeh;
+  d.neighbor = g->neighbor;
+  for (int k = 0; k < 6; k++) {
+    d.pbc[k] = g->pbc[k];
+    d.fbc[k] = g->fbc[k];
+  }
+  return d;
+}
+
+// Neighbour of voxel v across face (order -x,-y,-z,+x,+y,+z) as the
+// reference's table holds it (src/grid/ops.cc:45-76 + join_grid/set_pbc):
+// the explicit table when given, else derived from the coordinates.
+__device__ inline int64_t grid_neighbor(const GridDev &G, int v, int face) {
+  if (G.neighbor) return G.neighbor[6 * (int64_t)v + face];
+  const int sx1 = G.nx + 2, sy1 = G.ny + 2;
+  int x = v % sx1;
+  int t = v / sx1;
+  int y = t % sy1;
+  int z = t / sy1;
+  if (x == 0 || x == G.nx + 1 || y == 0 || y == G.ny + 1 || z == 0 ||
+      z == G.nz + 1)
+    return VPIC_BC_REFLECT_PARTICLES;
+  switch (face) {
+    case 0:
+      if (x == 1)
+        return G.pbc[0] == VPIC_BC_PERIODIC ? G.rangel + (v + G.nx - 1)
+                                            : (int64_t)G.pbc[0];
+      return G.rangel + (v - 1);
+    case 1:
+      if (y == 1)
+        return G.pbc[1] == VPIC_BC_PERIODIC
+                   ? G.rangel + (v + (int64_t)(G.ny - 1) * G.sy)
+                   : (int64_t)G.pbc[1];
+      return G.rangel + (v - G.sy);
+    case 2:
+      if (z == 1)
+        return G.pbc[2] == VPIC_BC_PERIODIC
+                   ? G.rangel + (v + (int64_t)(G.nz - 1) * G.sz)
+                   : (int64_t)G.pbc[2];
+      return G.rangel + (v - G.sz);
+    case 3:
+      if (x == G.nx)
+        return G.pbc[3] == VPIC_BC_PERIODIC ? G.rangel + (v - (G.nx - 1))
+                                            : (int64_t)G.pbc[3];
+      return G.rangel + (v + 1);
+    case 4:
+      if (y == G.ny)
+        return G.pbc[4] == VPIC_BC_PERIODIC
+                   ? G.rangel + (v - (int64_t)(G.ny - 1) * G.sy)
+                   : (int64_t)G.pbc[4];
+      return G.rangel + (v + G.sy);
+    default:
+      if (z == G.nz)
+        return G.pbc[5] == VPIC_BC_PERIODIC
+                   ? G.rangel + (v - (int64_t)(G.nz - 1) * G.sz)
+                   : (int64_t)G.pbc[5];
+      return G.rangel + (v + G.sz);
+  }
+}
+
+// Hardware float atomic add (global_atomic_add_f32, no return).
+__device__ inline void atomic_add_f32(float *a, float v) {
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void atomic_add_f64(double *a, double v) {
+  __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// ---------------------------------------------------------------------------
+// Host-side helpers shared by the launchers (defined in api.hip)
+// ---------------------------------------------------------------------------
+
+// Record an error message for vpic_hip_last_error and return `code`.
+int vpic_set_error(int code, const char *fmt, ...);
+// Check a HIP status; on failure record it and return a negative code.
+int vpic_check_hip(hipError_t e, const char *what);
+
+#define VPIC_HIP(call)                                   \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return vpic_check_hip(e_, #call); \
+  } while (0)
+
+#define VPIC_LAUNCHED(name)                                   \
+  do {                                                        \
+    hipError_t e_ = hipGetLastError();                        \
+    if (e_ != hipSuccess) return vpic_check_hip(e_, name);    \
+  } while (0)
+
+enum {
+  VPIC_OK = 0,
+  VPIC_EINVAL = -1,
+  VPIC_EHIP = -2,
+  VPIC_ENOMEM = -3,
+  VPIC_EUNSUPPORTED = -4
+};
+
+// Per-device scratch kept by the library (grown on demand).
+struct DeviceScratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+int scratch_reserve(DeviceScratch &s, size_t bytes);
+
+// Launch-side kernels entry points implemented in the .hip files.
+int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
+                     vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
+                     vpic_accumulator_t *a, const vpic_interpolator_t *f,
+                     const vpic_grid_t *g, hipStream_t s);
+int launch_move_p_list(vpic_particle_t *p, vpic_particle_mover_t *pm, int nm,
+                       vpic_accumulator_t *a, const vpic_grid_t *g, float q,
+                       int *dev_ret, hipStream_t s);
+int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                  int *partition, const vpic_grid_t *g, int in_place,
+                  hipStream_t s);
+int launch_energy_p(const vpic_particle_t *p, int np, float q, float m,
+                    const vpic_interpolator_t *f, const vpic_grid_t *g,
+                    double *host_en, hipStream_t s);
+int launch_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
+                            float q, const vpic_grid_t *g, hipStream_t s);
+int launch_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
+                             const vpic_grid_t *g, hipStream_t s);
+int launch_clear_accumulator(vpic_accumulator_t *a, const vpic_grid_t *g,
+                             hipStream_t s);
+int launch_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
+                              const vpic_grid_t *g, hipStream_t s);
+int launch_clear_jf(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
+int launch_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
+int launch_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
+                          hipStream_t s);
+int launch_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac,
+                     hipStream_t s);
+int launch_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
+                            const vpic_field_params_t *mp, hipStream_t s);
+int launch_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
+                    const vpic_field_params_t *mp, hipStream_t s);
+int launch_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
+                               int ppc, float uthx, float uthy, float uthz,
+                               float udx, float udy, float udz, float w,
+                               uint64_t seed, hipStream_t s);
+
+// Push-kernel timing hooks (api.hip).
+bool push_timing_on();
+void push_timing_record(hipStream_t s, int np, bool begin);

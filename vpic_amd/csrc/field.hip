@@ -1,0 +1,619 @@
+// field.hip -- voxel kernels of the step on gfx950: load_interpolator,
+// clear/unload of the accumulator, clear_jf / synchronize_jf, advance_b,
+// vacuum_advance_e (with its tangential-B ghosts and local adjustments) and
+// the FP64 field energy.
+//
+// field_t (80 B) is read and written as five 16-B vectors:
+//   [0] ex ey ez div_e_err   [1] cbx cby cbz div_b_err
+//   [2] tcax tcay tcaz rhob  [3] jfx jfy jfz rhof   [4] material ids
+// One lane per voxel, consecutive lanes on consecutive x, so every vector
+// access of a wavefront is a contiguous 5 KiB span.
+//
+// Each stencil is the reference's expression with the same association, so
+// the results are bit-identical:
+//   load_interpolator  interpolator_array_pipeline.cc:66-121
+//   unload_accumulator unload_accumulator_pipeline.cc:65-67,137-139
+//   advance_b          advance_b_pipeline.h:26-28,57-59
+//   vacuum_advance_e   vacuum_advance_e_pipeline.h:27-33,56-69
+//   ghosts / adjusts   standard/local.cc:50-122,224-296,336-368,
+//                      standard/remote.cc:61-134,418-507
+//   energy_f           vacuum_energy_f_pipeline.h:17-75, .cc:67-96
+#include "common.h"
+
+namespace {
+
+struct Box {
+  int x0, y0, z0, nxr, nyr, nzr;
+};
+
+__device__ __forceinline__ bool box_coords(const Box &b, int64_t l, int &x,
+                                           int &y, int &z) {
+  if (l >= (int64_t)b.nxr * b.nyr * b.nzr) return false;
+  int t = (int)(l / b.nxr);
+  x = b.x0 + (int)(l - (int64_t)t * b.nxr);
+  z = t / b.nyr;
+  y = b.y0 + (t - z * b.nyr);
+  z += b.z0;
+  return true;
+}
+
+inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+__device__ __forceinline__ float4 *F4(vpic_field_t *f, int v) {
+  return reinterpret_cast<float4 *>(f + v);
+}
+__device__ __forceinline__ const float4 *F4(const vpic_field_t *f, int v) {
+  return reinterpret_cast<const float4 *>(f + v);
+}
+
+// ---------------------------------------------------------------------------
+// load_interpolator_array
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_load_interpolator(
+    vpic_interpolator_t *__restrict__ fi, const vpic_field_t *__restrict__ f,
+    Box b, int nx, int ny) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const float fourth = 0.25f, half = 0.50f;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  const float4 e0 = F4(f, v)[0], b0 = F4(f, v)[1];
+  const float4 ex_ = F4(f, v + 1)[0], bx_ = F4(f, v + 1)[1];
+  const float4 ey_ = F4(f, v + sy)[0], by_ = F4(f, v + sy)[1];
+  const float4 ez_ = F4(f, v + sz)[0], bz_ = F4(f, v + sz)[1];
+  const float e_yz = F4(f, v + sy + sz)[0].x;  // pfyz->ex
+  const float e_zx = F4(f, v + sz + 1)[0].y;   // pfzx->ey
+  const float e_xy = F4(f, v + 1 + sy)[0].z;   // pfxy->ez
+  float w0, w1, w2, w3;
+  float4 o0, o1, o2, o3;
+  w0 = e0.x; w1 = ey_.x; w2 = ez_.x; w3 = e_yz;
+  o0.x = fourth * ((w3 + w0) + (w1 + w2));
+  o0.y = fourth * ((w3 - w0) + (w1 - w2));
+  o0.z = fourth * ((w3 - w0) - (w1 - w2));
+  o0.w = fourth * ((w3 + w0) - (w1 + w2));
+  w0 = e0.y; w1 = ez_.y; w2 = ex_.y; w3 = e_zx;
+  o1.x = fourth * ((w3 + w0) + (w1 + w2));
+  o1.y = fourth * ((w3 - w0) + (w1 - w2));
+  o1.z = fourth * ((w3 - w0) - (w1 - w2));
+  o1.w = fourth * ((w3 + w0) - (w1 + w2));
+  w0 = e0.z; w1 = ex_.z; w2 = ey_.z; w3 = e_xy;
+  o2.x = fourth * ((w3 + w0) + (w1 + w2));
+  o2.y = fourth * ((w3 - w0) + (w1 - w2));
+  o2.z = fourth * ((w3 - w0) - (w1 - w2));
+  o2.w = fourth * ((w3 + w0) - (w1 + w2));
+  w0 = b0.x; w1 = bx_.x;
+  o3.x = half * (w1 + w0);
+  o3.y = half * (w1 - w0);
+  w0 = b0.y; w1 = by_.y;
+  o3.z = half * (w1 + w0);
+  o3.w = half * (w1 - w0);
+  w0 = b0.z; w1 = bz_.z;
+  float4 *out = reinterpret_cast<float4 *>(fi + v);
+  out[0] = o0;
+  out[1] = o1;
+  out[2] = o2;
+  out[3] = o3;
+  reinterpret_cast<float2 *>(out + 4)[0] =
+      make_float2(half * (w1 + w0), half * (w1 - w0));
+}
+
+// ---------------------------------------------------------------------------
+// accumulators and jf
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill_zero4(float4 *a, int64_t n4) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n4;
+       k += (int64_t)gridDim.x * blockDim.x)
+    a[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_clear_jf(vpic_field_t *f, int nv,
+                                                  int rhof) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += gridDim.x * blockDim.x) {
+    if (rhof) {
+      f[v].rhof = 0.0f;
+    } else {
+      f[v].jfx = 0.0f;
+      f[v].jfy = 0.0f;
+      f[v].jfz = 0.0f;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_unload_accumulator(
+    vpic_field_t *__restrict__ f, const vpic_accumulator_t *__restrict__ a,
+    Box b, int nx, int ny, float cx, float cy, float cz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  const vpic_accumulator_t *a0 = a + v, *ax = a + v - 1, *ay = a + v - sy,
+                           *az = a + v - sz, *ayz = a + v - sy - sz,
+                           *azx = a + v - 1 - sz, *axy = a + v - 1 - sy;
+  float4 jf = F4(f, v)[3];
+  jf.x += cx * (a0->jx[0] + ay->jx[1] + az->jx[2] + ayz->jx[3]);
+  jf.y += cy * (a0->jy[0] + az->jy[1] + ax->jy[2] + azx->jy[3]);
+  jf.z += cz * (a0->jz[0] + ax->jz[1] + ay->jz[2] + axy->jz[3]);
+  F4(f, v)[3] = jf;
+}
+
+// Rotated face addressing: normal axis X, tangential Y=(X+1)%3, Z=(X+2)%3.
+struct FaceGeom {
+  int n[3];  // nx, ny, nz
+  int s[3];  // strides 1, sy, sz
+};
+
+__device__ __forceinline__ int face_vox(const FaceGeom &G, int X, int plane,
+                                        int yy, int zz) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  return plane * G.s[X] + yy * G.s[Y] + zz * G.s[Z];
+}
+
+__device__ __forceinline__ float &fcomp(vpic_field_t *f, int v, int block,
+                                        int axis) {
+  return reinterpret_cast<float *>(f + v)[4 * block + axis];
+}
+
+// local_adjust_jf (local.cc:336-368) on one non-periodic face k (order
+// -x,-y,-z,+x,+y,+z).  Faces are launched one after another, as the
+// reference applies them, because edges shared by two faces are scaled by
+// both.  mode: 1 = zero (anti-symmetric), 2 = double (the other local BCs).
+__global__ void k_adjust_jf(vpic_field_t *f, FaceGeom G, int k, int mode) {
+  const int X = k % 3, Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int plane = k >= 3 ? G.n[X] + 1 : 1;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    const int v = face_vox(G, X, plane, yy, zz);
+    if (yy <= G.n[Y]) {  // Y##Z## edges: Y in 1..nY, Z in 1..nZ+1
+      float &j = fcomp(f, v, 3, Y);
+      if (mode == 1) j = 0; else j *= 2.f;
+    }
+    if (zz <= G.n[Z]) {  // Z##Y## edges: Y in 1..nY+1, Z in 1..nZ
+      float &j = fcomp(f, v, 3, Z);
+      if (mode == 1) j = 0; else j *= 2.f;
+    }
+  }
+}
+
+// One axis of the periodic jf fold (remote.cc:474-502 with lw = rw = 1).
+__global__ void k_fold_jf(vpic_field_t *f, FaceGeom G, int X) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const float lw = 1.0f, rw = 1.0f;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    const int lo = face_vox(G, X, 1, yy, zz);
+    const int hi = face_vox(G, X, G.n[X] + 1, yy, zz);
+    if (yy <= G.n[Y]) {
+      float &a = fcomp(f, lo, 3, Y), &b = fcomp(f, hi, 3, Y);
+      const float sa = a, sb = b;
+      b = lw * sb + rw * sa;
+      a = lw * sa + rw * sb;
+    }
+    if (zz <= G.n[Z]) {
+      float &a = fcomp(f, lo, 3, Z), &b = fcomp(f, hi, 3, Z);
+      const float sa = a, sb = b;
+      b = lw * sb + rw * sa;
+      a = lw * sa + rw * sb;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// advance_b
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advance_b(vpic_field_t *__restrict__ f,
+                                                   Box b, int nx, int ny,
+                                                   int nz, float px, float py,
+                                                   float pz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  const float4 e0 = F4(f, v)[0];
+  float4 b0 = F4(f, v)[1];
+  const bool ux = x <= nx, uy = y <= ny, uz = z <= nz;
+  // cbx on x in 1..nx+1 (y<=ny, z<=nz), cby on y in 1..ny+1, cbz on z in
+  // 1..nz+1; each neighbour is read only where its component is updated.
+  float4 ex1 = e0, ey1 = e0, ez1 = e0;
+  if (ux && (uy || uz)) ex1 = F4(f, v + 1)[0];
+  if (uy && (ux || uz)) ey1 = F4(f, v + sy)[0];
+  if (uz && (ux || uy)) ez1 = F4(f, v + sz)[0];
+  if (uy && uz) b0.x -= (py * (ey1.z - e0.z) - pz * (ez1.y - e0.y));
+  if (ux && uz) b0.y -= (pz * (ez1.x - e0.x) - px * (ex1.z - e0.z));
+  if (ux && uy) b0.z -= (px * (ex1.y - e0.y) - py * (ey1.x - e0.x));
+  if ((uy && uz) || (ux && uz) || (ux && uy)) F4(f, v)[1] = b0;
+}
+
+// local_adjust_norm_b (local.cc:266-296): symmetric faces zero cb normal.
+__global__ void k_adjust_norm_b(vpic_field_t *f, FaceGeom G, int4 m0,
+                                int2 m1) {
+  const int k = blockIdx.y;
+  const int mode = k == 0 ? m0.x : k == 1 ? m0.y : k == 2 ? m0.z
+                   : k == 3 ? m0.w : k == 4 ? m1.x : m1.y;
+  if (mode != VPIC_BC_SYMMETRIC_FIELDS) return;
+  const int X = k % 3, Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int plane = k >= 3 ? G.n[X] + 1 : 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < G.n[Y] * G.n[Z];
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % G.n[Y], zz = 1 + l / G.n[Y];
+    fcomp(f, face_vox(G, X, plane, yy, zz), 1, X) = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vacuum_advance_e
+// ---------------------------------------------------------------------------
+
+// Tangential-B ghosts of one normal axis X (both sides): periodic partners
+// take the opposite interior face with rw = 1, lw = 0 (remote.cc:105-116 for
+// equal cell sizes); local faces copy (anti-symmetric) or negate
+// (symmetric/pmc) the first interior plane (local.cc:67-81).
+__global__ void k_ghost_tang_b(vpic_field_t *f, FaceGeom G, int X, int mlo,
+                               int mhi) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int nX = G.n[X];
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const float rw = 1.0f, lw = 0.0f;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    const int g0 = face_vox(G, X, 0, yy, zz);
+    const int g1 = face_vox(G, X, nX + 1, yy, zz);
+    const int i1 = face_vox(G, X, 1, yy, zz);
+    const int iN = face_vox(G, X, nX, yy, zz);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int comp = c == 0 ? Y : Z;
+      // cb_Y on (Y in 1..nY+1, Z in 1..nZ), cb_Z on (Y in 1..nY, Z in 1..nZ+1)
+      if (c == 0 ? (zz > G.n[Z]) : (yy > G.n[Y])) continue;
+      const float a1 = fcomp(f, i1, 1, comp), aN = fcomp(f, iN, 1, comp);
+      // low ghost (plane 0)
+      if (mlo == VPIC_BC_PERIODIC)
+        fcomp(f, g0, 1, comp) = rw * aN + lw * a1;
+      else if (mlo == VPIC_BC_ANTI_SYMMETRIC_FIELDS)
+        fcomp(f, g0, 1, comp) = a1;
+      else
+        fcomp(f, g0, 1, comp) = -a1;
+      // high ghost (plane nX+1)
+      if (mhi == VPIC_BC_PERIODIC)
+        fcomp(f, g1, 1, comp) = rw * a1 + lw * aN;
+      else if (mhi == VPIC_BC_ANTI_SYMMETRIC_FIELDS)
+        fcomp(f, g1, 1, comp) = aN;
+      else
+        fcomp(f, g1, 1, comp) = -aN;
+    }
+  }
+}
+
+struct ECoef {
+  float decayx, drivex, decayy, drivey, decayz, drivez, damp;
+  float px_muz, px_muy, py_mux, py_muz, pz_muy, pz_mux, cj;
+};
+
+__global__ __launch_bounds__(256) void k_advance_e(vpic_field_t *__restrict__ f,
+                                                   Box b, int nx, int ny,
+                                                   int nz, ECoef c) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  float4 e0 = F4(f, v)[0];
+  const float4 b0 = F4(f, v)[1];
+  float4 t0 = F4(f, v)[2];
+  const float4 j0 = F4(f, v)[3];
+  const float4 bx = F4(f, v - 1)[1], by = F4(f, v - sy)[1],
+               bz = F4(f, v - sz)[1];
+  if (x <= nx) {
+    t0.x = (c.py_muz * (b0.z - by.z) - c.pz_muy * (b0.y - bz.y)) -
+           c.damp * t0.x;
+    e0.x = c.decayx * e0.x + c.drivex * (t0.x - c.cj * j0.x);
+  }
+  if (y <= ny) {
+    t0.y = (c.pz_mux * (b0.x - bz.x) - c.px_muz * (b0.z - bx.z)) -
+           c.damp * t0.y;
+    e0.y = c.decayy * e0.y + c.drivey * (t0.y - c.cj * j0.y);
+  }
+  if (z <= nz) {
+    t0.z = (c.px_muy * (b0.y - bx.y) - c.py_mux * (b0.x - by.x)) -
+           c.damp * t0.z;
+    e0.z = c.decayz * e0.z + c.drivez * (t0.z - c.cj * j0.z);
+  }
+  F4(f, v)[0] = e0;
+  F4(f, v)[2] = t0;
+}
+
+// local_adjust_tang_e (local.cc:224-264): PEC faces zero e and tca tangential.
+__global__ void k_adjust_tang_e(vpic_field_t *f, FaceGeom G, int4 m0,
+                                int2 m1) {
+  const int k = blockIdx.y;
+  const int mode = k == 0 ? m0.x : k == 1 ? m0.y : k == 2 ? m0.z
+                   : k == 3 ? m0.w : k == 4 ? m1.x : m1.y;
+  if (mode != VPIC_BC_ANTI_SYMMETRIC_FIELDS) return;
+  const int X = k % 3, Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int plane = k >= 3 ? G.n[X] + 1 : 1;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    const int v = face_vox(G, X, plane, yy, zz);
+    if (yy <= G.n[Y]) {
+      fcomp(f, v, 0, Y) = 0;
+      fcomp(f, v, 2, Y) = 0;
+    }
+    if (zz <= G.n[Z]) {
+      fcomp(f, v, 0, Z) = 0;
+      fcomp(f, v, 2, Z) = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// energy_f: per-voxel FP32 terms, FP64 sums (per-block partials, then one
+// block folds them in a fixed order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_energy_f(const vpic_field_t *__restrict__ f,
+                                                  Box b, int nx, int ny,
+                                                  float4 q, float4 h,
+                                                  double *__restrict__ part) {
+  __shared__ double red[6][256];
+  int x, y, z;
+  double en[6] = {0, 0, 0, 0, 0, 0};
+  if (box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z)) {
+    const int v = vox(x, y, z, nx, ny);
+    const int sy = nx + 2, sz = (ny + 2) * sy;
+    const float4 e0 = F4(f, v)[0], b0 = F4(f, v)[1];
+    const float4 ex_ = F4(f, v + 1)[0], bx_ = F4(f, v + 1)[1];
+    const float4 ey_ = F4(f, v + sy)[0], by_ = F4(f, v + sy)[1];
+    const float4 ez_ = F4(f, v + sz)[0], bz_ = F4(f, v + sz)[1];
+    const float eyz = F4(f, v + sy + sz)[0].x;
+    const float ezx = F4(f, v + sz + 1)[0].y;
+    const float exy = F4(f, v + 1 + sy)[0].z;
+    en[0] = (double)(q.x * (e0.x * e0.x + ey_.x * ey_.x + ez_.x * ez_.x +
+                            eyz * eyz));
+    en[1] = (double)(q.y * (e0.y * e0.y + ez_.y * ez_.y + ex_.y * ex_.y +
+                            ezx * ezx));
+    en[2] = (double)(q.z * (e0.z * e0.z + ex_.z * ex_.z + ey_.z * ey_.z +
+                            exy * exy));
+    en[3] = (double)(h.x * (b0.x * b0.x + bx_.x * bx_.x));
+    en[4] = (double)(h.y * (b0.y * b0.y + by_.y * by_.y));
+    en[5] = (double)(h.z * (b0.z * b0.z + bz_.z * bz_.z));
+  }
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = en[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; k++)
+        red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * (int64_t)blockIdx.x + threadIdx.x] =
+      red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_fold_partials(const double *part,
+                                                       int nblk, int width,
+                                                       double *out) {
+  __shared__ double red[256];
+  for (int k = 0; k < width; k++) {
+    double s = 0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+      s += part[(int64_t)width * i + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
+}
+
+FaceGeom face_geom(const vpic_grid_t *g) {
+  FaceGeom G;
+  G.n[0] = g->nx;
+  G.n[1] = g->ny;
+  G.n[2] = g->nz;
+  G.s[0] = 1;
+  G.s[1] = g->sy;
+  G.s[2] = g->sz;
+  return G;
+}
+
+int face_blocks(const vpic_grid_t *g) {
+  int m = g->nx > g->ny ? g->nx : g->ny;
+  m = m > g->nz ? m : g->nz;
+  return blocks_for((int64_t)(m + 1) * (m + 1), 256) > 1024
+             ? 1024
+             : blocks_for((int64_t)(m + 1) * (m + 1), 256);
+}
+
+struct EnergyScratch {
+  DeviceScratch part, out;
+};
+EnergyScratch &energy_scratch() {
+  static thread_local EnergyScratch s[16];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return s[dev & 15];
+}
+
+}  // namespace
+
+int launch_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
+                             const vpic_grid_t *g, hipStream_t s) {
+  Box b = {1, 1, 1, g->nx, g->ny, g->nz};
+  int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  hipLaunchKernelGGL(k_load_interpolator, dim3(blocks_for(n, 256)), dim3(256),
+                     0, s, fi, f, b, g->nx, g->ny);
+  VPIC_LAUNCHED("k_load_interpolator");
+  return VPIC_OK;
+}
+
+int launch_clear_accumulator(vpic_accumulator_t *a, const vpic_grid_t *g,
+                             hipStream_t s) {
+  int64_t n4 = (int64_t)g->nv * 3;
+  int blk = blocks_for(n4, 256);
+  if (blk > 65536) blk = 65536;
+  hipLaunchKernelGGL(k_fill_zero4, dim3(blk), dim3(256), 0, s, (float4 *)a,
+                     n4);
+  VPIC_LAUNCHED("k_fill_zero4");
+  return VPIC_OK;
+}
+
+int launch_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
+                              const vpic_grid_t *g, hipStream_t s) {
+  Box b = {1, 1, 1, g->nx + 1, g->ny + 1, g->nz + 1};
+  int64_t n = (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+  // unload_accumulator_pipeline.cc:137-139: double, stored as float.
+  const float cx = 0.25 * g->rdy * g->rdz / g->dt;
+  const float cy = 0.25 * g->rdz * g->rdx / g->dt;
+  const float cz = 0.25 * g->rdx * g->rdy / g->dt;
+  hipLaunchKernelGGL(k_unload_accumulator, dim3(blocks_for(n, 256)),
+                     dim3(256), 0, s, f, a, b, g->nx, g->ny, cx, cy, cz);
+  VPIC_LAUNCHED("k_unload_accumulator");
+  return VPIC_OK;
+}
+
+int launch_clear_jf(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s) {
+  int blk = blocks_for(g->nv, 256);
+  if (blk > 65536) blk = 65536;
+  hipLaunchKernelGGL(k_clear_jf, dim3(blk), dim3(256), 0, s, f, g->nv, 0);
+  VPIC_LAUNCHED("k_clear_jf");
+  return VPIC_OK;
+}
+
+int launch_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s) {
+  int blk = blocks_for(g->nv, 256);
+  if (blk > 65536) blk = 65536;
+  hipLaunchKernelGGL(k_clear_jf, dim3(blk), dim3(256), 0, s, f, g->nv, 1);
+  VPIC_LAUNCHED("k_clear_jf");
+  return VPIC_OK;
+}
+
+static void face_modes(const vpic_grid_t *g, int4 &m0, int2 &m1) {
+  m0 = make_int4(g->fbc[0], g->fbc[1], g->fbc[2], g->fbc[3]);
+  m1 = make_int2(g->fbc[4], g->fbc[5]);
+}
+
+int launch_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
+                          hipStream_t s) {
+  FaceGeom G = face_geom(g);
+  const int fb = face_blocks(g);
+  for (int k = 0; k < 6; k++) {
+    if (g->fbc[k] == VPIC_BC_PERIODIC) continue;
+    const int mode = g->fbc[k] == VPIC_BC_ANTI_SYMMETRIC_FIELDS ? 1 : 2;
+    hipLaunchKernelGGL(k_adjust_jf, dim3(fb), dim3(256), 0, s, f, G, k, mode);
+    VPIC_LAUNCHED("k_adjust_jf");
+  }
+  for (int X = 0; X < 3; X++) {
+    if (g->fbc[X] != VPIC_BC_PERIODIC) continue;
+    hipLaunchKernelGGL(k_fold_jf, dim3(fb), dim3(256), 0, s, f, G, X);
+    VPIC_LAUNCHED("k_fold_jf");
+  }
+  return VPIC_OK;
+}
+
+int launch_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac,
+                     hipStream_t s) {
+  // advance_b_pipeline.h:26-28, in float.
+  const float px = (g->nx > 1) ? frac * g->cvac * g->dt * g->rdx : 0;
+  const float py = (g->ny > 1) ? frac * g->cvac * g->dt * g->rdy : 0;
+  const float pz = (g->nz > 1) ? frac * g->cvac * g->dt * g->rdz : 0;
+  Box b = {1, 1, 1, g->nx + 1, g->ny + 1, g->nz + 1};
+  int64_t n = (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+  hipLaunchKernelGGL(k_advance_b, dim3(blocks_for(n, 256)), dim3(256), 0, s, f,
+                     b, g->nx, g->ny, g->nz, px, py, pz);
+  VPIC_LAUNCHED("k_advance_b");
+  bool sym = false;
+  for (int k = 0; k < 6; k++) sym |= g->fbc[k] == VPIC_BC_SYMMETRIC_FIELDS;
+  if (sym) {
+    int4 m0;
+    int2 m1;
+    face_modes(g, m0, m1);
+    hipLaunchKernelGGL(k_adjust_norm_b, dim3(face_blocks(g), 6), dim3(256), 0,
+                       s, f, face_geom(g), m0, m1);
+    VPIC_LAUNCHED("k_adjust_norm_b");
+  }
+  return VPIC_OK;
+}
+
+int launch_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
+                            const vpic_field_params_t *m, hipStream_t s) {
+  FaceGeom G = face_geom(g);
+  const int fb = face_blocks(g);
+  for (int X = 0; X < 3; X++) {
+    hipLaunchKernelGGL(k_ghost_tang_b, dim3(fb), dim3(256), 0, s, f, G, X,
+                       g->fbc[X], g->fbc[X + 3]);
+    VPIC_LAUNCHED("k_ghost_tang_b");
+  }
+  // vacuum_advance_e_pipeline.h:23-33, in float.
+  ECoef c;
+  c.decayx = m->decayx;
+  c.drivex = m->drivex;
+  c.decayy = m->decayy;
+  c.drivey = m->drivey;
+  c.decayz = m->decayz;
+  c.drivez = m->drivez;
+  c.damp = m->damp;
+  const float damp = m->damp;
+  c.px_muz = ((g->nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuz;
+  c.px_muy = ((g->nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuy;
+  c.py_mux = ((g->ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmux;
+  c.py_muz = ((g->ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmuz;
+  c.pz_muy = ((g->nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmuy;
+  c.pz_mux = ((g->nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmux;
+  c.cj = g->dt / g->eps0;
+  Box b = {1, 1, 1, g->nx + 1, g->ny + 1, g->nz + 1};
+  int64_t n = (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+  hipLaunchKernelGGL(k_advance_e, dim3(blocks_for(n, 256)), dim3(256), 0, s, f,
+                     b, g->nx, g->ny, g->nz, c);
+  VPIC_LAUNCHED("k_advance_e");
+  bool pec = false;
+  for (int k = 0; k < 6; k++) pec |= g->fbc[k] == VPIC_BC_ANTI_SYMMETRIC_FIELDS;
+  if (pec) {
+    int4 m0;
+    int2 m1;
+    face_modes(g, m0, m1);
+    hipLaunchKernelGGL(k_adjust_tang_e, dim3(fb, 6), dim3(256), 0, s, f, G, m0,
+                       m1);
+    VPIC_LAUNCHED("k_adjust_tang_e");
+  }
+  return VPIC_OK;
+}
+
+int launch_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
+                    const vpic_field_params_t *m, hipStream_t s) {
+  EnergyScratch &S = energy_scratch();
+  Box b = {1, 1, 1, g->nx, g->ny, g->nz};
+  int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  int nblk = blocks_for(n, 256);
+  int rc;
+  if ((rc = scratch_reserve(S.part, sizeof(double) * 6 * (size_t)nblk)) ||
+      (rc = scratch_reserve(S.out, sizeof(double) * 8)))
+    return rc;
+  // vacuum_energy_f_pipeline.h:24-29: float coefficients.
+  float4 q = make_float4(0.25 * m->epsx, 0.25 * m->epsy, 0.25 * m->epsz, 0);
+  float4 h = make_float4(0.50 * m->rmux, 0.50 * m->rmuy, 0.50 * m->rmuz, 0);
+  hipLaunchKernelGGL(k_energy_f, dim3(nblk), dim3(256), 0, s, f, b, g->nx,
+                     g->ny, q, h, (double *)S.part.ptr);
+  VPIC_LAUNCHED("k_energy_f");
+  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(256), 0, s,
+                     (const double *)S.part.ptr, nblk, 6, (double *)S.out.ptr);
+  VPIC_LAUNCHED("k_fold_partials");
+  double out[6];
+  VPIC_HIP(hipMemcpyAsync(out, S.out.ptr, sizeof(out), hipMemcpyDeviceToHost,
+                          s));
+  VPIC_HIP(hipStreamSynchronize(s));
+  const double v0 = 0.5 * g->eps0 * g->dV;
+  for (int k = 0; k < 6; k++) en[k] = out[k] * v0;
+  return VPIC_OK;
+}

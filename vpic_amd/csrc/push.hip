@@ -1,0 +1,467 @@
+// push.hip -- advance_p on gfx950: Boris push with energy-conserving
+// interpolation, in-cell Villasenor-Buneman deposit, and move_p for the
+// particles that leave their cell.
+//
+// Reference: src/species_advance/standard/pipeline/advance_p_pipeline.cc
+// (scalar kernel :21-245, driver :253-340) and
+// src/species_advance/standard/move_p.cc:215-378.  The FP32 arithmetic of
+// each particle is the reference's, operation for operation, so positions,
+// momenta and cell indices are bit-identical; only the order in which the
+// current contributions are summed differs (atomics).
+//
+// Launch structure of one advance_p:
+//   k_clear_counters   zero the crosser / mover counters (no memset, so the
+//                      sequence can be captured in a graph)
+//   k_advance_p        one lane per particle.  In-cell particles deposit;
+//                      lanes of a wavefront that hold the same voxel are
+//                      summed with a segmented wavefront scan and only the
+//                      last lane of each run issues the 12 atomics.  Cell
+//                      crossers store their new momentum and are appended to
+//                      a compact crosser list (ballot + one atomic per wave).
+//   k_move_p_list      one lane per crosser: the move_p face walk.
+//   k_order_movers     movers the grid cannot handle, sorted by particle
+//                      index, truncated at max_nm (excess rolled back to the
+//                      voxel index, advance_p_pipeline.cc:228-236).
+#include "common.h"
+
+namespace {
+
+struct PushCoef {
+  float qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
+};
+
+// The twelve accumulator updates of advance_p_pipeline.cc:186-208 /
+// move_p.cc:281-303 for one axis: out[0..3] receive the four quarter-face
+// contributions.
+__device__ __forceinline__ void streak4(float out[4], float qd, float mY,
+                                        float mZ, float v5) {
+  float v0, v1, v2, v3, v4;
+  v4 = qd;
+  v1 = v4 * mY;
+  v0 = v4 - v1;
+  v1 += v4;
+  v4 = 1.0f + mZ;
+  v2 = v0 * v4;
+  v3 = v1 * v4;
+  v4 = 1.0f - mZ;
+  v0 *= v4;
+  v1 *= v4;
+  v0 += v5;
+  v1 -= v5;
+  v2 -= v5;
+  v3 += v5;
+  out[0] = v0;
+  out[1] = v1;
+  out[2] = v2;
+  out[3] = v3;
+}
+
+__device__ __forceinline__ void deposit_atomic(float *acc, int voxel,
+                                               const float j[12]) {
+  float *a = acc + 12 * (int64_t)voxel;
+#pragma unroll
+  for (int k = 0; k < 12; k++) atomic_add_f32(a + k, j[k]);
+}
+
+// move_p (move_p.cc:215-378).  The particle lives in registers for the
+// walk; `mv` is the mover (remaining displacement, particle index).
+// Returns 1 when the particle hit a face the grid cannot handle; its voxel
+// index is then 8*voxel + face.
+__device__ int dev_move_p(vpic_particle_t *__restrict__ p0,
+                          vpic_particle_mover_t &mv, float *__restrict__ acc,
+                          const GridDev &G, float qsp) {
+  float4 *pp = reinterpret_cast<float4 *>(p0 + mv.i);
+  float4 A = pp[0], B = pp[1];
+  float pd[3] = {A.x, A.y, A.z};
+  int pi = __float_as_int(A.w);
+  float pu[3] = {B.x, B.y, B.z};
+  float md[3] = {mv.dispx, mv.dispy, mv.dispz};
+  const float q = qsp * B.w;
+  int ret = 0;
+  for (;;) {
+    float s_mid[3] = {pd[0], pd[1], pd[2]};
+    float s_disp[3] = {md[0], md[1], md[2]};
+    float s_dir[3], v[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_dir[k] = (s_disp[k] > 0.0f) ? 1.0f : -1.0f;
+      v[k] = (s_disp[k] == 0.0f) ? 3.4e38f : (s_dir[k] - s_mid[k]) / s_disp[k];
+    }
+    float v3 = 2.0f;
+    int axis = 3;
+    if (v[0] < v3) v3 = v[0], axis = 0;
+    if (v[1] < v3) v3 = v[1], axis = 1;
+    if (v[2] < v3) v3 = v[2], axis = 2;
+    v3 *= 0.5f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_disp[k] *= v3;
+      s_mid[k] += s_disp[k];
+    }
+    // move_p.cc:277: promoted to double by the 1.0/3.0 constant.
+    const float v5 =
+        (float)((double)(q * s_disp[0] * s_disp[1] * s_disp[2]) * (1.0 / 3.0));
+    float j[12];
+    streak4(j + 0, q * s_disp[0], s_mid[1], s_mid[2], v5);
+    streak4(j + 4, q * s_disp[1], s_mid[2], s_mid[0], v5);
+    streak4(j + 8, q * s_disp[2], s_mid[0], s_mid[1], v5);
+    deposit_atomic(acc, pi, j);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      md[k] -= s_disp[k];
+      pd[k] += s_disp[k] + s_disp[k];
+    }
+    if (axis == 3) break;
+    const float d0 = s_dir[axis];
+    pd[axis] = d0;
+    const int face = axis + ((d0 > 0.0f) ? 3 : 0);
+    const int64_t nb = grid_neighbor(G, pi, face);
+    if (nb == VPIC_BC_REFLECT_PARTICLES) {
+      pu[axis] = -pu[axis];
+      md[axis] = -md[axis];
+      continue;
+    }
+    if (nb < G.rangel || nb > G.rangeh) {
+      pi = 8 * pi + face;
+      ret = 1;
+      break;
+    }
+    pi = (int)(nb - G.rangel);
+    pd[axis] = -d0;
+  }
+  pp[0] = make_float4(pd[0], pd[1], pd[2], __int_as_float(pi));
+  pp[1] = make_float4(pu[0], pu[1], pu[2], B.w);
+  mv.dispx = md[0];
+  mv.dispy = md[1];
+  mv.dispz = md[2];
+  return ret;
+}
+
+// Counters: [0] crossers listed, [1] movers returned, [2] movers lost for
+// lack of list space, [3] particles with an out-of-range voxel index.
+__global__ void k_clear_counters(int *c) {
+  if (threadIdx.x < 4) c[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ void push_returned_mover(
+    vpic_particle_t *p, const vpic_particle_mover_t &mv,
+    vpic_particle_mover_t *rlist, int rcap, int *cnt) {
+  int s = atomicAdd(cnt + 1, 1);
+  if (s < rcap) {
+    rlist[s] = mv;
+  } else {
+    atomicAdd(cnt + 2, 1);
+    int *pi = reinterpret_cast<int *>(&p[mv.i].i);
+    *pi = *pi >> 3;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_advance_p(
+    vpic_particle_t *__restrict__ p, int np,
+    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
+    vpic_particle_mover_t *__restrict__ xlist, int xcap,
+    vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt,
+    PushCoef C, GridDev G) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
+  const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
+
+  float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
+  float4 *pp = reinterpret_cast<float4 *>(p) + 2 * (size_t)(n < np ? n : 0);
+  if (n < np) {
+    A = pp[0];
+    B = pp[1];
+  }
+  // A voxel index outside the grid (e.g. a mover left pending as
+  // 8*voxel+face) is never dereferenced: the particle is left untouched and
+  // counted, and the call reports an error.
+  const int iraw = __float_as_int(A.w);
+  const bool active = n < np && (unsigned)iraw < (unsigned)G.nv;
+  if (n < np && !active) atomicAdd(cnt + 3, 1);
+  float dx = A.x, dy = A.y, dz = A.z;
+  const int ii = active ? iraw : 0;
+
+  // Interpolate E and B (advance_p_pipeline.cc:96-109).
+  const float4 *fp = reinterpret_cast<const float4 *>(f0 + ii);
+  const float4 fe0 = fp[0], fe1 = fp[1], fe2 = fp[2], fb0 = fp[3];
+  const float2 fb1 = reinterpret_cast<const float2 *>(fp + 4)[0];
+  const float hax =
+      C.qdt_2mc * ((fe0.x + dy * fe0.y) + dz * (fe0.z + dy * fe0.w));
+  const float hay =
+      C.qdt_2mc * ((fe1.x + dz * fe1.y) + dx * (fe1.z + dz * fe1.w));
+  const float haz =
+      C.qdt_2mc * ((fe2.x + dx * fe2.y) + dy * (fe2.z + dx * fe2.w));
+  const float cbx = fb0.x + dx * fb0.y;
+  const float cby = fb0.z + dy * fb0.w;
+  const float cbz = fb1.x + dz * fb1.y;
+
+  float ux = B.x, uy = B.y, uz = B.z, q = B.w;
+  float v0, v1, v2, v3, v4, v5;
+
+  // Boris rotation (advance_p_pipeline.cc:116-143).
+  ux += hax;
+  uy += hay;
+  uz += haz;
+  v0 = C.qdt_2mc / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+  v1 = cbx * cbx + (cby * cby + cbz * cbz);
+  v2 = (v0 * v0) * v1;
+  v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
+  v4 = v3 / (one + v1 * (v3 * v3));
+  v4 += v4;
+  v0 = ux + v3 * (uy * cbz - uz * cby);
+  v1 = uy + v3 * (uz * cbx - ux * cbz);
+  v2 = uz + v3 * (ux * cby - uy * cbx);
+  ux += v4 * (v1 * cbz - v2 * cby);
+  uy += v4 * (v2 * cbx - v0 * cbz);
+  uz += v4 * (v0 * cby - v1 * cbx);
+  ux += hax;
+  uy += hay;
+  uz += haz;
+  const float4 Bnew = make_float4(ux, uy, uz, q);
+
+  // Normalised displacement (advance_p_pipeline.cc:145-162).
+  v0 = one / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+  ux *= C.cdt_dx;
+  uy *= C.cdt_dy;
+  uz *= C.cdt_dz;
+  ux *= v0;
+  uy *= v0;
+  uz *= v0;
+  v0 = dx + ux;
+  v1 = dy + uy;
+  v2 = dz + uz;
+  v3 = v0 + ux;
+  v4 = v1 + uy;
+  v5 = v2 + uz;
+  const bool inbnds = v3 <= one && v4 <= one && v5 <= one && -v3 <= one &&
+                      -v4 <= one && -v5 <= one;
+
+  float j[12];
+  if (active && inbnds) {
+    q *= C.qsp;
+    pp[0] = make_float4(v3, v4, v5, A.w);
+    pp[1] = Bnew;
+    dx = v0;
+    dy = v1;
+    dz = v2;
+    v5 = q * ux * uy * uz * one_third;
+    streak4(j + 0, q * ux, dy, dz, v5);
+    streak4(j + 4, q * uy, dz, dx, v5);
+    streak4(j + 8, q * uz, dx, dy, v5);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; k++) j[k] = 0.0f;
+    if (active) pp[1] = Bnew;  // momentum stored, position left for move_p
+  }
+
+  // Run-segmented inclusive scan over the wavefront: consecutive lanes
+  // with the same voxel form one run; the last lane of a run ends up with
+  // the run's sum.  Crossers contribute zeros and keep their voxel as key
+  // so they do not split runs.
+  const int key = active ? ii : -1;
+  const int kprev = __shfl_up(key, 1);
+  const bool head = (lane == 0) || (kprev != key);
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int seg_start = 63 - __clzll(heads & upto);
+#pragma unroll
+  for (int d = 1; d < VPIC_WAVE; d <<= 1) {
+    const bool take = (lane - d) >= seg_start;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      const float t = __shfl_up(j[k], d);
+      if (take) j[k] += t;
+    }
+  }
+  const int knext = __shfl_down(key, 1);
+  const bool tail = (lane == 63) || (knext != key);
+  if (tail && key >= 0) deposit_atomic(acc, key, j);
+
+  // Cell crossers: compact into the crosser list (ballot + one atomic per
+  // wavefront); the rare overflow of the list is moved right here.
+  const bool cross = active && !inbnds;
+  const uint64_t cm = __ballot(cross);
+  if (cm) {
+    const int leader = __ffsll((unsigned long long)cm) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(cm));
+    base = __shfl(base, leader);
+    if (cross) {
+      const int slot = base + __popcll(cm & ((1ull << lane) - 1ull));
+      vpic_particle_mover_t mv;
+      mv.dispx = ux;
+      mv.dispy = uy;
+      mv.dispz = uz;
+      mv.i = n;
+      if (slot < xcap) {
+        xlist[slot] = mv;
+      } else if (dev_move_p(p, mv, acc, G, C.qsp)) {
+        push_returned_mover(p, mv, rlist, xcap, cnt);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_move_p_list(
+    vpic_particle_t *__restrict__ p, const vpic_particle_mover_t *xlist,
+    int xcap, vpic_particle_mover_t *__restrict__ rlist,
+    float *__restrict__ acc, int *__restrict__ cnt, float qsp, GridDev G) {
+  const int n = min(cnt[0], xcap);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += gridDim.x * blockDim.x) {
+    vpic_particle_mover_t mv = xlist[k];
+    if (dev_move_p(p, mv, acc, G, qsp))
+      push_returned_mover(p, mv, rlist, xcap, cnt);
+  }
+}
+
+// Order the returned movers by particle index (bitonic sort of
+// (index, slot) pairs in global memory by one workgroup), keep the first
+// max_nm and roll the rest back.  dev_nm[0] = nm, dev_nm[1] = n_ignored,
+// dev_nm[2] = particles skipped for an out-of-range voxel index.
+__global__ __launch_bounds__(1024) void k_order_movers(
+    vpic_particle_t *__restrict__ p, const vpic_particle_mover_t *rlist,
+    int rcap, int2 *__restrict__ buf, vpic_particle_mover_t *__restrict__ pm,
+    int max_nm, const int *__restrict__ cnt, int *__restrict__ dev_nm) {
+  const int n = min(cnt[1], rcap);
+  const int lost = cnt[2];
+  const int t = threadIdx.x, T = blockDim.x;
+  if (n == 0) {
+    if (t == 0) {
+      dev_nm[0] = 0;
+      dev_nm[1] = lost;
+      dev_nm[2] = cnt[3];
+    }
+    return;
+  }
+  int pad = 1;
+  while (pad < n) pad <<= 1;
+  for (int k = t; k < pad; k += T)
+    buf[k] = k < n ? make_int2(rlist[k].i, k) : make_int2(0x7fffffff, k);
+  __syncthreads();
+  for (int size = 2; size <= pad; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = t; k < pad; k += T) {
+        const int l = k ^ stride;
+        if (l > k) {
+          const bool up = (k & size) == 0;
+          const int2 a = buf[k], b = buf[l];
+          if ((a.x > b.x) == up) {
+            buf[k] = b;
+            buf[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int keep = max_nm < 0 ? 0 : min(n, max_nm);
+  for (int k = t; k < n; k += T) {
+    const vpic_particle_mover_t mv = rlist[buf[k].y];
+    if (k < keep) {
+      pm[k] = mv;
+    } else {
+      int *pi = reinterpret_cast<int *>(&p[mv.i].i);
+      *pi = *pi >> 3;
+    }
+  }
+  if (t == 0) {
+    dev_nm[0] = keep;
+    dev_nm[1] = (n - keep) + lost;
+    dev_nm[2] = cnt[3];
+  }
+}
+
+__global__ void k_move_p_serial(vpic_particle_t *p, vpic_particle_mover_t *pm,
+                                int nm, float *acc, GridDev G, float q,
+                                int *ret) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int k = 0; k < nm; k++) {
+    vpic_particle_mover_t mv = pm[k];
+    ret[k] = dev_move_p(p, mv, acc, G, q);
+    pm[k] = mv;
+  }
+}
+
+// Scratch of the push: counters, crosser list, returned-mover list and the
+// ordering buffer, one set per device.
+struct PushScratch {
+  DeviceScratch counters, xlist, rlist, sortbuf;
+};
+
+PushScratch &push_scratch() {
+  static thread_local PushScratch s[16];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return s[dev & 15];
+}
+
+int pow2_at_least(int64_t n) {
+  int64_t p = 1;
+  while (p < n) p <<= 1;
+  return (int)p;
+}
+
+}  // namespace
+
+int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
+                     vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
+                     vpic_accumulator_t *a, const vpic_interpolator_t *f,
+                     const vpic_grid_t *g, hipStream_t s) {
+  PushScratch &S = push_scratch();
+  // Crosser list sized for ~20% of the particles per step; the rare
+  // excess is moved inline by the push kernel.
+  int64_t xcap64 = np / 5;
+  if (xcap64 < 65536) xcap64 = 65536;
+  if (xcap64 > np) xcap64 = np < 1 ? 1 : np;
+  const int xcap = (int)xcap64;
+  int rc;
+  if ((rc = scratch_reserve(S.counters, 64)) ||
+      (rc = scratch_reserve(S.xlist, sizeof(vpic_particle_mover_t) * xcap)) ||
+      (rc = scratch_reserve(S.rlist, sizeof(vpic_particle_mover_t) * xcap)) ||
+      (rc = scratch_reserve(S.sortbuf, sizeof(int2) * pow2_at_least(xcap))))
+    return rc;
+  int *cnt = (int *)S.counters.ptr;
+  vpic_particle_mover_t *xlist = (vpic_particle_mover_t *)S.xlist.ptr;
+  vpic_particle_mover_t *rlist = (vpic_particle_mover_t *)S.rlist.ptr;
+
+  // advance_p_pipeline.cc:279-283, in float.
+  PushCoef C;
+  C.qdt_2mc = (q * g->dt) / (2 * m * g->cvac);
+  C.cdt_dx = g->cvac * g->dt * g->rdx;
+  C.cdt_dy = g->cvac * g->dt * g->rdy;
+  C.cdt_dz = g->cvac * g->dt * g->rdz;
+  C.qsp = q;
+  const GridDev G = make_grid_dev(g);
+
+  hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
+  VPIC_LAUNCHED("k_clear_counters");
+  if (np > 0) {
+    const bool timed = push_timing_on();
+    if (timed) push_timing_record(s, np, true);
+    const int blocks = (int)(((int64_t)np + 255) / 256);
+    hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p, np, f,
+                       (float *)a, xlist, xcap, rlist, cnt, C, G);
+    VPIC_LAUNCHED("k_advance_p");
+    if (timed) push_timing_record(s, np, false);
+    int mblocks = (xcap + 255) / 256;
+    if (mblocks > 8192) mblocks = 8192;
+    hipLaunchKernelGGL(k_move_p_list, dim3(mblocks), dim3(256), 0, s, p,
+                       xlist, xcap, rlist, (float *)a, cnt, q, G);
+    VPIC_LAUNCHED("k_move_p_list");
+  }
+  hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
+                     xcap, (int2 *)S.sortbuf.ptr, pm, max_nm, cnt, dev_nm);
+  VPIC_LAUNCHED("k_order_movers");
+  return VPIC_OK;
+}
+
+int launch_move_p_list(vpic_particle_t *p, vpic_particle_mover_t *pm, int nm,
+                       vpic_accumulator_t *a, const vpic_grid_t *g, float q,
+                       int *dev_ret, hipStream_t s) {
+  const GridDev G = make_grid_dev(g);
+  hipLaunchKernelGGL(k_move_p_serial, dim3(1), dim3(64), 0, s, p, pm, nm,
+                     (float *)a, G, q, dev_ret);
+  VPIC_LAUNCHED("k_move_p_serial");
+  return VPIC_OK;
+}

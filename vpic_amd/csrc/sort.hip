@@ -1,0 +1,344 @@
+// sort.hip -- sort_p on gfx950: stable counting sort of the particles by
+// voxel, and the voxel partition.
+//
+// Reference: src/species_advance/standard/pipeline/sort_p_pipeline.cc
+// (:143-214 fine subsort, :221-371 driver).  The reference's result is the
+// stable sort by p.i (the coarse/fine split never reorders equal keys), and
+// partition[v] = index of the first particle of voxel v, 0 below the first
+// interior voxel and np above the last (:335-340, :354-360).
+//
+// Device passes (np particles, nv voxels):
+//   k_count      key[k] = p[k].i; run-aggregated histogram (one atomic per run
+//                of equal keys in a wavefront)
+//   scan         partition = exclusive scan of the histogram (partition[nv]=np)
+//   k_scatter    perm[cursor[key]++] = k (one atomic per run; lanes of a run
+//                take consecutive slots in index order)
+//   k_fixup      equal-key segments sorted by particle index in LDS (one
+//                wavefront per voxel), which makes the order stable whatever
+//                order the atomics of different wavefronts took
+//   k_gather     out[j] = p[perm[j]]
+#include "common.h"
+
+namespace {
+
+__device__ __forceinline__ void run_info(int key, int lane, int &seg_start,
+                                         int &run_tail, bool &is_tail) {
+  const int kprev = __shfl_up(key, 1);
+  const int knext = __shfl_down(key, 1);
+  const bool head = (lane == 0) || (kprev != key);
+  is_tail = (lane == 63) || (knext != key);
+  const uint64_t heads = __ballot(head);
+  const uint64_t tails = __ballot(is_tail);
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  seg_start = 63 - __clzll(heads & upto);
+  run_tail = __ffsll((unsigned long long)(tails & ~((1ull << lane) - 1ull))) - 1;
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int *a, int64_t n, int v) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    a[k] = v;
+}
+
+__global__ __launch_bounds__(256) void k_count(
+    const vpic_particle_t *__restrict__ p, int np, int nv,
+    int *__restrict__ keys, int *__restrict__ count, int *__restrict__ nbad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
+  int key = k < np ? p[k].i : -1;
+  if (k < np && (unsigned)key >= (unsigned)nv) {
+    atomicAdd(nbad, 1);  // never used as an index; the sort reports it
+    key = -1;
+  }
+  if (k < np) keys[k] = key;
+  int s0, tail_lane;
+  bool tail;
+  run_info(key, lane, s0, tail_lane, tail);
+  if (tail && key >= 0) atomicAdd(count + key, lane - s0 + 1);
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const int *__restrict__ keys,
+                                                 int np, int *__restrict__ cursor,
+                                                 int *__restrict__ perm) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
+  const int key = k < np ? keys[k] : -1;
+  int s0, tail_lane;
+  bool tail;
+  run_info(key, lane, s0, tail_lane, tail);
+  int base = 0;
+  if (tail && key >= 0) base = atomicAdd(cursor + key, lane - s0 + 1);
+  base = __shfl(base, tail_lane);
+  if (key >= 0) perm[base + (lane - s0)] = k;
+}
+
+// Block-wise exclusive scan: 256 threads x 8 items.
+constexpr int SCAN_ITEMS = 2048;
+
+__device__ int block_exclusive_scan(int v, int *sh) {
+  // inclusive scan of 256 values in shared memory, returns exclusive
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    int x = t >= d ? sh[t - d] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  int incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_local(const int *__restrict__ in,
+                                                    int *__restrict__ out,
+                                                    int64_t n,
+                                                    int *__restrict__ sums) {
+  __shared__ int sh[256];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_ITEMS + threadIdx.x * 8;
+  int v[8], s = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    s += v[i];
+  }
+  int off = block_exclusive_scan(s, sh);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    if (base + i < n) out[base + i] = off;
+    off += v[i];
+  }
+  if (threadIdx.x == 255) sums[blockIdx.x] = off;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(int *sums, int nb) {
+  __shared__ int sh[256];
+  int carry = 0;
+  for (int c = 0; c < nb; c += 256) {
+    const int i = c + threadIdx.x;
+    const int v = i < nb ? sums[i] : 0;
+    const int ex = block_exclusive_scan(v, sh);
+    if (i < nb) sums[i] = ex + carry;
+    __syncthreads();
+    // total of this chunk
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int j = 0; j < 256; j++) t += sh[j];
+      sh[0] = t;
+    }
+    __syncthreads();
+    carry += sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int *out, int64_t n,
+                                                  const int *sums) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_ITEMS;
+  const int add = sums[blockIdx.x];
+  for (int i = threadIdx.x; i < SCAN_ITEMS; i += 256)
+    if (base + i < n) out[base + i] += add;
+}
+
+// One single-wavefront workgroup per voxel segment (so every barrier below
+// is uniform); segments longer than FIX_CAP go to a list handled by
+// k_fixup_big.
+constexpr int FIX_CAP = 1024;
+
+__global__ __launch_bounds__(64) void k_fixup(const int *__restrict__ part,
+                                              int nv, int *__restrict__ perm,
+                                              int *__restrict__ big,
+                                              int *__restrict__ nbig) {
+  __shared__ int s[FIX_CAP];
+  const int lane = threadIdx.x;
+  const int v = blockIdx.x;
+  if (v >= nv) return;
+  const int b = part[v], c = part[v + 1] - b;
+  if (c <= 1) return;
+  if (c > FIX_CAP) {
+    if (lane == 0) {
+      big[atomicAdd(nbig, 1)] = v;
+      atomicMax(nbig + 1, c);
+    }
+    return;
+  }
+  // already ascending?
+  bool ok = true;
+  for (int k = lane; k + 1 < c; k += 64) ok &= perm[b + k] < perm[b + k + 1];
+  if (__all(ok)) return;
+  int pad = 1;
+  while (pad < c) pad <<= 1;
+  for (int k = lane; k < pad; k += 64) s[k] = k < c ? perm[b + k] : 0x7fffffff;
+  __syncthreads();
+  for (int size = 2; size <= pad; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = lane; k < pad; k += 64) {
+        const int l = k ^ stride;
+        if (l > k) {
+          const bool up = (k & size) == 0;
+          const int x = s[k], y = s[l];
+          if ((x > y) == up) {
+            s[k] = y;
+            s[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int k = lane; k < c; k += 64) perm[b + k] = s[k];
+}
+
+// Long segments: one workgroup each, bitonic in global memory on a padded
+// copy held in `tmp` (sized for the longest segment).
+__global__ __launch_bounds__(1024) void k_fixup_big(const int *__restrict__ part,
+                                                    int *__restrict__ perm,
+                                                    const int *__restrict__ big,
+                                                    const int *__restrict__ nbig,
+                                                    int *__restrict__ tmp,
+                                                    int64_t tmp_stride) {
+  for (int j = blockIdx.x; j < *nbig; j += gridDim.x) {
+    const int v = big[j];
+    const int b = part[v], c = part[v + 1] - b;
+    int *s = tmp + (int64_t)blockIdx.x * tmp_stride;
+    int pad = 1;
+    while (pad < c) pad <<= 1;
+    for (int k = threadIdx.x; k < pad; k += blockDim.x)
+      s[k] = k < c ? perm[b + k] : 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= pad; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int k = threadIdx.x; k < pad; k += blockDim.x) {
+          const int l = k ^ stride;
+          if (l > k) {
+            const bool up = (k & size) == 0;
+            const int a = s[k], bb = s[l];
+            if ((a > bb) == up) {
+              s[k] = bb;
+              s[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (int k = threadIdx.x; k < c; k += blockDim.x) perm[b + k] = s[k];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather(
+    const vpic_particle_t *__restrict__ src, const int *__restrict__ perm,
+    int np, vpic_particle_t *__restrict__ dst) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const float4 *s = reinterpret_cast<const float4 *>(src + perm[j]);
+  float4 *d = reinterpret_cast<float4 *>(dst + j);
+  const float4 a = s[0], b = s[1];
+  d[0] = a;
+  d[1] = b;
+}
+
+struct SortScratch {
+  DeviceScratch keys, perm, count, sums, big, nbig, tmp;
+};
+SortScratch &sort_scratch() {
+  static thread_local SortScratch s[16];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return s[dev & 15];
+}
+
+inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// Exclusive scan of n ints (in may equal out).  sums: ceil(n/2048) ints.
+int device_exclusive_scan(const int *in, int *out, int64_t n, int *sums,
+                          hipStream_t s) {
+  const int nb = blocks_for(n, SCAN_ITEMS);
+  hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, s, in, out, n,
+                     sums);
+  VPIC_LAUNCHED("k_scan_local");
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, sums, nb);
+  VPIC_LAUNCHED("k_scan_sums");
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, s, out, n,
+                     (const int *)sums);
+  VPIC_LAUNCHED("k_scan_add");
+  return VPIC_OK;
+}
+
+int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                  int *partition, const vpic_grid_t *g, int in_place,
+                  hipStream_t s) {
+  SortScratch &S = sort_scratch();
+  const int nv = g->nv;
+  const int64_t nslots = (int64_t)nv + 1;
+  int rc;
+  if ((rc = scratch_reserve(S.keys, sizeof(int) * (size_t)(np > 0 ? np : 1))) ||
+      (rc = scratch_reserve(S.perm, sizeof(int) * (size_t)(np > 0 ? np : 1))) ||
+      (rc = scratch_reserve(S.count, sizeof(int) * (size_t)nslots)) ||
+      (rc = scratch_reserve(S.sums,
+                            sizeof(int) * (size_t)(blocks_for(nslots, SCAN_ITEMS) + 1))) ||
+      (rc = scratch_reserve(S.big, sizeof(int) * (size_t)nv)) ||
+      (rc = scratch_reserve(S.nbig, 64)))
+    return rc;
+  int *keys = (int *)S.keys.ptr, *perm = (int *)S.perm.ptr;
+  int *count = (int *)S.count.ptr, *sums = (int *)S.sums.ptr;
+  int *big = (int *)S.big.ptr, *nbig = (int *)S.nbig.ptr;
+
+  int fb = blocks_for(nslots, 256);
+  if (fb > 65536) fb = 65536;
+  hipLaunchKernelGGL(k_fill_int, dim3(fb), dim3(256), 0, s, count, nslots, 0);
+  VPIC_LAUNCHED("k_fill_int");
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, nbig, (int64_t)3, 0);
+  VPIC_LAUNCHED("k_fill_int");
+  if (np > 0) {
+    hipLaunchKernelGGL(k_count, dim3(blocks_for(np, 256)), dim3(256), 0, s, p,
+                       np, nv, keys, count, nbig + 2);
+    VPIC_LAUNCHED("k_count");
+  }
+  // partition = exclusive scan of the counts over nv+1 slots; count[nv] = 0
+  // so partition[nv] = np.
+  if ((rc = device_exclusive_scan(count, partition, nslots, sums, s)))
+    return rc;
+  if (np == 0) return VPIC_OK;
+  // cursor = partition (reuse the count array)
+  VPIC_HIP(hipMemcpyAsync(count, partition, sizeof(int) * (size_t)nslots,
+                          hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_scatter, dim3(blocks_for(np, 256)), dim3(256), 0, s,
+                     keys, np, count, perm);
+  VPIC_LAUNCHED("k_scatter");
+  hipLaunchKernelGGL(k_fixup, dim3(nv), dim3(64), 0, s,
+                     (const int *)partition, nv, perm, big, nbig);
+  VPIC_LAUNCHED("k_fixup");
+  // Long segments (rare): size the padded scratch for the longest possible
+  // segment only when some exist.
+  int h_nbig[3] = {0, 0, 0};
+  VPIC_HIP(hipMemcpyAsync(h_nbig, nbig, sizeof(h_nbig), hipMemcpyDeviceToHost,
+                          s));
+  VPIC_HIP(hipStreamSynchronize(s));
+  if (h_nbig[2] > 0)
+    return vpic_set_error(VPIC_EINVAL,
+                          "sort_p: %d particles have a voxel index outside "
+                          "[0,nv); particles left unsorted", h_nbig[2]);
+  if (h_nbig[0] > 0) {
+    int64_t pad = 1;
+    while (pad < h_nbig[1]) pad <<= 1;
+    const int nblk = h_nbig[0] < 64 ? h_nbig[0] : 64;
+    if ((rc = scratch_reserve(S.tmp, sizeof(int) * (size_t)(pad * nblk))))
+      return rc;
+    hipLaunchKernelGGL(k_fixup_big, dim3(nblk), dim3(1024), 0, s,
+                       (const int *)partition, perm, (const int *)big,
+                       (const int *)nbig, (int *)S.tmp.ptr, pad);
+    VPIC_LAUNCHED("k_fixup_big");
+  }
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(np, 256)), dim3(256), 0, s, p,
+                     perm, np, aux);
+  VPIC_LAUNCHED("k_gather");
+  if (in_place)
+    VPIC_HIP(hipMemcpyAsync(p, aux, sizeof(vpic_particle_t) * (size_t)np,
+                            hipMemcpyDeviceToDevice, s));
+  return VPIC_OK;
+}
