@@ -1,0 +1,231 @@
+"""Benchmark of the per-step PIC hot path on one MI355X.
+
+Workload (the plain method at scale): a 3D periodic uniform thermal plasma
+on 256x128x128 cells (dx = d_e = 1, natural units c = eps0 = m_e = |q_e| =
+n0 = 1), electrons and ions (mi/me = 25), 100 particles per cell per
+species (838,860,800 particles), uth_e = 0.1c, uth_i = 0.02c, equal
+weights n0*dV/ppc, dt = 0.95 of the Courant limit, fields initially zero,
+sort every 20 steps.  A step is one pass of the whole hot path:
+sort_p (every 20th step), clear/advance_p x2/reduce, unload + synchronize_jf,
+advance_b / advance_e / advance_b, load_interpolator.
+
+value = particle-steps/s over the timed steps (whole job, data resident in
+HBM).  roofline = the push kernel (k_advance_p): algorithmic 64 B per
+particle (one 32-B particle read, one 32-B write) per launch over its mean
+launch time from hipEvents on its stream.  cpu_baseline = the reference's
+own advance_p (oracle/_ref, compiled from the lanl/vpic sources; else the
+oracle's restatement) on the host cores, on a bounded sample of the same
+plasma.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("particle-steps/s on 1 MI355X, % HBM roofline; ensemble runs/s; "
+          "adjoint/fwd time")
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE = 64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_sim(nx, ny, nz, ppc, seed=2026, sort_interval=20):
+    from vpic_amd import sim as S
+    from vpic_amd import types as T
+    dt = 0.95 * T.courant_length(nx, ny, nz, nx, ny, nz)
+    g = T.grid_periodic(0, 0, 0, nx, ny, nz, nx, ny, nz, dt)
+    s = S.Simulation(g)
+    w = 1.0 / ppc  # n0 * dV / ppc with n0 = dV = 1
+    S.uniform_plasma(s, ppc, -1.0, 1.0, 0.1, seed, "electron", w=w,
+                     sort_interval=sort_interval)
+    S.uniform_plasma(s, ppc, 1.0, 25.0, 0.02, seed + 1, "ion", w=w,
+                     sort_interval=sort_interval)
+    s.load_interpolator_array()
+    return s
+
+
+def cpu_baseline(nx=64, ny=64, nz=64, ppc=100, budget_s=12.0):
+    """The reference's advance_p on the host cores: N pthreads, each one
+    DISTRIBUTE(np,16) block with a private accumulator copy, the caller on
+    the tail (advance_p_pipeline.cc:253-340).  Sample: the same plasma on a
+    nx*ny*nz-cell box (both species), zero fields."""
+    import numpy as np
+    import torch
+    from oracle.oracle import Oracle, Ref, host_grid
+    from vpic_amd import types as T
+
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS",
+                                                     threads)), 16))
+    s = build_sim(nx, ny, nz, ppc, seed=99, sort_interval=0)
+    torch.cuda.synchronize()
+    parts = [sp.p.download().copy() for sp in s.species]
+    qm = [(sp.q, sp.m) for sp in s.species]
+    g = s.g
+    hg = host_grid(g)
+    fi = np.zeros(g.nv, dtype=T.INTERPOLATOR)
+    stride = (g.nv + 1) & ~1
+    use_ref = Ref.available()
+    if use_ref:
+        ref = Ref.get()
+        acc = np.zeros(((threads + 1) * stride, 12), dtype=np.float32)
+        pm = np.zeros(max(2 * len(parts[0]) // 25, 16 * 273), dtype=T.MOVER)
+
+        def push(p, q, m):
+            ref.advance_p_pipelines(p, q, m, acc, fi, hg, threads, pm)
+    else:
+        orc = Oracle.get()
+        acc = np.zeros(((threads + 1) * g.nv, 12), dtype=np.float32)
+
+        def push(p, q, m):
+            orc.advance_p_threaded(p, q, m, acc, fi, hg, threads,
+                                   max(2 * len(p) // 25, 16 * 273))
+    # 3 warm-up calls, then timed calls (sample/bench/advance_p:46-52)
+    for _ in range(3):
+        for p, (q, m) in zip(parts, qm):
+            push(p, q, m)
+    n_step, t0 = 0, time.perf_counter()
+    while True:
+        for p, (q, m) in zip(parts, qm):
+            push(p, q, m)
+        n_step += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n_step >= 50:
+            break
+    npart = sum(len(p) for p in parts)
+    return {"value": npart * n_step / el, "unit": "particle-steps/s",
+            "cores": threads, "kind": "reference" if use_ref else "port",
+            "sample": ("advance_p of both species of the same plasma on a "
+                       "%dx%dx%d-cell box (%d particles), zero fields, %d "
+                       "timed steps after 3 warm-ups, %.1f s"
+                       % (nx, ny, nz, npart, n_step, el))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a "
+              "single MI355X (multi-GPU sharding is design-only); use "
+              "--gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from vpic_amd import _lib
+    from vpic_amd import sim as S
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    _lib.call("vpic_hip_init", 0)
+    log("building %dx%dx%d, %d ppc/species" % (args.nx, args.ny, args.nz,
+                                               args.ppc))
+    s = build_sim(args.nx, args.ny, args.nz, args.ppc)
+    npart = sum(sp.np for sp in s.species)
+    torch.cuda.synchronize()
+    log("particles: %d" % npart)
+    for _ in range(args.warmup):
+        s.advance()
+    torch.cuda.synchronize()
+    nm, ign, bad = s.check_movers()
+    assert bad == 0 and nm == 0, (nm, ign, bad)
+    log("warmup done (%d steps)" % args.warmup)
+
+    timer = S.PhaseTimer()
+    _lib.call("vpic_hip_push_timing", None, None, None)  # reset
+    _lib.call("vpic_hip_push_timing_enable", 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.advance(timer)
+        if (k + 1) % 50 == 0:
+            log("step %d" % (k + 1))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _lib.call("vpic_hip_push_timing_enable", 0)
+    launches, push_ms, push_parts = (ctypes.c_int64(), ctypes.c_double(),
+                                     ctypes.c_int64())
+    _lib.call("vpic_hip_push_timing", ctypes.byref(launches),
+              ctypes.byref(push_ms), ctypes.byref(push_parts))
+    phases = timer.totals_ms()
+    nm, ign, bad = s.check_movers()
+    assert bad == 0
+
+    value = npart * args.steps / el
+    avg_launch_s = push_ms.value / 1e3 / max(launches.value, 1)
+    bytes_per_launch = BYTES_PER_PARTICLE * push_parts.value / max(
+        launches.value, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("3D periodic uniform thermal plasma %dx%dx%d cells, "
+                         "electrons+ions mi/me=25, %d ppc/species "
+                         "(%d particles), uth_e=0.1c uth_i=0.02c, dt=0.95 "
+                         "Courant, sort every 20 steps"
+                         % (args.nx, args.ny, args.nz, args.ppc, npart)),
+            "particles": npart,
+            "cells": args.nx * args.ny * args.nz,
+            "sort_interval": 20,
+        },
+        "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
+        "push_kernel": {
+            "name": "k_advance_p",
+            "launches": launches.value,
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "particle_steps_per_s": push_parts.value / (push_ms.value / 1e3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    pmc = os.path.join(ROOT, "profiles", "k_advance_p_c2_traffic.json")
+    if os.path.exists(pmc) and (args.nx, args.ny, args.nz, args.ppc) == (
+            256, 128, 128, 100):
+        with open(pmc) as fh:
+            out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch")
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(budget_s=args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -170,6 +170,10 @@ class Ref:
                                       ctypes.c_int, _vp, _vp, _vp, _pg,
                                       ctypes.c_int]
         lib.ref_advance_p.restype = ctypes.c_int
+        lib.ref_advance_p_pipelines.argtypes = [_vp, ctypes.c_int, _f32, _f32,
+                                                _vp, ctypes.c_int, _vp, _vp,
+                                                _pg, ctypes.c_int]
+        lib.ref_advance_p_pipelines.restype = ctypes.c_int
         lib.ref_move_p.argtypes = [_vp, _vp, _vp, _pg, _f32]
         lib.ref_move_p.restype = ctypes.c_int
         lib.ref_energy_p.argtypes = [_vp, ctypes.c_int, _f32, _f32, _vp, _pg]
@@ -202,6 +206,14 @@ class Ref:
                                     _ptr(ign), _ptr(acc), _ptr(interp),
                                     ctypes.byref(hg.g), n_pipeline)
         return nm, int(ign[0]), pm[:nm].copy()
+
+    def advance_p_pipelines(self, p, q, m, acc_copies, interp, hg, n_pipeline,
+                            pm):
+        """Timed part of the reference's advance_p on n_pipeline threads;
+        acc_copies holds (n_pipeline+1)*POW2_CEIL(nv,2) accumulators."""
+        return self.lib.ref_advance_p_pipelines(
+            _ptr(p), len(p), q, m, _ptr(pm), len(pm), _ptr(acc_copies),
+            _ptr(interp), ctypes.byref(hg.g), n_pipeline)
 
     def move_p(self, p, mover, acc, hg, q):
         return self.lib.ref_move_p(_ptr(p), _ptr(mover), _ptr(acc),

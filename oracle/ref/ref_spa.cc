@@ -141,6 +141,59 @@ extern "C" int ref_advance_p(vpic_particle_t *p, int np, float q, float m,
   return nm;
 }
 
+// The timed part of advance_p only (EXEC_PIPELINES + WAIT_PIPELINES + mover
+// compaction, advance_p_pipeline.cc:272-339) on n_pipeline pthreads, with
+// caller-owned accumulator copies ((n_pipeline+1) * POW2_CEIL(nv,2)
+// entries, as new_accumulator_array allocates them).  Used as the CPU
+// baseline; the accumulator reduction is a separate step in the reference
+// (reduce_accumulator_array) and is not included.
+extern "C" int ref_advance_p_pipelines(vpic_particle_t *p, int np, float q,
+                                       float m, vpic_particle_mover_t *pm,
+                                       int max_nm, vpic_accumulator_t *acc,
+                                       const vpic_interpolator_t *f,
+                                       const vpic_grid_t *g, int n_pipeline) {
+  grid_t rg;
+  ref_fill_grid(&rg, g);
+  if (n_pipeline < 1) n_pipeline = 1;
+  if (n_pipeline > MAX_PIPELINE) n_pipeline = MAX_PIPELINE;
+  static particle_mover_seg_t seg[MAX_PIPELINE + 1];
+  advance_p_pipeline_args_t args[1];
+  args->p0 = (particle_t *)p;
+  args->pm = (particle_mover_t *)pm;
+  args->a0 = (accumulator_t *)acc;
+  args->f0 = (const interpolator_t *)f;
+  args->seg = seg;
+  args->g = &rg;
+  args->qdt_2mc = (q * rg.dt) / (2 * m * rg.cvac);
+  args->cdt_dx = rg.cvac * rg.dt * rg.rdx;
+  args->cdt_dy = rg.cvac * rg.dt * rg.rdy;
+  args->cdt_dz = rg.cvac * rg.dt * rg.rdz;
+  args->qsp = q;
+  args->np = np;
+  args->max_nm = max_nm;
+  args->nx = rg.nx;
+  args->ny = rg.ny;
+  args->nz = rg.nz;
+  pthread_t th[MAX_PIPELINE];
+  ref_push_job job[MAX_PIPELINE];
+  for (int r = 0; r < n_pipeline; r++) {
+    job[r].args = args;
+    job[r].rank = r;
+    job[r].n = n_pipeline;
+    pthread_create(&th[r], 0, ref_push_thread, &job[r]);
+  }
+  advance_p_pipeline_scalar(args, n_pipeline, n_pipeline);
+  for (int r = 0; r < n_pipeline; r++) pthread_join(th[r], 0);
+  int nm = 0;
+  for (int r = 0; r <= n_pipeline; r++) {
+    if ((particle_mover_t *)pm + nm != seg[r].pm)
+      memmove((particle_mover_t *)pm + nm, seg[r].pm,
+              sizeof(particle_mover_t) * (size_t)seg[r].nm);
+    nm += seg[r].nm;
+  }
+  return nm;
+}
+
 extern "C" int ref_move_p(vpic_particle_t *p0, vpic_particle_mover_t *pm,
                           vpic_accumulator_t *a0, const vpic_grid_t *g,
                           float qsp) {

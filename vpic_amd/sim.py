@@ -1,0 +1,200 @@
+"""Host-side step driver over the C-ABI, in the call order of
+vpic_simulation::advance() (src/vpic/advance.cc:15-207) restricted to the
+hot path: sort_p, clear_accumulator_array, advance_p per species,
+reduce_accumulator_array, clear_jf / unload_accumulator_array /
+synchronize_jf, advance_b(0.5), advance_e(1), advance_b(0.5),
+load_interpolator_array.  Collisions, emitters, boundary_p's inter-domain
+exchange, divergence cleaning and dumps are not part of the path.
+
+All arrays live in HBM (vpic_amd.device); every call is stream-ordered on
+one HIP stream, and the only per-step host synchronisation is the sort's.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib, device
+from . import types as T
+
+
+class Species:
+    """species_t (src/species_advance/species_advance_aos.h:54-94) with
+    HBM-resident particle, mover and partition arrays."""
+
+    def __init__(self, name, q, m, np_, grid, max_nm=None, sort_interval=0):
+        self.name = name
+        self.q = float(q)
+        self.m = float(m)
+        self.np = int(np_)
+        self.max_np = int(np_)
+        self.p = device.particles(np_)
+        # vpic.h:507-511: default max_nm = 2*np/25, at least 16*(MAX_PIPELINE+1)
+        if max_nm is None:
+            max_nm = max(2 * np_ // 25, 16 * 273)
+        self.max_nm = int(max_nm)
+        self.pm = device.movers(self.max_nm)
+        self.nm = 0
+        self.sort_interval = int(sort_interval)
+        self.last_sorted = -1
+        self.partition = device.DeviceArray((grid.nv + 1,), np.int32)
+
+
+class Simulation:
+    """The per-step hot path on one GPU."""
+
+    def __init__(self, grid, field_params=None):
+        self.g = grid
+        self.mp = field_params or T.vacuum_params()
+        self.fields = device.fields(grid)
+        self.fields.zero()
+        self.interp = device.interpolators(grid)
+        self.interp.zero()
+        self.acc = device.accumulators(grid)
+        self.acc.zero()
+        self.species = []
+        self.aux = None
+        self.step = 0
+        self.stream = None  # the default stream
+        self.dev_nm = device.DeviceArray((4,), np.int32)
+        self.nm_log = []
+
+    def add_species(self, sp):
+        self.species.append(sp)
+        if self.aux is None or self.aux.shape[0] < sp.np:
+            self.aux = device.particles(max(sp.np, 1))
+        return sp
+
+    # -- the reference-named operations ------------------------------------
+    def _G(self):
+        return ctypes.byref(self.g)
+
+    def sort_p(self, sp):
+        """sort_p (sort_p.cc:154-165): stable by voxel, into the shared aux
+        buffer, then the two buffers swap roles."""
+        _lib.call("vpic_hip_sort_p", sp.p.ptr, self.aux.ptr, sp.np,
+                  sp.partition.ptr, self._G(), 0, self.stream)
+        sp.p, self.aux = self.aux, sp.p
+        sp.last_sorted = self.step
+
+    def advance_p(self, sp):
+        _lib.call("vpic_hip_advance_p_async", sp.p.ptr, sp.np, sp.q, sp.m,
+                  sp.pm.ptr, sp.max_nm, self.dev_nm.ptr, self.acc.ptr,
+                  self.interp.ptr, self._G(), self.stream)
+
+    def load_interpolator_array(self):
+        _lib.call("vpic_hip_load_interpolator_array", self.interp.ptr,
+                  self.fields.ptr, self._G(), self.stream)
+
+    def clear_accumulator_array(self):
+        _lib.call("vpic_hip_clear_accumulator_array", self.acc.ptr,
+                  self._G(), self.stream)
+
+    def reduce_accumulator_array(self):
+        _lib.call("vpic_hip_reduce_accumulator_array", self.acc.ptr,
+                  self._G(), self.stream)
+
+    def unload_accumulator_array(self):
+        _lib.call("vpic_hip_clear_jf", self.fields.ptr, self._G(), self.stream)
+        _lib.call("vpic_hip_unload_accumulator_array", self.fields.ptr,
+                  self.acc.ptr, self._G(), self.stream)
+        _lib.call("vpic_hip_synchronize_jf", self.fields.ptr, self._G(),
+                  self.stream)
+
+    def advance_b(self, frac):
+        _lib.call("vpic_hip_advance_b", self.fields.ptr, self._G(), frac,
+                  self.stream)
+
+    def advance_e(self, frac=1.0):
+        _lib.call("vpic_hip_vacuum_advance_e", self.fields.ptr, self._G(),
+                  ctypes.byref(self.mp), frac, self.stream)
+
+    def energy_f(self):
+        en = (ctypes.c_double * 6)()
+        _lib.call("vpic_hip_energy_f", en, self.fields.ptr, self._G(),
+                  ctypes.byref(self.mp), self.stream)
+        return np.array(en[:])
+
+    def energy_p(self, sp):
+        en = ctypes.c_double()
+        _lib.call("vpic_hip_energy_p", sp.p.ptr, sp.np, sp.q, sp.m,
+                  self.interp.ptr, self._G(), ctypes.byref(en), self.stream)
+        return en.value
+
+    def check_movers(self):
+        """nm of the last advance_p (host read; synchronises)."""
+        torch.cuda.synchronize()
+        v = self.dev_nm.download()
+        return int(v[0]), int(v[1]), int(v[2])
+
+    # -- one time step (advance.cc:15-207) ----------------------------------
+    def advance(self, timer=None):
+        t = timer or (lambda name: _NullCtx())
+        with t("sort"):
+            for sp in self.species:
+                if sp.sort_interval > 0 and self.step % sp.sort_interval == 0:
+                    self.sort_p(sp)
+        with t("push"):
+            self.clear_accumulator_array()
+            for sp in self.species:
+                self.advance_p(sp)
+            self.reduce_accumulator_array()
+        with t("field"):
+            self.unload_accumulator_array()
+            self.advance_b(0.5)
+            self.advance_e(1.0)
+            self.advance_b(0.5)
+        with t("interp"):
+            self.load_interpolator_array()
+        self.step += 1
+
+
+class _NullCtx:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+class PhaseTimer:
+    """torch events (on the default stream, where the C-ABI calls run)
+    around each phase; elapsed times are collected at the end."""
+
+    def __init__(self):
+        self.ev = {}
+
+    def __call__(self, name):
+        timer = self
+
+        class _Ctx:
+            def __enter__(self_):
+                self_.a = torch.cuda.Event(enable_timing=True)
+                self_.a.record()
+                return self_
+
+            def __exit__(self_, *exc):
+                b = torch.cuda.Event(enable_timing=True)
+                b.record()
+                timer.ev.setdefault(name, []).append((self_.a, b))
+                return False
+
+        return _Ctx()
+
+    def totals_ms(self):
+        torch.cuda.synchronize()
+        return {k: sum(a.elapsed_time(b) for a, b in v)
+                for k, v in self.ev.items()}
+
+
+def uniform_plasma(sim, ppc, q, m, uth, seed, name, drift=(0.0, 0.0, 0.0),
+                   w=1.0, sort_interval=0):
+    """Species loaded as a uniform thermal plasma (device loader)."""
+    g = sim.g
+    np_ = ppc * g.nx * g.ny * g.nz
+    sp = Species(name, q, m, np_, g, sort_interval=sort_interval)
+    uth = np.broadcast_to(np.asarray(uth, dtype=np.float64), (3,))
+    _lib.call("vpic_hip_load_uniform_plasma", sp.p.ptr, ctypes.byref(g), ppc,
+              float(uth[0]), float(uth[1]), float(uth[2]), float(drift[0]),
+              float(drift[1]), float(drift[2]), float(w), int(seed), None)
+    return sim.add_species(sp)
