@@ -142,12 +142,19 @@ const char *vpic_hip_version(void);
  * cannot handle are returned as movers in `pm` in particle-index order
  * (up to max_nm, the rest rolled back as the reference does); `*host_nm`
  * receives their count and `*host_n_ignored` (may be NULL) the number
- * that did not fit.  Synchronous on return. */
+ * that did not fit.  `partition` (may be NULL) is the nv+1 voxel partition
+ * written by the last vpic_hip_sort_p of these particles (species_t's
+ * partition, species_advance_aos.h:69-89): when given, the particles are
+ * pushed tile by tile with LDS-resident current accumulation (any
+ * particles past partition[nv] are pushed too).  It changes only the
+ * speed, never the result beyond the summation order of the currents.
+ * Synchronous on return. */
 int vpic_hip_advance_p(vpic_particle_t *p, int32_t np, float q, float m,
                        vpic_particle_mover_t *pm, int32_t max_nm,
                        int32_t *host_nm, int32_t *host_n_ignored,
                        vpic_accumulator_t *a, const vpic_interpolator_t *f,
-                       const vpic_grid_t *g, void *stream);
+                       const vpic_grid_t *g, const int32_t *partition,
+                       void *stream);
 
 /* Stream-ordered form of vpic_hip_advance_p: writes {nm, n_ignored,
  * n_bad} to the three device ints at `dev_nm` instead of the host and does
@@ -159,7 +166,13 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
                              int32_t max_nm, int32_t *dev_nm,
                              vpic_accumulator_t *a,
                              const vpic_interpolator_t *f,
-                             const vpic_grid_t *g, void *stream);
+                             const vpic_grid_t *g, const int32_t *partition,
+                             void *stream);
+
+/* Tile shape (cells) of the partition-driven push; its LDS window holds
+ * (tx+2)(ty+2)(tz+2) accumulators.  Default 8x8x4.  No reference
+ * counterpart (the reference's pipelines split particles, not cells). */
+int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
 
 /* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),
  * applied to the nm movers pm[0..nm) in order on the device (one lane, as

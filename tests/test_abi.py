@@ -54,7 +54,7 @@ def test_bad_args_rejected():
     # NULL accumulator / interpolator
     rc = lib.vpic_hip_advance_p(None, 0, -1.0, 1.0, None, 0,
                                 ctypes.byref(nm), None, None, None,
-                                ctypes.byref(g), None)
+                                ctypes.byref(g), None, None)
     assert rc < 0 and b"NULL" in lib.vpic_hip_last_error()
     # inconsistent strides
     bad = g.copy()

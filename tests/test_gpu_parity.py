@@ -56,18 +56,21 @@ def _inputs(orc, g, ppc, uth, seed, e_amp=0.05, b_amp=0.05, shuffle=False):
     return p, fi
 
 
-def _gpu_advance_p(dev, p, q, m, fi, g, max_nm=None, acc=None):
+def _gpu_advance_p(dev, p, q, m, fi, g, max_nm=None, acc=None,
+                   partition=None):
     if max_nm is None:
         max_nm = max(len(p), 16)
     dp = dev.DeviceArray.from_host(p)
     dfi = dev.DeviceArray.from_host(fi)
     da = dev.DeviceArray.from_host(acc if acc is not None else H.acc_array(g))
     dpm = dev.movers(max_nm)
+    dpart = (dev.DeviceArray.from_host(partition.astype(np.int32))
+             if partition is not None else None)
     nm = ctypes.c_int32()
     ign = ctypes.c_int32()
     _lib.call("vpic_hip_advance_p", dp.ptr, len(p), q, m, dpm.ptr, max_nm,
               ctypes.byref(nm), ctypes.byref(ign), da.ptr, dfi.ptr,
-              ctypes.byref(g), None)
+              ctypes.byref(g), dpart.ptr if dpart is not None else None, None)
     pm = dpm.download()[:nm.value].copy()
     return dp.download(), da.download(), nm.value, ign.value, pm
 
@@ -327,3 +330,79 @@ def test_loader_statistics(dev):
     _lib.call("vpic_hip_load_uniform_plasma", dq.ptr, ctypes.byref(g), ppc,
               0.1, 0.1, 0.1, 0.0, 0.0, 0.05, 2.0, 1234, None)
     assert H.same_bits(dq.download(), p)
+
+
+# ---------------------------------------------------------------------------
+# The partition-driven (tiled, LDS-accumulating) push
+# ---------------------------------------------------------------------------
+TILES = [(8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
+
+
+@pytest.fixture
+def tile_reset():
+    yield
+    _lib.call("vpic_hip_push_tile", 8, 8, 4)
+
+
+@pytest.mark.parametrize("tile", TILES, ids=["x".join(map(str, t)) for t in TILES])
+def test_tiled_push_parity(dev, orc, tile, tile_reset):
+    _lib.call("vpic_hip_push_tile", *tile)
+    g = _box(7, 5, 6)
+    for (q, m, uth, steps) in ((-1.0, 1.0, 0.1, 0), (-1.0, 1.0, 0.3, 4),
+                               (1.0, 25.0, 0.02, 2)):
+        p, fi = _inputs(orc, g, 11, uth, 31, shuffle=True)
+        hg = host_grid(g)
+        part = orc.sort_p(p, g)
+        # let the particles drift off their sort-time cells (stale partition)
+        for _ in range(steps):
+            orc.advance_p(p, q, m, H.acc_array(g), fi, hg)
+        po, ao = p.copy(), H.acc_array(g)
+        orc.advance_p(po, q, m, ao, fi, hg)
+        say("tiled advance_p tile", tile, "uth", uth, "drift steps", steps)
+        pg, ag, nmg, igg, _ = _gpu_advance_p(dev, p, q, m, fi, g,
+                                             partition=part)
+        assert H.same_bits(pg, po)
+        assert (nmg, igg) == (0, 0)
+        assert H.acc_rel_err(ag, ao) <= ACC_TOL
+
+
+def test_tiled_push_np_changed(dev, orc):
+    """Particles appended after the sort (pushed by the flat tail) and
+    particles removed (partition clamped to np)."""
+    g = _box(6, 5, 4)
+    p, fi = _inputs(orc, g, 9, 0.2, 41, shuffle=True)
+    part = orc.sort_p(p, g)
+    hg = host_grid(g)
+    extra = H.thermal_particles(g, 1, 0.2, 42, shuffle=True)[:97]
+    for sub in (np.concatenate([p, extra]), p[:len(p) - 123].copy()):
+        po, ao = sub.copy(), H.acc_array(g)
+        orc.advance_p(po, -1.0, 1.0, ao, fi, hg)
+        say("tiled advance_p with np", len(sub), "partition total", part[g.nv])
+        pg, ag, nmg, _, _ = _gpu_advance_p(dev, sub, -1.0, 1.0, fi, g,
+                                           partition=part)
+        assert H.same_bits(pg, po)
+        assert nmg == 0
+        assert H.acc_rel_err(ag, ao) <= ACC_TOL
+
+
+def test_tiled_push_walls_movers(dev, orc):
+    g = _box(4, 5, 6)
+    g.pbc[0] = g.pbc[3] = T.BC_REFLECT_PARTICLES
+    g.pbc[2] = g.pbc[5] = T.BC_ABSORB_PARTICLES
+    p, fi = _inputs(orc, g, 7, 0.9, 5, e_amp=0.2, b_amp=0.2, shuffle=True)
+    part = orc.sort_p(p, g)
+    hg = host_grid(g)
+    po, ao = p.copy(), H.acc_array(g)
+    nmo, _, pmo = orc.advance_p(po, -1.0, 1.0, ao, fi, hg, max_nm=len(p))
+    say("tiled advance_p walls/movers")
+    pg, ag, nmg, igg, pmg = _gpu_advance_p(dev, p, -1.0, 1.0, fi, g,
+                                           max_nm=len(p), partition=part)
+    assert (nmg, igg) == (nmo, 0) and nmo > 0
+    assert H.same_bits(pmg, pmo)
+    assert H.same_bits(pg, po)
+    assert H.acc_rel_err(ag, ao) <= ACC_TOL
+
+
+def test_tile_shape_rejected():
+    with pytest.raises(_lib.VpicError, match="rejected"):
+        _lib.call("vpic_hip_push_tile", 64, 64, 64)

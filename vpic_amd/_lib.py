@@ -26,9 +26,10 @@ _pf64 = ctypes.POINTER(ctypes.c_double)
 _SIGS = {
     "vpic_hip_init": [ctypes.c_int],
     "vpic_hip_advance_p": [_vp, _i32, _f32, _f32, _vp, _i32, _pi32, _pi32,
-                           _vp, _vp, _pg, _vp],
+                           _vp, _vp, _pg, _vp, _vp],
     "vpic_hip_advance_p_async": [_vp, _i32, _f32, _f32, _vp, _i32, _vp, _vp,
-                                 _vp, _pg, _vp],
+                                 _vp, _pg, _vp, _vp],
+    "vpic_hip_push_tile": [_i32, _i32, _i32],
     "vpic_hip_move_p": [_vp, _vp, _i32, _vp, _pg, _f32, _vp, _vp],
     "vpic_hip_sort_p": [_vp, _vp, _i32, _vp, _pg, ctypes.c_int, _vp],
     "vpic_hip_energy_p": [_vp, _i32, _f32, _f32, _vp, _pg, _pf64, _vp],

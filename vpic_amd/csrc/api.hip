@@ -195,18 +195,23 @@ DeviceScratch &nm_scratch() {
 }
 }  // namespace
 
-extern "C" int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np,
-                                        float q, float m,
-                                        vpic_particle_mover_t *pm,
-                                        int32_t max_nm, int32_t *dev_nm,
-                                        vpic_accumulator_t *a,
-                                        const vpic_interpolator_t *f,
-                                        const vpic_grid_t *g, void *stream) {
+extern "C" int vpic_hip_advance_p_async(
+    vpic_particle_t *p, int32_t np, float q, float m,
+    vpic_particle_mover_t *pm, int32_t max_nm, int32_t *dev_nm,
+    vpic_accumulator_t *a, const vpic_interpolator_t *f,
+    const vpic_grid_t *g, const int32_t *partition, void *stream) {
   int rc = check_push_args(p, np, pm, max_nm, a, f, g);
   if (rc) return rc;
   CHECK(dev_nm, "dev_nm is NULL");
-  return launch_advance_p(p, np, q, m, pm, max_nm, dev_nm, a, f, g,
+  return launch_advance_p(p, np, q, m, pm, max_nm, dev_nm, a, f, g, partition,
                           S(stream));
+}
+
+extern "C" int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz) {
+  CHECK(push_set_tile(tx, ty, tz) == VPIC_OK,
+        "push tile %dx%dx%d rejected (window must fit 96 KiB of LDS, "
+        "ty*tz <= 256)", tx, ty, tz);
+  return VPIC_OK;
 }
 
 extern "C" int vpic_hip_advance_p(vpic_particle_t *p, int32_t np, float q,
@@ -215,14 +220,15 @@ extern "C" int vpic_hip_advance_p(vpic_particle_t *p, int32_t np, float q,
                                   int32_t *host_n_ignored,
                                   vpic_accumulator_t *a,
                                   const vpic_interpolator_t *f,
-                                  const vpic_grid_t *g, void *stream) {
+                                  const vpic_grid_t *g,
+                                  const int32_t *partition, void *stream) {
   int rc = check_push_args(p, np, pm, max_nm, a, f, g);
   if (rc) return rc;
   CHECK(host_nm, "host_nm is NULL");
   DeviceScratch &ns = nm_scratch();
   if ((rc = scratch_reserve(ns, 64))) return rc;
   rc = launch_advance_p(p, np, q, m, pm, max_nm, (int *)ns.ptr, a, f, g,
-                        S(stream));
+                        partition, S(stream));
   if (rc) return rc;
   int32_t out[3];
   VPIC_HIP(hipMemcpyAsync(out, ns.ptr, sizeof(out), hipMemcpyDeviceToHost,

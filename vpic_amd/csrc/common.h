@@ -146,7 +146,9 @@ int scratch_reserve(DeviceScratch &s, size_t bytes);
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
-                     const vpic_grid_t *g, hipStream_t s);
+                     const vpic_grid_t *g, const int *partition,
+                     hipStream_t s);
+int push_set_tile(int tx, int ty, int tz);
 int launch_move_p_list(vpic_particle_t *p, vpic_particle_mover_t *pm, int nm,
                        vpic_accumulator_t *a, const vpic_grid_t *g, float q,
                        int *dev_ret, hipStream_t s);

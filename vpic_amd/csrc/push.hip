@@ -9,19 +9,26 @@
 // momenta and cell indices are bit-identical; only the order in which the
 // current contributions are summed differs (atomics).
 //
-// Launch structure of one advance_p:
-//   k_clear_counters   zero the crosser / mover counters (no memset, so the
-//                      sequence can be captured in a graph)
-//   k_advance_p        one lane per particle.  In-cell particles deposit;
-//                      lanes of a wavefront that hold the same voxel are
-//                      summed with a segmented wavefront scan and only the
-//                      last lane of each run issues the 12 atomics.  Cell
-//                      crossers store their new momentum and are appended to
-//                      a compact crosser list (ballot + one atomic per wave).
-//   k_move_p_list      one lane per crosser: the move_p face walk.
-//   k_order_movers     movers the grid cannot handle, sorted by particle
-//                      index, truncated at max_nm (excess rolled back to the
-//                      voxel index, advance_p_pipeline.cc:228-236).
+// Two push kernels share the per-particle code:
+//
+//  k_advance_p_tiled (used when the sort-time voxel partition is given):
+//    one workgroup per 3D tile of cells.  The tile's particles are the rows
+//    of the partition written by the last sort_p.  Currents go to an LDS
+//    window (the tile plus a one-cell halo, 48 B per cell), first summed
+//    over runs of equal voxels inside each wavefront, and the window is
+//    added to the global accumulator once at the end.  Cell crossers are
+//    queued in LDS and moved by the same workgroup, so their segments also
+//    land in the window.  Deposits outside the window (fast particles,
+//    periodic wrap) fall back to global atomics.
+//
+//  k_advance_p (any particle order, and the particles after the partition):
+//    one lane per particle, run-summed deposits straight to global atomics;
+//    crossers are compacted into a list (ballot + one atomic per wavefront)
+//    and moved by k_move_p_list.
+//
+// Both end with k_order_movers: movers the grid cannot handle, sorted by
+// particle index, truncated at max_nm (excess rolled back to the voxel
+// index, advance_p_pipeline.cc:228-236).
 #include "common.h"
 
 namespace {
@@ -30,9 +37,8 @@ struct PushCoef {
   float qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
 };
 
-// The twelve accumulator updates of advance_p_pipeline.cc:186-208 /
-// move_p.cc:281-303 for one axis: out[0..3] receive the four quarter-face
-// contributions.
+// The four quarter-face updates of one axis, advance_p_pipeline.cc:186-208
+// and move_p.cc:281-303 (identical arithmetic).
 __device__ __forceinline__ void streak4(float out[4], float qd, float mY,
                                         float mZ, float v5) {
   float v0, v1, v2, v3, v4;
@@ -56,19 +62,58 @@ __device__ __forceinline__ void streak4(float out[4], float qd, float mY,
   out[3] = v3;
 }
 
-__device__ __forceinline__ void deposit_atomic(float *acc, int voxel,
-                                               const float j[12]) {
-  float *a = acc + 12 * (int64_t)voxel;
+// --------------------------------------------------------------------------
+// Deposit sinks
+// --------------------------------------------------------------------------
+struct GlobalSink {
+  float *acc;
+  __device__ __forceinline__ void deposit(int voxel, const float j[12]) const {
+    float *a = acc + 12 * (int64_t)voxel;
 #pragma unroll
-  for (int k = 0; k < 12; k++) atomic_add_f32(a + k, j[k]);
-}
+    for (int k = 0; k < 12; k++) atomic_add_f32(a + k, j[k]);
+  }
+};
 
-// move_p (move_p.cc:215-378).  The particle lives in registers for the
-// walk; `mv` is the mover (remaining displacement, particle index).
-// Returns 1 when the particle hit a face the grid cannot handle; its voxel
-// index is then 8*voxel + face.
+// A box of voxels [w0, w0+wn) per axis held in LDS, 12 floats per cell.
+struct WindowSink {
+  float *lds;  // window cells * 12 (LDS)
+  float *acc;  // global accumulator
+  int wx0, wy0, wz0, wnx, wny, wnz;
+  int sx1, sy1;  // nx+2, ny+2
+  __device__ __forceinline__ int slot(int v) const {
+    const int x = v % sx1;
+    const int t = v / sx1;
+    const int y = t % sy1 - wy0;
+    const int z = t / sy1 - wz0;
+    const int xx = x - wx0;
+    if ((unsigned)xx < (unsigned)wnx && (unsigned)y < (unsigned)wny &&
+        (unsigned)z < (unsigned)wnz)
+      return xx + wnx * (y + wny * z);
+    return -1;
+  }
+  __device__ __forceinline__ void deposit(int voxel, const float j[12]) const {
+    const int s = slot(voxel);
+    if (s >= 0) {
+      float *a = lds + 12 * s;
+#pragma unroll
+      for (int k = 0; k < 12; k++) atomicAdd(a + k, j[k]);
+    } else {
+      float *a = acc + 12 * (int64_t)voxel;
+#pragma unroll
+      for (int k = 0; k < 12; k++) atomic_add_f32(a + k, j[k]);
+    }
+  }
+};
+
+// --------------------------------------------------------------------------
+// move_p (move_p.cc:215-378).  The particle lives in registers for the walk;
+// `mv` is the mover (remaining displacement, particle index).  Returns 1
+// when the particle hit a face the grid cannot handle; its voxel index is
+// then 8*voxel + face.
+// --------------------------------------------------------------------------
+template <class Sink>
 __device__ int dev_move_p(vpic_particle_t *__restrict__ p0,
-                          vpic_particle_mover_t &mv, float *__restrict__ acc,
+                          vpic_particle_mover_t &mv, const Sink &sink,
                           const GridDev &G, float qsp) {
   float4 *pp = reinterpret_cast<float4 *>(p0 + mv.i);
   float4 A = pp[0], B = pp[1];
@@ -105,7 +150,7 @@ __device__ int dev_move_p(vpic_particle_t *__restrict__ p0,
     streak4(j + 0, q * s_disp[0], s_mid[1], s_mid[2], v5);
     streak4(j + 4, q * s_disp[1], s_mid[2], s_mid[0], v5);
     streak4(j + 8, q * s_disp[2], s_mid[0], s_mid[1], v5);
-    deposit_atomic(acc, pi, j);
+    sink.deposit(pi, j);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       md[k] -= s_disp[k];
@@ -156,19 +201,27 @@ __device__ __forceinline__ void push_returned_mover(
   }
 }
 
-__global__ __launch_bounds__(256) void k_advance_p(
-    vpic_particle_t *__restrict__ p, int np,
-    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
-    vpic_particle_mover_t *__restrict__ xlist, int xcap,
-    vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt,
-    PushCoef C, GridDev G) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = lane_id();
-  const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
+// --------------------------------------------------------------------------
+// The per-particle push (advance_p_pipeline.cc:91-211).  Loads particle n,
+// stores its new momentum (and position when it stays in the cell), and
+// returns the 12 current contributions (zero for crossers) and, for a
+// crosser, its half displacement (the mover of :215-219).
+// --------------------------------------------------------------------------
+struct PushResult {
+  bool active, cross;
+  int voxel;
+  float j[12];
+  float dispx, dispy, dispz;
+};
 
+__device__ __forceinline__ void push_particle(
+    vpic_particle_t *__restrict__ p, int n, bool valid_slot,
+    const vpic_interpolator_t *__restrict__ f0, const PushCoef &C, int nv,
+    int *__restrict__ nbad, PushResult &R) {
+  const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
   float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
-  float4 *pp = reinterpret_cast<float4 *>(p) + 2 * (size_t)(n < np ? n : 0);
-  if (n < np) {
+  float4 *pp = reinterpret_cast<float4 *>(p) + 2 * (size_t)(valid_slot ? n : 0);
+  if (valid_slot) {
     A = pp[0];
     B = pp[1];
   }
@@ -176,8 +229,8 @@ __global__ __launch_bounds__(256) void k_advance_p(
   // 8*voxel+face) is never dereferenced: the particle is left untouched and
   // counted, and the call reports an error.
   const int iraw = __float_as_int(A.w);
-  const bool active = n < np && (unsigned)iraw < (unsigned)G.nv;
-  if (n < np && !active) atomicAdd(cnt + 3, 1);
+  const bool active = valid_slot && (unsigned)iraw < (unsigned)nv;
+  if (valid_slot && !active) atomicAdd(nbad, 1);
   float dx = A.x, dy = A.y, dz = A.z;
   const int ii = active ? iraw : 0;
 
@@ -236,7 +289,12 @@ __global__ __launch_bounds__(256) void k_advance_p(
   const bool inbnds = v3 <= one && v4 <= one && v5 <= one && -v3 <= one &&
                       -v4 <= one && -v5 <= one;
 
-  float j[12];
+  R.active = active;
+  R.voxel = ii;
+  R.cross = active && !inbnds;
+  R.dispx = ux;
+  R.dispy = uy;
+  R.dispz = uz;
   if (active && inbnds) {
     q *= C.qsp;
     pp[0] = make_float4(v3, v4, v5, A.w);
@@ -245,20 +303,21 @@ __global__ __launch_bounds__(256) void k_advance_p(
     dy = v1;
     dz = v2;
     v5 = q * ux * uy * uz * one_third;
-    streak4(j + 0, q * ux, dy, dz, v5);
-    streak4(j + 4, q * uy, dz, dx, v5);
-    streak4(j + 8, q * uz, dx, dy, v5);
+    streak4(R.j + 0, q * ux, dy, dz, v5);
+    streak4(R.j + 4, q * uy, dz, dx, v5);
+    streak4(R.j + 8, q * uz, dx, dy, v5);
   } else {
 #pragma unroll
-    for (int k = 0; k < 12; k++) j[k] = 0.0f;
+    for (int k = 0; k < 12; k++) R.j[k] = 0.0f;
     if (active) pp[1] = Bnew;  // momentum stored, position left for move_p
   }
+}
 
-  // Run-segmented inclusive scan over the wavefront: consecutive lanes
-  // with the same voxel form one run; the last lane of a run ends up with
-  // the run's sum.  Crossers contribute zeros and keep their voxel as key
-  // so they do not split runs.
-  const int key = active ? ii : -1;
+// Run-segmented inclusive scan over the wavefront: consecutive lanes with
+// the same key form one run; the last lane of each run ends with the run's
+// sum (returned true).  Lanes with key < 0 take part in no run's deposit.
+__device__ __forceinline__ bool wave_run_sum(int key, float j[12]) {
+  const int lane = lane_id();
   const int kprev = __shfl_up(key, 1);
   const bool head = (lane == 0) || (kprev != key);
   const uint64_t heads = __ballot(head);
@@ -275,28 +334,51 @@ __global__ __launch_bounds__(256) void k_advance_p(
   }
   const int knext = __shfl_down(key, 1);
   const bool tail = (lane == 63) || (knext != key);
-  if (tail && key >= 0) deposit_atomic(acc, key, j);
+  return tail && key >= 0;
+}
 
-  // Cell crossers: compact into the crosser list (ballot + one atomic per
-  // wavefront); the rare overflow of the list is moved right here.
-  const bool cross = active && !inbnds;
-  const uint64_t cm = __ballot(cross);
-  if (cm) {
-    const int leader = __ffsll((unsigned long long)cm) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(cnt, __popcll(cm));
-    base = __shfl(base, leader);
-    if (cross) {
-      const int slot = base + __popcll(cm & ((1ull << lane) - 1ull));
-      vpic_particle_mover_t mv;
-      mv.dispx = ux;
-      mv.dispy = uy;
-      mv.dispz = uz;
-      mv.i = n;
-      if (slot < xcap) {
-        xlist[slot] = mv;
-      } else if (dev_move_p(p, mv, acc, G, C.qsp)) {
-        push_returned_mover(p, mv, rlist, xcap, cnt);
+// --------------------------------------------------------------------------
+// Flat kernel: particles [n0, np), any order, grid-stride over whole
+// wavefronts.  n0 = *start clamped to [0,np] (the end of the sort-time
+// partition when the tiled kernel ran first), or 0 when start is NULL.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advance_p(
+    vpic_particle_t *__restrict__ p, const int *__restrict__ start, int np,
+    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
+    vpic_particle_mover_t *__restrict__ xlist, int xcap,
+    vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt,
+    PushCoef C, GridDev G) {
+  const int n0 = start ? min(max(*start, 0), np) : 0;
+  const int lane = lane_id();
+  const GlobalSink sink{acc};
+  for (int64_t nn = n0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       nn - threadIdx.x % VPIC_WAVE < np;
+       nn += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)nn;
+    PushResult R;
+    push_particle(p, n, nn < np, f0, C, G.nv, cnt + 3, R);
+    if (wave_run_sum(R.active ? R.voxel : -1, R.j)) sink.deposit(R.voxel, R.j);
+
+    // Cell crossers: compact into the crosser list (ballot + one atomic per
+    // wavefront); the rare overflow of the list is moved right here.
+    const uint64_t cm = __ballot(R.cross);
+    if (cm) {
+      const int leader = __ffsll((unsigned long long)cm) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(cnt, __popcll(cm));
+      base = __shfl(base, leader);
+      if (R.cross) {
+        const int slot = base + __popcll(cm & ((1ull << lane) - 1ull));
+        vpic_particle_mover_t mv;
+        mv.dispx = R.dispx;
+        mv.dispy = R.dispy;
+        mv.dispz = R.dispz;
+        mv.i = n;
+        if (slot < xcap) {
+          xlist[slot] = mv;
+        } else if (dev_move_p(p, mv, sink, G, C.qsp)) {
+          push_returned_mover(p, mv, rlist, xcap, cnt);
+        }
       }
     }
   }
@@ -307,13 +389,152 @@ __global__ __launch_bounds__(256) void k_move_p_list(
     int xcap, vpic_particle_mover_t *__restrict__ rlist,
     float *__restrict__ acc, int *__restrict__ cnt, float qsp, GridDev G) {
   const int n = min(cnt[0], xcap);
+  const GlobalSink sink{acc};
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += gridDim.x * blockDim.x) {
     vpic_particle_mover_t mv = xlist[k];
-    if (dev_move_p(p, mv, acc, G, qsp))
+    if (dev_move_p(p, mv, sink, G, qsp))
       push_returned_mover(p, mv, rlist, xcap, cnt);
   }
 }
+
+// --------------------------------------------------------------------------
+// Tiled kernel
+// --------------------------------------------------------------------------
+struct TileGeom {
+  int tx, ty, tz;     // tile extent in cells
+  int ntx, nty, ntz;  // tiles per axis
+};
+
+constexpr int TILE_BS = 512;           // threads per workgroup
+constexpr int TILE_MAXROWS = 256;      // ty * tz
+constexpr int TILE_QCAP = 2 * TILE_BS;  // crosser queue
+
+__global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
+    vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
+    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
+    vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
+    PushCoef C, GridDev G, TileGeom T) {
+  extern __shared__ float lds_acc[];
+  __shared__ vpic_particle_mover_t q[TILE_QCAP];
+  __shared__ int row_start[TILE_MAXROWS];
+  __shared__ int row_pref[TILE_MAXROWS + 1];
+  __shared__ int qn;
+
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  int b = blockIdx.x;
+  const int btx = b % T.ntx;
+  b /= T.ntx;
+  const int bty = b % T.nty;
+  const int btz = b / T.nty;
+  const int x0 = 1 + btx * T.tx, y0 = 1 + bty * T.ty, z0 = 1 + btz * T.tz;
+  const int x1 = min(x0 + T.tx - 1, G.nx), y1 = min(y0 + T.ty - 1, G.ny),
+            z1 = min(z0 + T.tz - 1, G.nz);
+  const int rows_y = y1 - y0 + 1;
+  const int nrow = rows_y * (z1 - z0 + 1);
+
+  WindowSink W;
+  W.lds = lds_acc;
+  W.acc = acc;
+  W.wx0 = x0 - 1;
+  W.wy0 = y0 - 1;
+  W.wz0 = z0 - 1;
+  W.wnx = x1 - x0 + 3;
+  W.wny = y1 - y0 + 3;
+  W.wnz = z1 - z0 + 3;
+  W.sx1 = G.nx + 2;
+  W.sy1 = G.ny + 2;
+  const int ncell = W.wnx * W.wny * W.wnz;
+
+  // Rows of the tile in the sort-time partition, clamped to [0, np).
+  for (int r = t; r < nrow; r += TILE_BS) {
+    const int y = y0 + r % rows_y, z = z0 + r / rows_y;
+    int s = part[vox(x0, y, z, G.nx, G.ny)];
+    int e = part[vox(x1 + 1, y, z, G.nx, G.ny)];
+    s = min(max(s, 0), np);
+    e = min(max(e, s), np);
+    row_start[r] = s;
+    row_pref[r + 1] = e - s;
+  }
+  for (int k = t; k < 12 * ncell; k += TILE_BS) lds_acc[k] = 0.0f;
+  if (t == 0) qn = 0;
+  __syncthreads();
+  if (t == 0) {
+    row_pref[0] = 0;
+    for (int r = 0; r < nrow; r++) row_pref[r + 1] += row_pref[r];
+  }
+  __syncthreads();
+  const int total = row_pref[nrow];
+
+  for (int base = 0; base < total; base += TILE_BS) {
+    const int k = base + t;
+    const bool slot_ok = k < total;
+    int n = 0;
+    if (slot_ok) {
+      int lo = 0, hi = nrow - 1;  // last r with row_pref[r] <= k
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_pref[mid] <= k) lo = mid; else hi = mid - 1;
+      }
+      n = row_start[lo] + (k - row_pref[lo]);
+    }
+    PushResult R;
+    push_particle(p, n, slot_ok, f0, C, G.nv, cnt + 3, R);
+    if (wave_run_sum(R.active ? R.voxel : -1, R.j)) W.deposit(R.voxel, R.j);
+
+    // queue this round's crossers
+    const uint64_t cm = __ballot(R.cross);
+    if (cm) {
+      const int leader = __ffsll((unsigned long long)cm) - 1;
+      int qb = 0;
+      if (lane == leader) qb = atomicAdd(&qn, __popcll(cm));
+      qb = __shfl(qb, leader);
+      if (R.cross) {
+        vpic_particle_mover_t mv;
+        mv.dispx = R.dispx;
+        mv.dispy = R.dispy;
+        mv.dispz = R.dispz;
+        mv.i = n;
+        q[qb + __popcll(cm & ((1ull << lane) - 1ull))] = mv;
+      }
+    }
+    __syncthreads();
+    const int nq = qn;
+    __syncthreads();  // every lane has read qn before anyone changes it
+    if (nq >= TILE_BS) {  // uniform: move the top TILE_BS queued crossers
+      vpic_particle_mover_t mv = q[nq - TILE_BS + t];
+      if (t == 0) qn = nq - TILE_BS;
+      if (dev_move_p(p, mv, W, G, C.qsp))
+        push_returned_mover(p, mv, rlist, rcap, cnt);
+      __syncthreads();
+    }
+  }
+  {
+    const int nq = qn;
+    if (t < nq) {
+      vpic_particle_mover_t mv = q[t];
+      if (dev_move_p(p, mv, W, G, C.qsp))
+        push_returned_mover(p, mv, rlist, rcap, cnt);
+    }
+  }
+  __syncthreads();
+  // Add the window to the global accumulator (untouched cells skipped).
+  for (int k = t; k < 12 * ncell; k += TILE_BS) {
+    const float v = lds_acc[k];
+    if (v != 0.0f) {
+      const int c = k / 12, comp = k - 12 * c;
+      const int cx = c % W.wnx, cr = c / W.wnx;
+      const int cy = cr % W.wny, cz = cr / W.wny;
+      const int vv = vox(W.wx0 + cx, W.wy0 + cy, W.wz0 + cz, G.nx, G.ny);
+      atomic_add_f32(acc + 12 * (int64_t)vv + comp, v);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Returned movers in particle order
+// --------------------------------------------------------------------------
 
 // Order the returned movers by particle index (bitonic sort of
 // (index, slot) pairs in global memory by one workgroup), keep the first
@@ -345,9 +566,9 @@ __global__ __launch_bounds__(1024) void k_order_movers(
         const int l = k ^ stride;
         if (l > k) {
           const bool up = (k & size) == 0;
-          const int2 a = buf[k], b = buf[l];
-          if ((a.x > b.x) == up) {
-            buf[k] = b;
+          const int2 a = buf[k], bb = buf[l];
+          if ((a.x > bb.x) == up) {
+            buf[k] = bb;
             buf[l] = a;
           }
         }
@@ -376,9 +597,10 @@ __global__ void k_move_p_serial(vpic_particle_t *p, vpic_particle_mover_t *pm,
                                 int nm, float *acc, GridDev G, float q,
                                 int *ret) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const GlobalSink sink{acc};
   for (int k = 0; k < nm; k++) {
     vpic_particle_mover_t mv = pm[k];
-    ret[k] = dev_move_p(p, mv, acc, G, q);
+    ret[k] = dev_move_p(p, mv, sink, G, q);
     pm[k] = mv;
   }
 }
@@ -402,12 +624,27 @@ int pow2_at_least(int64_t n) {
   return (int)p;
 }
 
+TileGeom g_tile = {8, 8, 4, 0, 0, 0};
+
 }  // namespace
+
+// Tile shape of the tiled push (set by the tuner; the window of a tile is
+// (tx+2)(ty+2)(tz+2) cells of 48 B in LDS).
+int push_set_tile(int tx, int ty, int tz) {
+  if (tx < 1 || ty < 1 || tz < 1 || ty * tz > TILE_MAXROWS) return VPIC_EINVAL;
+  if ((size_t)(tx + 2) * (ty + 2) * (tz + 2) * 48 > 96 * 1024)
+    return VPIC_EINVAL;
+  g_tile.tx = tx;
+  g_tile.ty = ty;
+  g_tile.tz = tz;
+  return VPIC_OK;
+}
 
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
-                     const vpic_grid_t *g, hipStream_t s) {
+                     const vpic_grid_t *g, const int *partition,
+                     hipStream_t s) {
   PushScratch &S = push_scratch();
   // Crosser list sized for ~20% of the particles per step; the rare
   // excess is moved inline by the push kernel.
@@ -439,10 +676,37 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
   if (np > 0) {
     const bool timed = push_timing_on();
     if (timed) push_timing_record(s, np, true);
-    const int blocks = (int)(((int64_t)np + 255) / 256);
-    hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p, np, f,
-                       (float *)a, xlist, xcap, rlist, cnt, C, G);
-    VPIC_LAUNCHED("k_advance_p");
+    const int *flat_start = nullptr;
+    if (partition) {
+      TileGeom T = g_tile;
+      T.ntx = (g->nx + T.tx - 1) / T.tx;
+      T.nty = (g->ny + T.ty - 1) / T.ty;
+      T.ntz = (g->nz + T.tz - 1) / T.tz;
+      const size_t lds = sizeof(float) * 12 * (size_t)(T.tx + 2) *
+                         (T.ty + 2) * (T.tz + 2);
+      static bool attr_set = false;
+      if (!attr_set) {
+        VPIC_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(k_advance_p_tiled),
+            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+        attr_set = true;
+      }
+      const int nblk = T.ntx * T.nty * T.ntz;
+      hipLaunchKernelGGL(k_advance_p_tiled, dim3(nblk), dim3(TILE_BS), lds, s,
+                         p, np, partition, f, (float *)a, rlist, xcap, cnt, C,
+                         G, T);
+      VPIC_LAUNCHED("k_advance_p_tiled");
+      // particles past the partition (added since the sort) go flat
+      flat_start = partition + g->nv;
+    }
+    {
+      int64_t want = ((int64_t)np + 255) / 256;
+      const int blocks = (int)(want < 16384 ? want : 16384);
+      hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p,
+                         flat_start, np, f, (float *)a, xlist, xcap, rlist,
+                         cnt, C, G);
+      VPIC_LAUNCHED("k_advance_p");
+    }
     if (timed) push_timing_record(s, np, false);
     int mblocks = (xcap + 255) / 256;
     if (mblocks > 8192) mblocks = 8192;

@@ -78,9 +78,11 @@ class Simulation:
         sp.last_sorted = self.step
 
     def advance_p(self, sp):
+        # after a sort the partition drives the tiled (LDS) push
+        part = sp.partition.ptr if sp.last_sorted >= 0 else None
         _lib.call("vpic_hip_advance_p_async", sp.p.ptr, sp.np, sp.q, sp.m,
                   sp.pm.ptr, sp.max_nm, self.dev_nm.ptr, self.acc.ptr,
-                  self.interp.ptr, self._G(), self.stream)
+                  self.interp.ptr, self._G(), part, self.stream)
 
     def load_interpolator_array(self):
         _lib.call("vpic_hip_load_interpolator_array", self.interp.ptr,
