@@ -124,6 +124,16 @@ def main():
     ap.add_argument("--ppc", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--tile", type=str, default=None,
+                    help="push tile tx,ty,tz (cells)")
+    ap.add_argument("--direct-runs", type=int, default=None,
+                    help="push deposit threshold (runs per wavefront)")
+    ap.add_argument("--sweep", type=str, default=None,
+                    help="tune: 'tx,ty,tz:runs;...' each timed from a fresh "
+                         "load (prints one JSON line per candidate)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 4 GiB device-to-device copy three times "
+                         "(calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the hot path on a "
@@ -137,6 +147,48 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     _lib.call("vpic_hip_init", 0)
+    if args.copy_probe:
+        n = 1 << 30  # floats = 4 GiB
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)  # an elementwise read+write kernel
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n,
+                          "copy_probe_bytes_written": 4 * n}))
+        return
+    if args.sweep:
+        for cand in args.sweep.split(";"):
+            tile, runs = cand.split(":")
+            _lib.call("vpic_hip_push_tile", *[int(v) for v in tile.split(",")])
+            _lib.call("vpic_hip_push_direct_runs", int(runs))
+            s = build_sim(args.nx, args.ny, args.nz, args.ppc)
+            for _ in range(args.warmup):
+                s.advance()
+            _lib.call("vpic_hip_push_timing", None, None, None)
+            _lib.call("vpic_hip_push_timing_enable", 1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                s.advance()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            _lib.call("vpic_hip_push_timing_enable", 0)
+            launches, push_ms = ctypes.c_int64(), ctypes.c_double()
+            _lib.call("vpic_hip_push_timing", ctypes.byref(launches),
+                      ctypes.byref(push_ms), None)
+            npart = sum(sp.np for sp in s.species)
+            print(json.dumps({"tile": tile, "direct_runs": int(runs),
+                              "ms_per_step": el * 1e3 / args.steps,
+                              "push_ms": push_ms.value / launches.value,
+                              "value": npart * args.steps / el}), flush=True)
+            del s
+            torch.cuda.empty_cache()
+        return
+    if args.tile:
+        _lib.call("vpic_hip_push_tile", *[int(v) for v in args.tile.split(",")])
+    if args.direct_runs:
+        _lib.call("vpic_hip_push_direct_runs", args.direct_runs)
     log("building %dx%dx%d, %d ppc/species" % (args.nx, args.ny, args.nz,
                                                args.ppc))
     s = build_sim(args.nx, args.ny, args.nz, args.ppc)

@@ -174,6 +174,11 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
  * counterpart (the reference's pipelines split particles, not cells). */
 int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
 
+/* Wavefronts whose 64 particles fall in at least `runs` runs of equal
+ * voxels deposit lane by lane instead of summing runs first (default 24;
+ * 65 disables it).  A tuning knob; no reference counterpart. */
+int vpic_hip_push_direct_runs(int32_t runs);
+
 /* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),
  * applied to the nm movers pm[0..nm) in order on the device (one lane, as
  * the reference's callers do, e.g. vpic/misc.cc:95).  `dev_ret` receives

@@ -207,6 +207,12 @@ extern "C" int vpic_hip_advance_p_async(
                           S(stream));
 }
 
+extern "C" int vpic_hip_push_direct_runs(int32_t runs) {
+  CHECK(runs >= 2 && runs <= 65, "direct-deposit run threshold %d outside "
+        "[2,65]", runs);
+  return push_set_direct_runs(runs);
+}
+
 extern "C" int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz) {
   CHECK(push_set_tile(tx, ty, tz) == VPIC_OK,
         "push tile %dx%dx%d rejected (window must fit 96 KiB of LDS, "

@@ -14,17 +14,22 @@
 //  k_advance_p_tiled (used when the sort-time voxel partition is given):
 //    one workgroup per 3D tile of cells.  The tile's particles are the rows
 //    of the partition written by the last sort_p.  Currents go to an LDS
-//    window (the tile plus a one-cell halo, 48 B per cell), first summed
-//    over runs of equal voxels inside each wavefront, and the window is
+//    window (the tile plus a one-cell halo, 48 B per cell) and the window is
 //    added to the global accumulator once at the end.  Cell crossers are
 //    queued in LDS and moved by the same workgroup, so their segments also
 //    land in the window.  Deposits outside the window (fast particles,
 //    periodic wrap) fall back to global atomics.
 //
 //  k_advance_p (any particle order, and the particles after the partition):
-//    one lane per particle, run-summed deposits straight to global atomics;
+//    grid-stride over the particles, deposits straight to global atomics;
 //    crossers are compacted into a list (ballot + one atomic per wavefront)
 //    and moved by k_move_p_list.
+//
+// Deposits of a wavefront (wave_deposit) take one of three forms, chosen by
+// the number of runs of equal voxels among its 64 lanes: one run -> a plain
+// DPP reduction and one deposit; many short runs -> every lane deposits its
+// own 12 values; otherwise a run-segmented DPP scan and one deposit per run.
+// Both kernels prefetch the next particle while pushing the current one.
 //
 // Both end with k_order_movers: movers the grid cannot handle, sorted by
 // particle index, truncated at max_nm (excess rolled back to the voxel
@@ -74,21 +79,34 @@ struct GlobalSink {
   }
 };
 
+// q = v / d for v * d < 2^32 with m = ceil(2^32 / d) (exact: the rounding
+// excess v*(m*d - 2^32) / 2^32 stays below 1/d).
+__device__ __forceinline__ unsigned fast_div(unsigned v, unsigned m) {
+  return __umulhi(v, m);
+}
+
 // A box of voxels [w0, w0+wn) per axis held in LDS, 12 floats per cell.
 struct WindowSink {
   float *lds;  // window cells * 12 (LDS)
   float *acc;  // global accumulator
   int wx0, wy0, wz0, wnx, wny, wnz;
-  int sx1, sy1;  // nx+2, ny+2
+  int sx1, sy1;     // nx+2, ny+2
+  unsigned mx, my;  // fast_div multipliers for sx1, sy1 (0: divide)
   __device__ __forceinline__ int slot(int v) const {
-    const int x = v % sx1;
-    const int t = v / sx1;
-    const int y = t % sy1 - wy0;
-    const int z = t / sy1 - wz0;
-    const int xx = x - wx0;
-    if ((unsigned)xx < (unsigned)wnx && (unsigned)y < (unsigned)wny &&
-        (unsigned)z < (unsigned)wnz)
-      return xx + wnx * (y + wny * z);
+    unsigned t, z;
+    if (mx) {
+      t = fast_div((unsigned)v, mx);
+      z = fast_div(t, my);
+    } else {
+      t = (unsigned)v / (unsigned)sx1;
+      z = t / (unsigned)sy1;
+    }
+    const int x = v - (int)t * sx1 - wx0;
+    const int y = (int)t - (int)z * sy1 - wy0;
+    const int zz = (int)z - wz0;
+    if ((unsigned)x < (unsigned)wnx && (unsigned)y < (unsigned)wny &&
+        (unsigned)zz < (unsigned)wnz)
+      return x + wnx * (y + wny * zz);
     return -1;
   }
   __device__ __forceinline__ void deposit(int voxel, const float j[12]) const {
@@ -202,10 +220,12 @@ __device__ __forceinline__ void push_returned_mover(
 }
 
 // --------------------------------------------------------------------------
-// The per-particle push (advance_p_pipeline.cc:91-211).  Loads particle n,
-// stores its new momentum (and position when it stays in the cell), and
-// returns the 12 current contributions (zero for crossers) and, for a
-// crosser, its half displacement (the mover of :215-219).
+// The per-particle push (advance_p_pipeline.cc:91-211), split into the load
+// (so the next particle's load can be issued early) and the compute.  The
+// compute stores the new momentum (and the new position when the particle
+// stays in its cell) and returns the 12 current contributions (zero for
+// crossers) and, for a crosser, its half displacement (the mover of
+// :215-219).
 // --------------------------------------------------------------------------
 struct PushResult {
   bool active, cross;
@@ -214,17 +234,22 @@ struct PushResult {
   float dispx, dispy, dispz;
 };
 
-__device__ __forceinline__ void push_particle(
-    vpic_particle_t *__restrict__ p, int n, bool valid_slot,
+// Slots without a particle read particle 0 (every launch has np >= 1) and
+// are masked by `valid` in push_compute, so the load is branch-free.
+__device__ __forceinline__ void load_particle(const vpic_particle_t *p, int n,
+                                              bool valid, float4 &A,
+                                              float4 &B) {
+  const float4 *pp = reinterpret_cast<const float4 *>(p + (valid ? n : 0));
+  A = pp[0];
+  B = pp[1];
+}
+
+__device__ __forceinline__ void push_compute(
+    const float4 A, const float4 B, bool valid_slot,
+    vpic_particle_t *__restrict__ p, int n,
     const vpic_interpolator_t *__restrict__ f0, const PushCoef &C, int nv,
     int *__restrict__ nbad, PushResult &R) {
   const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
-  float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
-  float4 *pp = reinterpret_cast<float4 *>(p) + 2 * (size_t)(valid_slot ? n : 0);
-  if (valid_slot) {
-    A = pp[0];
-    B = pp[1];
-  }
   // A voxel index outside the grid (e.g. a mover left pending as
   // 8*voxel+face) is never dereferenced: the particle is left untouched and
   // counted, and the call reports an error.
@@ -289,6 +314,7 @@ __device__ __forceinline__ void push_particle(
   const bool inbnds = v3 <= one && v4 <= one && v5 <= one && -v3 <= one &&
                       -v4 <= one && -v5 <= one;
 
+  float4 *pp = reinterpret_cast<float4 *>(p + (active ? n : 0));
   R.active = active;
   R.voxel = ii;
   R.cross = active && !inbnds;
@@ -313,28 +339,77 @@ __device__ __forceinline__ void push_particle(
   }
 }
 
-// Run-segmented inclusive scan over the wavefront: consecutive lanes with
-// the same key form one run; the last lane of each run ends with the run's
-// sum (returned true).  Lanes with key < 0 take part in no run's deposit.
-__device__ __forceinline__ bool wave_run_sum(int key, float j[12]) {
+// --------------------------------------------------------------------------
+// Wavefront deposit
+// --------------------------------------------------------------------------
+
+// DPP move of a float across lanes (v_mov_b32_dpp); lanes without a valid
+// source (outside the row, or rows masked off) read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      0, __float_as_int(v), CTRL, ROW_MASK, 0xf, true));
+}
+
+// Wavefronts with at least this many runs of equal voxels deposit lane by
+// lane instead of scanning.
+__constant__ int c_direct_runs = 24;
+
+// Sum the 12 contributions of the lanes sharing a voxel and deposit them.
+// key < 0 marks lanes with nothing to deposit.
+template <class Sink>
+__device__ __forceinline__ void wave_deposit(int key, float j[12],
+                                             const Sink &sink) {
   const int lane = lane_id();
   const int kprev = __shfl_up(key, 1);
   const bool head = (lane == 0) || (kprev != key);
   const uint64_t heads = __ballot(head);
-  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const int seg_start = 63 - __clzll(heads & upto);
-#pragma unroll
-  for (int d = 1; d < VPIC_WAVE; d <<= 1) {
-    const bool take = (lane - d) >= seg_start;
+  const int runs = __popcll(heads);
+  bool dep;
+  if (runs == 1) {
+    // one run over the whole wavefront: plain scan, lane 63 holds the sum
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-      const float t = __shfl_up(j[k], d);
-      if (take) j[k] += t;
+      float v = j[k];
+      v += dpp_f32<0x111, 0xf>(v);
+      v += dpp_f32<0x112, 0xf>(v);
+      v += dpp_f32<0x114, 0xf>(v);
+      v += dpp_f32<0x118, 0xf>(v);
+      v += dpp_f32<0x142, 0xa>(v);
+      v += dpp_f32<0x143, 0xc>(v);
+      j[k] = v;
     }
+    dep = lane == 63;
+  } else if (runs >= c_direct_runs) {
+    dep = true;
+  } else {
+    // Run-segmented inclusive scan: Hillis-Steele steps of 1,2,4,8 lanes
+    // inside each 16-lane row (DPP row_shr), then the row carries (DPP
+    // row_bcast:15 to rows 1 and 3, row_bcast:31 to rows 2 and 3); a step
+    // adds only when the source lane lies in the same run.  The last lane of
+    // each run ends with the run's sum.
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int seg_start = 63 - __clzll(heads & upto);
+    const int r = lane & 15;
+#define VPIC_SCAN_STEP(CTRL, RMASK, TAKE)            \
+  {                                                  \
+    const bool take_ = (TAKE);                       \
+    _Pragma("unroll") for (int k = 0; k < 12; k++) { \
+      const float t_ = dpp_f32<CTRL, RMASK>(j[k]);   \
+      j[k] += take_ ? t_ : 0.0f;                     \
+    }                                                \
   }
-  const int knext = __shfl_down(key, 1);
-  const bool tail = (lane == 63) || (knext != key);
-  return tail && key >= 0;
+    VPIC_SCAN_STEP(0x111, 0xf, r >= 1 && lane - 1 >= seg_start)
+    VPIC_SCAN_STEP(0x112, 0xf, r >= 2 && lane - 2 >= seg_start)
+    VPIC_SCAN_STEP(0x114, 0xf, r >= 4 && lane - 4 >= seg_start)
+    VPIC_SCAN_STEP(0x118, 0xf, r >= 8 && lane - 8 >= seg_start)
+    VPIC_SCAN_STEP(0x142, 0xa,
+                   ((lane >> 4) & 1) && seg_start <= (lane & ~15) - 1)
+    VPIC_SCAN_STEP(0x143, 0xc, lane >= 32 && seg_start <= 31)
+#undef VPIC_SCAN_STEP
+    dep = (lane == 63) || (((heads >> lane) >> 1) & 1ull);
+  }
+  if (dep && key >= 0) sink.deposit(key, j);
 }
 
 // --------------------------------------------------------------------------
@@ -351,13 +426,18 @@ __global__ __launch_bounds__(256) void k_advance_p(
   const int n0 = start ? min(max(*start, 0), np) : 0;
   const int lane = lane_id();
   const GlobalSink sink{acc};
-  for (int64_t nn = n0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-       nn - threadIdx.x % VPIC_WAVE < np;
-       nn += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t nn = n0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float4 A, B;
+  load_particle(p, (int)nn, nn < np, A, B);
+  for (; nn - lane < np; nn += stride) {
     const int n = (int)nn;
+    const bool valid = nn < np;
+    float4 An, Bn;  // prefetch of the next particle
+    load_particle(p, (int)(nn + stride), nn + stride < np, An, Bn);
     PushResult R;
-    push_particle(p, n, nn < np, f0, C, G.nv, cnt + 3, R);
-    if (wave_run_sum(R.active ? R.voxel : -1, R.j)) sink.deposit(R.voxel, R.j);
+    push_compute(A, B, valid, p, n, f0, C, G.nv, cnt + 3, R);
+    wave_deposit(R.active ? R.voxel : -1, R.j, sink);
 
     // Cell crossers: compact into the crosser list (ballot + one atomic per
     // wavefront); the rare overflow of the list is moved right here.
@@ -381,6 +461,8 @@ __global__ __launch_bounds__(256) void k_advance_p(
         }
       }
     }
+    A = An;
+    B = Bn;
   }
 }
 
@@ -404,11 +486,22 @@ __global__ __launch_bounds__(256) void k_move_p_list(
 struct TileGeom {
   int tx, ty, tz;     // tile extent in cells
   int ntx, nty, ntz;  // tiles per axis
+  unsigned mx, my;    // fast_div multipliers for nx+2, ny+2 (0: divide)
 };
 
-constexpr int TILE_BS = 512;           // threads per workgroup
-constexpr int TILE_MAXROWS = 256;      // ty * tz
+constexpr int TILE_BS = 512;            // threads per workgroup
+constexpr int TILE_MAXROWS = 256;       // ty * tz
 constexpr int TILE_QCAP = 2 * TILE_BS;  // crosser queue
+
+// Particle index of slot k of the tile (rows of the sort-time partition);
+// r0 is a row at or before k's.
+__device__ __forceinline__ int tile_particle(const int *row_start,
+                                             const int *row_pref, int nrow,
+                                             int r0, int k) {
+  int r = r0;
+  while (r + 1 < nrow && row_pref[r + 1] <= k) r++;
+  return row_start[r] + (k - row_pref[r]);
+}
 
 __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
@@ -445,6 +538,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   W.wnz = z1 - z0 + 3;
   W.sx1 = G.nx + 2;
   W.sy1 = G.ny + 2;
+  W.mx = T.mx;
+  W.my = T.my;
   const int ncell = W.wnx * W.wny * W.wnz;
 
   // Rows of the tile in the sort-time partition, clamped to [0, np).
@@ -467,21 +562,24 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   __syncthreads();
   const int total = row_pref[nrow];
 
+  int r0 = 0;  // row holding particle slot `base` (same in every lane)
+  int n = total > t ? tile_particle(row_start, row_pref, nrow, 0, t) : 0;
+  float4 A, B;
+  load_particle(p, n, t < total, A, B);
   for (int base = 0; base < total; base += TILE_BS) {
-    const int k = base + t;
-    const bool slot_ok = k < total;
-    int n = 0;
-    if (slot_ok) {
-      int lo = 0, hi = nrow - 1;  // last r with row_pref[r] <= k
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (row_pref[mid] <= k) lo = mid; else hi = mid - 1;
-      }
-      n = row_start[lo] + (k - row_pref[lo]);
-    }
+    const bool slot_ok = base + t < total;
+    // prefetch the next round's particle
+    const int kn = base + TILE_BS + t;
+    int r1 = r0;
+    while (r1 + 1 < nrow && row_pref[r1 + 1] <= base + TILE_BS) r1++;
+    const int nn = kn < total ? tile_particle(row_start, row_pref, nrow, r1, kn)
+                              : 0;
+    float4 An, Bn;
+    load_particle(p, nn, kn < total, An, Bn);
+
     PushResult R;
-    push_particle(p, n, slot_ok, f0, C, G.nv, cnt + 3, R);
-    if (wave_run_sum(R.active ? R.voxel : -1, R.j)) W.deposit(R.voxel, R.j);
+    push_compute(A, B, slot_ok, p, n, f0, C, G.nv, cnt + 3, R);
+    wave_deposit(R.active ? R.voxel : -1, R.j, W);
 
     // queue this round's crossers
     const uint64_t cm = __ballot(R.cross);
@@ -509,6 +607,10 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
         push_returned_mover(p, mv, rlist, rcap, cnt);
       __syncthreads();
     }
+    r0 = r1;
+    n = nn;
+    A = An;
+    B = Bn;
   }
   {
     const int nq = qn;
@@ -624,7 +726,12 @@ int pow2_at_least(int64_t n) {
   return (int)p;
 }
 
-TileGeom g_tile = {8, 8, 4, 0, 0, 0};
+TileGeom g_tile = {8, 8, 4, 0, 0, 0, 0, 0};
+
+// ceil(2^32 / d) for fast_div.
+unsigned div_magic(unsigned d) {
+  return (unsigned)((((uint64_t)1 << 32) + d - 1) / d);
+}
 
 }  // namespace
 
@@ -638,6 +745,13 @@ int push_set_tile(int tx, int ty, int tz) {
   g_tile.ty = ty;
   g_tile.tz = tz;
   return VPIC_OK;
+}
+
+int push_set_direct_runs(int runs) {
+  if (runs < 2 || runs > 65) return VPIC_EINVAL;
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_direct_runs), &runs,
+                                   sizeof(int));
+  return e == hipSuccess ? VPIC_OK : vpic_check_hip(e, "hipMemcpyToSymbol");
 }
 
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
@@ -682,6 +796,10 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       T.ntx = (g->nx + T.tx - 1) / T.tx;
       T.nty = (g->ny + T.ty - 1) / T.ty;
       T.ntz = (g->nz + T.tz - 1) / T.tz;
+      const unsigned sx1 = g->nx + 2, sy1 = g->ny + 2;
+      const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
+      T.mx = magic_ok ? div_magic(sx1) : 0;
+      T.my = magic_ok ? div_magic(sy1) : 0;
       const size_t lds = sizeof(float) * 12 * (size_t)(T.tx + 2) *
                          (T.ty + 2) * (T.tz + 2);
       static bool attr_set = false;
