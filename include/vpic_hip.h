@@ -250,6 +250,13 @@ int vpic_hip_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, void *stream);
 int vpic_hip_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
                             void *stream);
 
+/* synchronize_rho: local_adjust_rhof/rhob then the shared-node fold of
+ * rhof (sum) and rhob (average) (standard/remote.cc:534-625,
+ * standard/local.cc:376-440).  Used with accumulate_rho_p for the charge
+ * conservation diagnostic. */
+int vpic_hip_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g,
+                             void *stream);
+
 /* advance_b: cB -= frac*c*dt*curl E on all faces, then local_adjust_norm_b
  * (advance_b_pipeline.h:17-59; advance_b_pipeline.cc:20-125). */
 int vpic_hip_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac,

@@ -801,6 +801,69 @@ void orc_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g) {
   }
 }
 
+/* synchronize_rho (remote.cc:534-625): local_adjust_rhof / rhob
+ * (local.cc:376-440) on local faces, then the x, y, z node-plane folds with
+ * lw = rw = 1 for rhof and hlw = hrw = 1/2 for rhob (equal cell sizes). */
+void orc_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g) {
+  for (int k = 0; k < 6; k++) {
+    int bc = g->fbc[k];
+    if (bc == VPIC_BC_PERIODIC) continue;
+    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
+    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
+    int c[3];
+    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+        c[X] = face; c[Y] = yy; c[Z] = zz;
+        vpic_field_t *fp = f + fvox(g, c);
+        if (bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS) fp->rhof = 0;
+        else fp->rhof *= 2;
+      }
+  }
+  for (int k = 0; k < 6; k++) {
+    if (g->fbc[k] != VPIC_BC_ANTI_SYMMETRIC_FIELDS) continue;
+    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
+    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
+    int c[3];
+    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+        c[X] = face; c[Y] = yy; c[Z] = zz;
+        f[fvox(g, c)].rhob = 0;
+      }
+  }
+  for (int X = 0; X < 3; X++) {
+    if (g->fbc[X] != VPIC_BC_PERIODIC || g->fbc[X + 3] != VPIC_BC_PERIODIC)
+      continue;
+    int Y = (X + 1) % 3, Z = (X + 2) % 3;
+    int n = (nd(g, Y) + 1) * (nd(g, Z) + 1);
+    float *snd[2];
+    int c[3];
+    for (int side = 0; side < 2; side++) {
+      int face = side == 0 ? 1 : nd(g, X) + 1, m = 0;
+      snd[side] = (float *)malloc(sizeof(float) * 2 * (size_t)n);
+      for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+        for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+          c[X] = face; c[Y] = yy; c[Z] = zz;
+          snd[side][m++] = f[fvox(g, c)].rhof;
+          snd[side][m++] = f[fvox(g, c)].rhob;
+        }
+    }
+    for (int side = 0; side < 2; side++) {
+      const float *p = snd[side];
+      int face = side == 0 ? nd(g, X) + 1 : 1, m = 0;
+      const float hlw = 0.5f, hrw = 0.5f, lw = hlw + hlw, rw = hrw + hrw;
+      for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+        for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+          c[X] = face; c[Y] = yy; c[Z] = zz;
+          vpic_field_t *fp = f + fvox(g, c);
+          fp->rhof = lw * fp->rhof + rw * p[m++];
+          fp->rhob = hlw * fp->rhob + hrw * p[m++];
+        }
+    }
+    free(snd[0]);
+    free(snd[1]);
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* FDTD                                                                     */
 /* ------------------------------------------------------------------------ */

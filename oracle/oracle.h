@@ -71,6 +71,10 @@ void orc_clear_rhof(vpic_field_t *f, const vpic_grid_t *g);
  * order (standard/remote.cc:418-507, standard/local.cc:336-368). */
 void orc_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g);
 
+/* synchronize_rho for one domain: local_adjust_rhof/rhob then the periodic
+ * node-plane folds (standard/remote.cc:534-625, standard/local.cc:376-440). */
+void orc_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g);
+
 /* advance_b: bulk + host leftovers + local_adjust_norm_b
  * (advance_b_pipeline.h:17-59, advance_b_pipeline.cc:20-125,
  * standard/local.cc:266-296). */

@@ -81,6 +81,7 @@ class Oracle:
         lib.orc_energy_f.argtypes = [_vp, _vp, _pg,
                                      ctypes.POINTER(T.FieldParams)]
         lib.orc_accumulate_rho_p.argtypes = [_vp, _vp, _i32, _f32, _pg]
+        lib.orc_synchronize_rho.argtypes = [_vp, _pg]
 
     @classmethod
     def get(cls):
@@ -142,6 +143,9 @@ class Oracle:
 
     def synchronize_jf(self, f, g):
         self.lib.orc_synchronize_jf(_ptr(f), ctypes.byref(g))
+
+    def synchronize_rho(self, f, g):
+        self.lib.orc_synchronize_rho(_ptr(f), ctypes.byref(g))
 
     def advance_b(self, f, g, frac):
         self.lib.orc_advance_b(_ptr(f), ctypes.byref(g), frac)

@@ -257,6 +257,19 @@ def test_field_kernels_bitexact(dev, orc):
         assert np.allclose(np.array(en[:]), eo, rtol=1e-12, atol=0)
 
 
+def test_synchronize_rho_bitexact(dev, orc):
+    for name, g in _field_bc_grids():
+        f = H.random_full_fields(g, 5)
+        fo = f.copy()
+        orc.synchronize_rho(fo, g)
+        df = dev.DeviceArray.from_host(f)
+        say("synchronize_rho,", name)
+        _lib.call("vpic_hip_synchronize_rho", df.ptr, ctypes.byref(g), None)
+        got = df.download()
+        for c in T.FIELD_FLOATS:
+            assert H.same_bits(got[c], fo[c]), (name, c)
+
+
 @pytest.mark.parametrize("mode", ["shuffled", "drifted", "one-hot-voxel"])
 def test_sort_p_bitexact(dev, orc, mode):
     g = _box(6, 5, 4)

@@ -42,6 +42,7 @@ _SIGS = {
     "vpic_hip_clear_jf": [_vp, _pg, _vp],
     "vpic_hip_clear_rhof": [_vp, _pg, _vp],
     "vpic_hip_synchronize_jf": [_vp, _pg, _vp],
+    "vpic_hip_synchronize_rho": [_vp, _pg, _vp],
     "vpic_hip_advance_b": [_vp, _pg, _f32, _vp],
     "vpic_hip_vacuum_advance_e": [_vp, _pg, _pmp, _f32, _vp],
     "vpic_hip_energy_f": [_pf64, _vp, _pg, _pmp, _vp],

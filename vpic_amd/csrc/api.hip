@@ -363,6 +363,14 @@ extern "C" int vpic_hip_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
   return launch_synchronize_jf(f, g, S(stream));
 }
 
+extern "C" int vpic_hip_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g,
+                                        void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(f, "field array is NULL");
+  return launch_synchronize_rho(f, g, S(stream));
+}
+
 extern "C" int vpic_hip_advance_b(vpic_field_t *f, const vpic_grid_t *g,
                                   float frac, void *stream) {
   int rc = check_grid(g);

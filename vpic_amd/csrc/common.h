@@ -171,6 +171,8 @@ int launch_clear_jf(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
 int launch_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
 int launch_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
                           hipStream_t s);
+int launch_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g,
+                           hipStream_t s);
 int launch_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac,
                      hipStream_t s);
 int launch_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,

@@ -204,6 +204,45 @@ __global__ void k_fold_jf(vpic_field_t *f, FaceGeom G, int X) {
   }
 }
 
+// local_adjust_rhof + local_adjust_rhob (local.cc:376-440) on one local face:
+// mode 1 (anti-symmetric) zeroes rhof and rhob, mode 2 doubles rhof.
+__global__ void k_adjust_rho(vpic_field_t *f, FaceGeom G, int k, int mode) {
+  const int X = k % 3, Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int plane = k >= 3 ? G.n[X] + 1 : 1;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    vpic_field_t &c = f[face_vox(G, X, plane, yy, zz)];
+    if (mode == 1) {
+      c.rhof = 0;
+      c.rhob = 0;
+    } else {
+      c.rhof *= 2;
+    }
+  }
+}
+
+// One axis of the periodic node fold of synchronize_rho (remote.cc:534-625):
+// rhof = lw*rhof + rw*remote with lw = rw = 1, rhob = hlw*rhob + hrw*remote
+// with hlw = hrw = 1/2 (equal cell sizes).
+__global__ void k_fold_rho(vpic_field_t *f, FaceGeom G, int X) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const float hlw = 0.5f, hrw = 0.5f, lw = hlw + hlw, rw = hrw + hrw;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    vpic_field_t &a = f[face_vox(G, X, 1, yy, zz)];
+    vpic_field_t &b = f[face_vox(G, X, G.n[X] + 1, yy, zz)];
+    const float af = a.rhof, bf = b.rhof, ab = a.rhob, bb = b.rhob;
+    b.rhof = lw * bf + rw * af;
+    b.rhob = hlw * bb + hrw * ab;
+    a.rhof = lw * af + rw * bf;
+    a.rhob = hlw * ab + hrw * bb;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // advance_b
 // ---------------------------------------------------------------------------
@@ -518,6 +557,24 @@ int launch_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
     if (g->fbc[X] != VPIC_BC_PERIODIC) continue;
     hipLaunchKernelGGL(k_fold_jf, dim3(fb), dim3(256), 0, s, f, G, X);
     VPIC_LAUNCHED("k_fold_jf");
+  }
+  return VPIC_OK;
+}
+
+int launch_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g,
+                           hipStream_t s) {
+  FaceGeom G = face_geom(g);
+  const int fb = face_blocks(g);
+  for (int k = 0; k < 6; k++) {
+    if (g->fbc[k] == VPIC_BC_PERIODIC) continue;
+    const int mode = g->fbc[k] == VPIC_BC_ANTI_SYMMETRIC_FIELDS ? 1 : 2;
+    hipLaunchKernelGGL(k_adjust_rho, dim3(fb), dim3(256), 0, s, f, G, k, mode);
+    VPIC_LAUNCHED("k_adjust_rho");
+  }
+  for (int X = 0; X < 3; X++) {
+    if (g->fbc[X] != VPIC_BC_PERIODIC) continue;
+    hipLaunchKernelGGL(k_fold_rho, dim3(fb), dim3(256), 0, s, f, G, X);
+    VPIC_LAUNCHED("k_fold_rho");
   }
   return VPIC_OK;
 }
