@@ -175,8 +175,9 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
 int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
 
 /* Wavefronts whose 64 particles fall in at least `runs` runs of equal
- * voxels deposit lane by lane instead of summing runs first (default 24;
- * 65 disables it).  A tuning knob; no reference counterpart. */
+ * voxels deposit lane by lane instead of summing runs first (default 65,
+ * which disables it: on gfx950 the run-segmented scan was faster at every
+ * threshold tried).  A tuning knob; no reference counterpart. */
 int vpic_hip_push_direct_runs(int32_t runs);
 
 /* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),
@@ -207,6 +208,16 @@ int vpic_hip_energy_p(const vpic_particle_t *p, int32_t np, float q, float m,
 int vpic_hip_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p,
                               int32_t np, float q, const vpic_grid_t *g,
                               void *stream);
+
+/* center_p (uncenter = 0) / uncenter_p (uncenter = 1): half E kick and half
+ * Boris rotation of the momenta, forward or backward in time; positions are
+ * untouched.  Replaces center_p / uncenter_p(species_t*, const
+ * interpolator_array_t*) (species_advance.h:83-110;
+ * center_p_pipeline.cc:15-130, uncenter_p_pipeline.cc:15-130).
+ * Synchronous (reports particles with a voxel index outside [0,nv)). */
+int vpic_hip_center_p(vpic_particle_t *p, int32_t np, float q, float m,
+                      const vpic_interpolator_t *f, const vpic_grid_t *g,
+                      int32_t uncenter, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Field/particle interface (src/sf_interface/sf_interface.h:82-176)        */

@@ -106,6 +106,13 @@ double orc_energy_p(const vpic_particle_t *p, int np, float q, float m,
 void orc_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
                   const vpic_field_params_t *mp);
 
+/* center_p (uncenter = 0) and uncenter_p (uncenter = 1): half E kick and
+ * half Boris rotation of the momenta, forward or backward in time
+ * (center_p_pipeline.cc:15-97, uncenter_p_pipeline.cc:15-97). */
+void orc_center_p(vpic_particle_t *p, int np, float q, float m,
+                  const vpic_interpolator_t *f0, const vpic_grid_t *g,
+                  int uncenter);
+
 /* accumulate_rho_p (src/species_advance/standard/rho_p.cc:23-113). */
 void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
                           float q, const vpic_grid_t *g);

@@ -77,6 +77,9 @@ class Oracle:
                                              ctypes.POINTER(T.FieldParams)]
         lib.orc_sort_p.argtypes = [_vp, _vp, _i32, _vp, _pg]
         lib.orc_energy_p.argtypes = [_vp, _i32, _f32, _f32, _vp, _pg]
+        lib.orc_center_p.argtypes = [_vp, _i32, _f32, _f32, _vp, _pg,
+                                     ctypes.c_int]
+        lib.orc_center_p.restype = None
         lib.orc_energy_p.restype = ctypes.c_double
         lib.orc_energy_f.argtypes = [_vp, _vp, _pg,
                                      ctypes.POINTER(T.FieldParams)]
@@ -119,6 +122,10 @@ class Oracle:
         self.lib.orc_sort_p(_ptr(p), _ptr(aux), len(p), _ptr(part),
                             ctypes.byref(g))
         return part
+
+    def center_p(self, p, q, m, interp, g, uncenter=False):
+        self.lib.orc_center_p(_ptr(p), len(p), q, m, _ptr(interp),
+                              ctypes.byref(g), int(uncenter))
 
     def energy_p(self, p, q, m, interp, g):
         return self.lib.orc_energy_p(_ptr(p), len(p), q, m, _ptr(interp),
@@ -182,6 +189,9 @@ class Ref:
         lib.ref_move_p.restype = ctypes.c_int
         lib.ref_energy_p.argtypes = [_vp, ctypes.c_int, _f32, _f32, _vp, _pg]
         lib.ref_energy_p.restype = ctypes.c_double
+        lib.ref_center_p.argtypes = [_vp, ctypes.c_int, _f32, _f32, _vp, _pg,
+                                     ctypes.c_int]
+        lib.ref_center_p.restype = None
         lib.ref_sort_p.argtypes = [_vp, ctypes.c_int, _vp, _pg]
         lib.ref_load_interpolator.argtypes = [_vp, _vp, _pg]
         lib.ref_unload_accumulator.argtypes = [_vp, _vp, _pg]
@@ -222,6 +232,10 @@ class Ref:
     def move_p(self, p, mover, acc, hg, q):
         return self.lib.ref_move_p(_ptr(p), _ptr(mover), _ptr(acc),
                                    ctypes.byref(hg.g), q)
+
+    def center_p(self, p, q, m, interp, g, uncenter=False):
+        self.lib.ref_center_p(_ptr(p), len(p), q, m, _ptr(interp),
+                              ctypes.byref(g), int(uncenter))
 
     def energy_p(self, p, q, m, interp, g):
         return self.lib.ref_energy_p(_ptr(p), len(p), q, m, _ptr(interp),

@@ -1,6 +1,7 @@
 // ref_spa.cc -- harness for the reference's species-advance kernels:
-// advance_p_pipeline_scalar, move_p, energy_p_pipeline_scalar and
-// subsort_pipeline_scalar (built from lanl/vpic src/species_advance).
+// advance_p_pipeline_scalar, move_p, energy_p_pipeline_scalar,
+// center_p/uncenter_p_pipeline_scalar and subsort_pipeline_scalar (built
+// from lanl/vpic src/species_advance).
 // TEST INFRASTRUCTURE ONLY: the checker for oracle/oracle.c and the timed
 // "reference" CPU baseline.
 #define IN_spa
@@ -220,6 +221,24 @@ extern "C" double ref_energy_p(const vpic_particle_t *p, int np, float q,
   double local = 0.0;
   for (int r = 0; r <= 1; r++) local += en[r];
   return local * ((double)g->cvac * (double)g->cvac);
+}
+
+// center_p / uncenter_p with one pipeline + host straggler
+// (center_p_pipeline.cc:105-130, uncenter_p_pipeline.cc:105-130).
+extern "C" void ref_center_p(vpic_particle_t *p, int np, float q, float m,
+                             const vpic_interpolator_t *f,
+                             const vpic_grid_t *g, int uncenter) {
+  center_p_pipeline_args_t args[1];
+  args->p0 = (particle_t *)p;
+  args->f0 = (const interpolator_t *)f;
+  args->qdt_2mc = (q * g->dt) / (2 * m * g->cvac);
+  args->np = np;
+  for (int rank = 0; rank <= 1; rank++) {
+    if (uncenter)
+      uncenter_p_pipeline_scalar(args, rank, 1);
+    else
+      center_p_pipeline_scalar(args, rank, 1);
+  }
 }
 
 // sort_p with one pipeline (sort_p_pipeline.cc:345-370).

@@ -4,6 +4,7 @@ reference's "Bad args" paths, e.g. advance_p_pipeline.cc:263-270)."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -13,15 +14,20 @@ from vpic_amd import types as T
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    names = set()
-    inc = os.path.join(ROOT, "include")
-    for fn in os.listdir(inc):
-        if fn.endswith(".h"):
-            src = open(os.path.join(inc, fn)).read()
-            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-            names |= set(re.findall(r"\b(vpic_hip_\w+)\s*\(", src))
-    return sorted(names)
+def _header(fn):
+    src = open(os.path.join(ROOT, "include", fn)).read()
+    return re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+
+
+def _declared(fn="vpic_hip.h"):
+    return sorted(set(re.findall(r"\b(vpic_hip_\w+)\s*\(", _header(fn))))
+
+
+# The reference-named functions of vpic_dropin.h (function declarations:
+# a return type, the name, an argument list ending in ");").
+DROPIN_NAMES = sorted(set(re.findall(
+    r"^(?:void|int|double)\s+(\w+)\s*\([^;]*\);", _header("vpic_dropin.h"),
+    flags=re.M)))
 
 
 def test_library_loads():
@@ -29,14 +35,66 @@ def test_library_loads():
     assert b"gfx950" in lib.vpic_hip_version()
 
 
-@pytest.mark.parametrize("name", _declared())
+@pytest.mark.parametrize("name", _declared() + DROPIN_NAMES)
 def test_symbol_exported(name):
     lib = _lib.load()
     assert hasattr(lib, name), name
 
 
+def test_dropin_declares_reference_kernels():
+    for name in ("advance_p", "sort_p", "move_p", "energy_p",
+                 "accumulate_rho_p", "center_p", "uncenter_p",
+                 "load_interpolator_array", "clear_accumulator_array",
+                 "reduce_accumulator_array", "unload_accumulator_array",
+                 "vpic_hip_install_field_kernels"):
+        assert name in DROPIN_NAMES, name
+
+
 def test_binding_covers_header():
     assert set(_declared()) <= set(_lib.EXPORTS) | {"vpic_hip_version"}
+
+
+def test_dropin_header_in_c(tmp_path):
+    """vpic_dropin.h compiles as C99, and the library's layout table agrees
+    with the header's own types (no device needed)."""
+    src = tmp_path / "t.c"
+    src.write_text(
+        '#include "vpic_dropin.h"\n#include "vpic_dropin_layout.h"\n'
+        "#include <stdio.h>\n"
+        "int main(void) {\n"
+        "  const int64_t mine[] = {VPIC_DROPIN_LAYOUT(VPIC_DROPIN_SIZE,"
+        " VPIC_DROPIN_OFFSET)};\n"
+        "  int64_t lib[512];\n"
+        "  int n = (int)(sizeof mine / sizeof mine[0]);\n"
+        "  int c = vpic_hip_dropin_layout(lib, 512);\n"
+        "  if (c != n) return 2;\n"
+        "  for (int k = 0; k < n; k++) if (lib[k] != mine[k]) return 3;\n"
+        "  printf(\"%d\\n\", n);\n  return 0;\n}\n")
+    exe = tmp_path / "t"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I",
+                    os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L" + libdir, "-l:libvpichip.so",
+                    "-Wl,-rpath," + libdir, "-Wl,-rpath,/opt/rocm/lib"],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out
+    assert int(out.stdout) > 150
+
+
+REF_CHECK = os.path.join(ROOT, "oracle", "_ref", "dropin_check")
+
+
+@pytest.mark.ref
+@pytest.mark.skipif(not os.path.exists(REF_CHECK),
+                    reason="oracle/_ref (reference build) not present")
+def test_dropin_layout_matches_reference_headers():
+    """A program compiled against lanl/vpic's own headers sees the same
+    sizes and offsets as the library."""
+    out = subprocess.run([REF_CHECK, "--layout"], capture_output=True,
+                         text=True)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "ok layout" in out.stdout
 
 
 def test_grid_helper_matches_reference_math():

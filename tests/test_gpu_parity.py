@@ -322,6 +322,20 @@ def test_energy_p_and_rho(dev, orc):
     assert H.acc_rel_err(got, fo["rhof"]) <= 1e-6
 
 
+@pytest.mark.parametrize("uncenter", [0, 1], ids=["center", "uncenter"])
+def test_center_p_bitexact(dev, orc, uncenter):
+    g = _box(6, 5, 4)
+    p, fi = _inputs(orc, g, 9, 0.3, 21)
+    po = p.copy()
+    orc.center_p(po, 1.0, 25.0, fi, g, bool(uncenter))
+    dp = dev.DeviceArray.from_host(p)
+    dfi = dev.DeviceArray.from_host(fi)
+    say("center_p uncenter =", uncenter)
+    _lib.call("vpic_hip_center_p", dp.ptr, len(p), 1.0, 25.0, dfi.ptr,
+              ctypes.byref(g), uncenter, None)
+    assert H.same_bits(dp.download(), po)
+
+
 def test_loader_statistics(dev):
     g = _box(8, 8, 8)
     ppc = 64

@@ -250,6 +250,44 @@ def test_energy_p(orc, ref):
     assert ea == eb
 
 
+@pytest.mark.parametrize("uncenter", [False, True], ids=["center", "uncenter"])
+def test_center_p_bitexact(orc, ref, uncenter):
+    g = _box(6, 5, 4)
+    p, fi = _push_inputs(g, 7, 0.3, 14, e_amp=0.3, b_amp=0.4)
+    pa, pb = p.copy(), p.copy()
+    orc.center_p(pa, -1.0, 1.0, fi, g, uncenter)
+    ref.center_p(pb, -1.0, 1.0, fi, g, uncenter)
+    assert H.same_bits(pa, pb)
+    assert not H.same_bits(pa, p)
+
+
+def test_center_p_closed_forms(orc):
+    """Zero fields: both are the identity.  Uniform E, no B: center_p adds
+    exactly qdt_2mc*E and uncenter_p subtracts it.  uncenter_p undoes
+    center_p to rounding."""
+    g = _box(4, 4, 4)
+    p = H.thermal_particles(g, 5, 0.2, 3)
+    fi = H.interp_array(g)
+    for unc in (False, True):
+        q = p.copy()
+        orc.center_p(q, -1.0, 1.0, fi, g, unc)
+        assert H.same_bits(q, p)
+    fi["ex"], fi["ey"], fi["ez"] = 0.5, -0.25, 0.125
+    qdt_2mc = T.push_coefficients(-1.0, 1.0, g)[0]
+    q = p.copy()
+    orc.center_p(q, -1.0, 1.0, fi, g, False)
+    for c, e in (("ux", 0.5), ("uy", -0.25), ("uz", 0.125)):
+        assert np.array_equal(q[c], p[c] + np.float32(qdt_2mc * np.float32(e)))
+    fi2 = H.interp_array(g)
+    f = H.random_fields(g, 5, 0.2, 0.3)
+    orc.load_interpolator(fi2, f, g)
+    q = p.copy()
+    orc.center_p(q, -1.0, 1.0, fi2, g, False)
+    orc.center_p(q, -1.0, 1.0, fi2, g, True)
+    for c in ("ux", "uy", "uz"):
+        assert np.allclose(q[c], p[c], rtol=0, atol=4e-7 * (1 + np.abs(p[c])))
+
+
 def test_energy_f(orc, ref):
     g = _box(6, 5, 4)
     f = H.random_full_fields(g, 13)

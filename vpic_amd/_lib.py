@@ -35,6 +35,7 @@ _SIGS = {
     "vpic_hip_sort_p": [_vp, _vp, _i32, _vp, _pg, ctypes.c_int, _vp],
     "vpic_hip_energy_p": [_vp, _i32, _f32, _f32, _vp, _pg, _pf64, _vp],
     "vpic_hip_accumulate_rho_p": [_vp, _vp, _i32, _f32, _pg, _vp],
+    "vpic_hip_center_p": [_vp, _i32, _f32, _f32, _vp, _pg, _i32, _vp],
     "vpic_hip_load_interpolator_array": [_vp, _vp, _pg, _vp],
     "vpic_hip_clear_accumulator_array": [_vp, _pg, _vp],
     "vpic_hip_reduce_accumulator_array": [_vp, _pg, _vp],

@@ -287,6 +287,17 @@ extern "C" int vpic_hip_energy_p(const vpic_particle_t *p, int32_t np,
   return launch_energy_p(p, np, q, m, f, g, host_en, S(stream));
 }
 
+extern "C" int vpic_hip_center_p(vpic_particle_t *p, int32_t np, float q,
+                                 float m, const vpic_interpolator_t *f,
+                                 const vpic_grid_t *g, int32_t uncenter,
+                                 void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(f && (np == 0 || p), "NULL argument");
+  CHECK(np >= 0, "np < 0");
+  return launch_center_p(p, np, q, m, f, g, uncenter != 0, S(stream));
+}
+
 extern "C" int vpic_hip_accumulate_rho_p(vpic_field_t *f,
                                          const vpic_particle_t *p, int32_t np,
                                          float q, const vpic_grid_t *g,

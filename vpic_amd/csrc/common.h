@@ -161,6 +161,9 @@ int launch_energy_p(const vpic_particle_t *p, int np, float q, float m,
                     double *host_en, hipStream_t s);
 int launch_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
                             float q, const vpic_grid_t *g, hipStream_t s);
+int launch_center_p(vpic_particle_t *p, int np, float q, float m,
+                    const vpic_interpolator_t *f, const vpic_grid_t *g,
+                    int uncenter, hipStream_t s);
 int launch_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
                              const vpic_grid_t *g, hipStream_t s);
 int launch_clear_accumulator(vpic_accumulator_t *a, const vpic_grid_t *g,

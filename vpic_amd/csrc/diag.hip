@@ -46,6 +46,69 @@ __global__ __launch_bounds__(256) void k_energy_p(
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// center_p / uncenter_p, center_p_pipeline.cc:47-94 and
+// uncenter_p_pipeline.cc:47-94: half E kick and half Boris rotation of the
+// momenta (kick first when centering, rotation first when uncentering).
+// sqrtf here is correctly rounded, which equals the reference's
+// (float)sqrt of a float argument.
+template <bool UNCENTER>
+__global__ __launch_bounds__(256) void k_center_p(
+    vpic_particle_t *__restrict__ p, int np,
+    const vpic_interpolator_t *__restrict__ f, int nv, float qdt_2mc,
+    float qdt_4mc, int *__restrict__ nbad) {
+  const float one = 1.0f, one_third = 1.0f / 3.0f,
+              two_fifteenths = 2.0f / 15.0f;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < np;
+       n += gridDim.x * blockDim.x) {
+    float4 *pp = reinterpret_cast<float4 *>(p + n);
+    const float4 A = pp[0];
+    float4 B = pp[1];
+    const float dx = A.x, dy = A.y, dz = A.z;
+    const int i = __float_as_int(A.w);
+    if ((unsigned)i >= (unsigned)nv) {
+      atomicAdd(nbad, 1);  // never dereferenced; the call reports it
+      continue;
+    }
+    const vpic_interpolator_t &F = f[i];
+    const float hax = qdt_2mc * ((F.ex + dy * F.dexdy) +
+                                 dz * (F.dexdz + dy * F.d2exdydz));
+    const float hay = qdt_2mc * ((F.ey + dz * F.deydz) +
+                                 dx * (F.deydx + dz * F.d2eydzdx));
+    const float haz = qdt_2mc * ((F.ez + dx * F.dezdx) +
+                                 dy * (F.dezdy + dx * F.d2ezdxdy));
+    const float cbx = F.cbx + dx * F.dcbxdx;
+    const float cby = F.cby + dy * F.dcbydy;
+    const float cbz = F.cbz + dz * F.dcbzdz;
+    float ux = B.x, uy = B.y, uz = B.z;
+    if (!UNCENTER) {
+      ux += hax;
+      uy += hay;
+      uz += haz;
+    }
+    float v0 = qdt_4mc / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+    float v1 = cbx * cbx + (cby * cby + cbz * cbz);
+    float v2 = (v0 * v0) * v1;
+    const float v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
+    float v4 = v3 / (one + v1 * (v3 * v3));
+    v4 += v4;
+    v0 = ux + v3 * (uy * cbz - uz * cby);
+    v1 = uy + v3 * (uz * cbx - ux * cbz);
+    v2 = uz + v3 * (ux * cby - uy * cbx);
+    ux += v4 * (v1 * cbz - v2 * cby);
+    uy += v4 * (v2 * cbx - v0 * cbz);
+    uz += v4 * (v0 * cby - v1 * cbx);
+    if (UNCENTER) {
+      ux += hax;
+      uy += hay;
+      uz += haz;
+    }
+    B.x = ux;
+    B.y = uy;
+    B.z = uz;
+    pp[1] = B;
+  }
+}
+
 __global__ void k_clear_int(int *a) {
   if (threadIdx.x == 0) a[0] = 0;
 }
@@ -196,6 +259,38 @@ int launch_energy_p(const vpic_particle_t *p, int np, float q, float m,
                           "energy_p: %d particles have a voxel index outside "
                           "[0,nv)", bad);
   *host_en = out[0] * ((double)g->cvac * (double)g->cvac);
+  return VPIC_OK;
+}
+
+int launch_center_p(vpic_particle_t *p, int np, float q, float m,
+                    const vpic_interpolator_t *f, const vpic_grid_t *g,
+                    int uncenter, hipStream_t s) {
+  if (np <= 0) return VPIC_OK;
+  DiagScratch &S = diag_scratch();
+  int rc;
+  if ((rc = scratch_reserve(S.out, 64))) return rc;
+  int *nbad = (int *)((char *)S.out.ptr + 32);
+  hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, s, nbad);
+  VPIC_LAUNCHED("k_clear_int");
+  // center_p_pipeline.cc:123 / uncenter_p_pipeline.cc:123 and :26-27.
+  const float qdt_2mc = (q * g->dt) / (2 * m * g->cvac);
+  const float hk = uncenter ? -qdt_2mc : qdt_2mc;
+  const float hr = uncenter ? (float)(-0.5 * qdt_2mc) : (float)(0.5 * qdt_2mc);
+  const int nblk = blocks_for(np, 256) < 8192 ? blocks_for(np, 256) : 8192;
+  if (uncenter)
+    hipLaunchKernelGGL(k_center_p<true>, dim3(nblk), dim3(256), 0, s, p, np,
+                       f, g->nv, hk, hr, nbad);
+  else
+    hipLaunchKernelGGL(k_center_p<false>, dim3(nblk), dim3(256), 0, s, p, np,
+                       f, g->nv, hk, hr, nbad);
+  VPIC_LAUNCHED("k_center_p");
+  int bad = 0;
+  VPIC_HIP(hipMemcpyAsync(&bad, nbad, sizeof(int), hipMemcpyDeviceToHost, s));
+  VPIC_HIP(hipStreamSynchronize(s));
+  if (bad)
+    return vpic_set_error(VPIC_EINVAL,
+                          "center_p: %d particles have a voxel index outside "
+                          "[0,nv)", bad);
   return VPIC_OK;
 }
 

@@ -353,7 +353,7 @@ __device__ __forceinline__ float dpp_f32(float v) {
 
 // Wavefronts with at least this many runs of equal voxels deposit lane by
 // lane instead of scanning.
-__constant__ int c_direct_runs = 24;
+__constant__ int c_direct_runs = 65;
 
 // Sum the 12 contributions of the lanes sharing a voxel and deposit them.
 // key < 0 marks lanes with nothing to deposit.
