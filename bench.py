@@ -129,8 +129,9 @@ def main():
     ap.add_argument("--direct-runs", type=int, default=None,
                     help="push deposit threshold (runs per wavefront)")
     ap.add_argument("--sweep", type=str, default=None,
-                    help="tune: 'tx,ty,tz:runs;...' each timed from a fresh "
-                         "load (prints one JSON line per candidate)")
+                    help="tune: 'tx,ty,tz:runs[:group];...' each "
+                         "timed in turn on one plasma (one JSON line per "
+                         "candidate)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 4 GiB device-to-device copy three times "
                          "(calibrates the HBM byte counters)")
@@ -158,13 +159,20 @@ def main():
                           "copy_probe_bytes_written": 4 * n}))
         return
     if args.sweep:
+        # one plasma, several push settings in turn (each timed over the
+        # same number of steps, sorts included)
+        s = build_sim(args.nx, args.ny, args.nz, args.ppc)
+        npart = sum(sp.np for sp in s.species)
+        for _ in range(args.warmup):
+            s.advance()
         for cand in args.sweep.split(";"):
-            tile, runs = cand.split(":")
+            f = cand.split(":")
+            tile, runs = f[0], int(f[1])
+            gs = int(f[2]) if len(f) > 2 else 4
             _lib.call("vpic_hip_push_tile", *[int(v) for v in tile.split(",")])
-            _lib.call("vpic_hip_push_direct_runs", int(runs))
-            s = build_sim(args.nx, args.ny, args.nz, args.ppc)
-            for _ in range(args.warmup):
-                s.advance()
+            _lib.call("vpic_hip_push_direct_runs", runs)
+            _lib.call("vpic_hip_push_group", gs)
+            s.advance()  # settle the new setting
             _lib.call("vpic_hip_push_timing", None, None, None)
             _lib.call("vpic_hip_push_timing_enable", 1)
             torch.cuda.synchronize()
@@ -177,13 +185,10 @@ def main():
             launches, push_ms = ctypes.c_int64(), ctypes.c_double()
             _lib.call("vpic_hip_push_timing", ctypes.byref(launches),
                       ctypes.byref(push_ms), None)
-            npart = sum(sp.np for sp in s.species)
-            print(json.dumps({"tile": tile, "direct_runs": int(runs),
+            print(json.dumps({"tile": tile, "direct_runs": runs, "group": gs,
                               "ms_per_step": el * 1e3 / args.steps,
                               "push_ms": push_ms.value / launches.value,
                               "value": npart * args.steps / el}), flush=True)
-            del s
-            torch.cuda.empty_cache()
         return
     if args.tile:
         _lib.call("vpic_hip_push_tile", *[int(v) for v in args.tile.split(",")])

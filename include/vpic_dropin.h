@@ -226,7 +226,7 @@ int move_p(particle_t *p0, particle_mover_t *m, accumulator_t *a0,
 /* species_advance.h:119-120 (energy_p_pipeline.cc:18-115). */
 double energy_p(const species_t *sp, const interpolator_array_t *ia);
 
-/* species_advance.h:128-130 (rho_p.cc:23-69): adds the species' charge
+/* species_advance.h:128-130 (rho_p.cc:23-113): adds the species' charge
  * density into fa->f[].rhof. */
 void accumulate_rho_p(field_array_t *fa, const species_t *sp);
 

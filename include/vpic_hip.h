@@ -180,6 +180,12 @@ int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
  * threshold tried).  A tuning knob; no reference counterpart. */
 int vpic_hip_push_direct_runs(int32_t runs);
 
+/* Lanes that share one run of particles in the partition-driven push (1,
+ * 2, 4, 8 or 16; default 4): a group reads its run's particles in turn, so
+ * larger groups read wider contiguous spans.  A tuning knob; no reference
+ * counterpart. */
+int vpic_hip_push_group(int32_t gs);
+
 /* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),
  * applied to the nm movers pm[0..nm) in order on the device (one lane, as
  * the reference's callers do, e.g. vpic/misc.cc:95).  `dev_ret` receives

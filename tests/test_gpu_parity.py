@@ -369,6 +369,33 @@ TILES = [(8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
 def tile_reset():
     yield
     _lib.call("vpic_hip_push_tile", 8, 8, 4)
+    _lib.call("vpic_hip_push_group", 4)
+
+
+GROUPS = [1, 2, 4, 8, 16]
+
+
+@pytest.mark.parametrize("gs", GROUPS, ids=["gs%d" % gs for gs in GROUPS])
+def test_tiled_push_groups(dev, orc, gs, tile_reset):
+    """Every lane-group size of the tiled push, on drifted particles and on
+    fast ones whose crossers fill the wavefront queues many times over."""
+    _lib.call("vpic_hip_push_group", gs)
+    g = _box(9, 8, 5)
+    for (q, m, uth, ppc, steps) in ((-1.0, 1.0, 0.3, 13, 3),
+                                    (-1.0, 1.0, 1.5, 40, 0)):
+        p, fi = _inputs(orc, g, ppc, uth, 37, shuffle=True)
+        hg = host_grid(g)
+        part = orc.sort_p(p, g)
+        for _ in range(steps):
+            orc.advance_p(p, q, m, H.acc_array(g), fi, hg)
+        po, ao = p.copy(), H.acc_array(g)
+        orc.advance_p(po, q, m, ao, fi, hg)
+        say("tiled advance_p group", gs, "uth", uth, "ppc", ppc)
+        pg, ag, nmg, igg, _ = _gpu_advance_p(dev, p, q, m, fi, g,
+                                             partition=part)
+        assert H.same_bits(pg, po)
+        assert (nmg, igg) == (0, 0)
+        assert H.acc_rel_err(ag, ao) <= ACC_TOL
 
 
 @pytest.mark.parametrize("tile", TILES, ids=["x".join(map(str, t)) for t in TILES])
@@ -391,6 +418,41 @@ def test_tiled_push_parity(dev, orc, tile, tile_reset):
         assert H.same_bits(pg, po)
         assert (nmg, igg) == (0, 0)
         assert H.acc_rel_err(ag, ao) <= ACC_TOL
+
+
+def test_tiled_push_multistep_device_sort(dev, orc):
+    """sort_p and several tiled pushes entirely on the device (the partition
+    goes stale as the particles drift), each step against the oracle."""
+    g = _box(8, 8, 4)
+    f = H.random_full_fields(g, 51, amp=0.3)
+    fi = H.interp_array(g)
+    orc.load_interpolator(fi, f, g)
+    p = H.thermal_particles(g, 6, 0.3, 61, shuffle=True)
+    hg = host_grid(g)
+    po = p.copy()
+    orc.sort_p(po, g)
+    dp = dev.DeviceArray.from_host(p)
+    aux = dev.particles(len(p))
+    dpart = dev.DeviceArray((g.nv + 1,), np.int32)
+    dfi = dev.DeviceArray.from_host(fi)
+    da = dev.DeviceArray.from_host(H.acc_array(g))
+    dpm = dev.movers(len(p))
+    G = ctypes.byref(g)
+    say("device sort + 4 tiled pushes")
+    _lib.call("vpic_hip_sort_p", dp.ptr, aux.ptr, len(p), dpart.ptr, G, 1,
+              None)
+    assert H.same_bits(dp.download(), po)
+    nm = ctypes.c_int32()
+    for step in range(4):
+        ao = H.acc_array(g)
+        orc.advance_p(po, -1.0, 1.0, ao, fi, hg)
+        _lib.call("vpic_hip_clear_accumulator_array", da.ptr, G, None)
+        _lib.call("vpic_hip_advance_p", dp.ptr, len(p), -1.0, 1.0, dpm.ptr,
+                  len(p), ctypes.byref(nm), None, da.ptr, dfi.ptr, G,
+                  dpart.ptr, None)
+        say("  step", step)
+        assert H.same_bits(dp.download(), po), step
+        assert H.acc_rel_err(da.download(), ao) <= ACC_TOL, step
 
 
 def test_tiled_push_np_changed(dev, orc):

@@ -207,6 +207,12 @@ extern "C" int vpic_hip_advance_p_async(
                           S(stream));
 }
 
+extern "C" int vpic_hip_push_group(int32_t gs) {
+  CHECK(push_set_group(gs) == VPIC_OK,
+        "push lane group %d not one of 1, 2, 4, 8, 16", gs);
+  return VPIC_OK;
+}
+
 extern "C" int vpic_hip_push_direct_runs(int32_t runs) {
   CHECK(runs >= 2 && runs <= 65, "direct-deposit run threshold %d outside "
         "[2,65]", runs);

@@ -150,6 +150,7 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      hipStream_t s);
 int push_set_tile(int tx, int ty, int tz);
 int push_set_direct_runs(int runs);
+int push_set_group(int gs);
 int launch_move_p_list(vpic_particle_t *p, vpic_particle_mover_t *pm, int nm,
                        vpic_accumulator_t *a, const vpic_grid_t *g, float q,
                        int *dev_ret, hipStream_t s);

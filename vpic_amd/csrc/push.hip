@@ -85,10 +85,27 @@ __device__ __forceinline__ unsigned fast_div(unsigned v, unsigned m) {
   return __umulhi(v, m);
 }
 
-// A box of voxels [w0, w0+wn) per axis held in LDS, 12 floats per cell.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+// round(x * 2^k) as a two's-complement int64, exact for |x * 2^k| < 2^51:
+// adding 1.5 * 2^52 puts the rounded integer in the low mantissa bits.
+__device__ __forceinline__ long long to_fixed(double y) {
+  const double magic = 6755399441055744.0;
+  return __double_as_longlong(y + magic) - __double_as_longlong(magic);
+}
+
+// A box of voxels [w0, w0+wn) per axis held in LDS as 64-bit fixed point
+// (value * scale, scale a power of two), component-major (component k of
+// cell s at lds[k * ncell + s]).  gfx950's LDS adds 64-bit integers ~25
+// times faster than it adds floats, and the integer sum does not depend
+// on the order of the additions.  A value too large for the fixed-point
+// range, and any cell outside the window, goes to the global accumulator
+// with float atomics.
 struct WindowSink {
-  float *lds;  // window cells * 12 (LDS)
-  float *acc;  // global accumulator
+  lds_u64 *lds;  // 12 * ncell
+  int ncell;
+  double scale;  // 2^k, set once per workgroup before the first deposit
+  float *acc;    // global accumulator
   int wx0, wy0, wz0, wnx, wny, wnz;
   int sx1, sy1;     // nx+2, ny+2
   unsigned mx, my;  // fast_div multipliers for sx1, sy1 (0: divide)
@@ -111,14 +128,22 @@ struct WindowSink {
   }
   __device__ __forceinline__ void deposit(int voxel, const float j[12]) const {
     const int s = slot(voxel);
+    float *g = acc + 12 * (int64_t)voxel;
     if (s >= 0) {
-      float *a = lds + 12 * s;
+      lds_u64 *a = lds + s;
 #pragma unroll
-      for (int k = 0; k < 12; k++) atomicAdd(a + k, j[k]);
+      for (int k = 0; k < 12; k++) {
+        const double y = (double)j[k] * scale;
+        if (fabs(y) < 2251799813685248.0)  // 2^51
+          __hip_atomic_fetch_add(a + k * ncell,
+                                 (unsigned long long)to_fixed(y),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          atomic_add_f32(g + k, j[k]);
+      }
     } else {
-      float *a = acc + 12 * (int64_t)voxel;
 #pragma unroll
-      for (int k = 0; k < 12; k++) atomic_add_f32(a + k, j[k]);
+      for (int k = 0; k < 12; k++) atomic_add_f32(g + k, j[k]);
     }
   }
 };
@@ -200,6 +225,125 @@ __device__ int dev_move_p(vpic_particle_t *__restrict__ p0,
   return ret;
 }
 
+// Voxel coordinates of v (0..n+1 per axis), with the fast_div multipliers
+// of nx+2 and ny+2 when they are exact for this grid (m == 0: divide).
+struct VoxDiv {
+  unsigned mx, my;
+  __device__ __forceinline__ void coords(const GridDev &G, int v, int &x,
+                                         int &y, int &z) const {
+    const unsigned sx1 = G.nx + 2, sy1 = G.ny + 2;
+    unsigned t, zz;
+    if (mx) {
+      t = fast_div((unsigned)v, mx);
+      zz = fast_div(t, my);
+    } else {
+      t = (unsigned)v / sx1;
+      zz = t / sy1;
+    }
+    x = v - (int)(t * sx1);
+    y = (int)(t - zz * sy1);
+    z = (int)zz;
+  }
+};
+
+// move_p (move_p.cc:215-378) for a particle held in registers: A =
+// {dx,dy,dz,i}, B = {ux,uy,uz,w} (momentum already updated), mv = its
+// mover.  Same arithmetic as dev_move_p.  The voxel coordinates are tracked
+// so that a face inside the domain goes to the adjacent voxel without a
+// table lookup: interior entries of the reference's neighbor table are
+// always v -/+ stride (size_grid, ops.cc:45-76); the table (or the particle
+// boundary codes) is read only on the domain's faces.  Writes the particle
+// back to p0[mv.i]; returns 1 for a mover the grid cannot handle.
+template <class Sink>
+__device__ int dev_move_p_reg(vpic_particle_t *__restrict__ p0, float4 A,
+                              const float4 B, vpic_particle_mover_t &mv,
+                              const Sink &sink, const GridDev &G,
+                              const VoxDiv &VD, float qsp) {
+  float pd[3] = {A.x, A.y, A.z};
+  int pi = __float_as_int(A.w);
+  float pu[3] = {B.x, B.y, B.z};
+  float md[3] = {mv.dispx, mv.dispy, mv.dispz};
+  const float q = qsp * B.w;
+  int c[3];
+  VD.coords(G, pi, c[0], c[1], c[2]);
+  const int nmax[3] = {G.nx, G.ny, G.nz};
+  const int stride[3] = {1, G.sy, G.sz};
+  int ret = 0;
+  for (;;) {
+    float s_mid[3] = {pd[0], pd[1], pd[2]};
+    float s_disp[3] = {md[0], md[1], md[2]};
+    float s_dir[3], v[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_dir[k] = (s_disp[k] > 0.0f) ? 1.0f : -1.0f;
+      v[k] = (s_disp[k] == 0.0f) ? 3.4e38f : (s_dir[k] - s_mid[k]) / s_disp[k];
+    }
+    float v3 = 2.0f;
+    int axis = 3;
+    if (v[0] < v3) v3 = v[0], axis = 0;
+    if (v[1] < v3) v3 = v[1], axis = 1;
+    if (v[2] < v3) v3 = v[2], axis = 2;
+    v3 *= 0.5f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s_disp[k] *= v3;
+      s_mid[k] += s_disp[k];
+    }
+    // move_p.cc:277: promoted to double by the 1.0/3.0 constant.
+    const float v5 =
+        (float)((double)(q * s_disp[0] * s_disp[1] * s_disp[2]) * (1.0 / 3.0));
+    float j[12];
+    streak4(j + 0, q * s_disp[0], s_mid[1], s_mid[2], v5);
+    streak4(j + 4, q * s_disp[1], s_mid[2], s_mid[0], v5);
+    streak4(j + 8, q * s_disp[2], s_mid[0], s_mid[1], v5);
+    sink.deposit(pi, j);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      md[k] -= s_disp[k];
+      pd[k] += s_disp[k] + s_disp[k];
+    }
+    if (axis == 3) break;
+    const float d0 = s_dir[axis];
+    pd[axis] = d0;
+    const bool up = d0 > 0.0f;
+    const int face = axis + (up ? 3 : 0);
+    int ca = axis == 0 ? c[0] : axis == 1 ? c[1] : c[2];
+    const int na = axis == 0 ? nmax[0] : axis == 1 ? nmax[1] : nmax[2];
+    const int st = axis == 0 ? stride[0] : axis == 1 ? stride[1] : stride[2];
+    if (up ? ca < na : ca > 1) {
+      // interior face: the adjacent voxel
+      pi += up ? st : -st;
+      ca += up ? 1 : -1;
+      if (axis == 0) c[0] = ca;
+      else if (axis == 1) c[1] = ca;
+      else c[2] = ca;
+      pd[axis] = -d0;
+      continue;
+    }
+    const int64_t nb = grid_neighbor(G, pi, face);
+    if (nb == VPIC_BC_REFLECT_PARTICLES) {
+      pu[axis] = -pu[axis];
+      md[axis] = -md[axis];
+      continue;
+    }
+    if (nb < G.rangel || nb > G.rangeh) {
+      pi = 8 * pi + face;
+      ret = 1;
+      break;
+    }
+    pi = (int)(nb - G.rangel);
+    VD.coords(G, pi, c[0], c[1], c[2]);
+    pd[axis] = -d0;
+  }
+  float4 *pp = reinterpret_cast<float4 *>(p0 + mv.i);
+  pp[0] = make_float4(pd[0], pd[1], pd[2], __int_as_float(pi));
+  pp[1] = make_float4(pu[0], pu[1], pu[2], B.w);
+  mv.dispx = md[0];
+  mv.dispy = md[1];
+  mv.dispz = md[2];
+  return ret;
+}
+
 // Counters: [0] crossers listed, [1] movers returned, [2] movers lost for
 // lack of list space, [3] particles with an out-of-range voxel index.
 __global__ void k_clear_counters(int *c) {
@@ -232,6 +376,7 @@ struct PushResult {
   int voxel;
   float j[12];
   float dispx, dispy, dispz;
+  float4 Bnew;  // updated momentum and weight (used for crossers)
 };
 
 // Slots without a particle read particle 0 (every launch has np >= 1) and
@@ -244,25 +389,48 @@ __device__ __forceinline__ void load_particle(const vpic_particle_t *p, int n,
   B = pp[1];
 }
 
-__device__ __forceinline__ void push_compute(
-    const float4 A, const float4 B, bool valid_slot,
-    vpic_particle_t *__restrict__ p, int n,
-    const vpic_interpolator_t *__restrict__ f0, const PushCoef &C, int nv,
-    int *__restrict__ nbad, PushResult &R) {
-  const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
-  // A voxel index outside the grid (e.g. a mover left pending as
-  // 8*voxel+face) is never dereferenced: the particle is left untouched and
-  // counted, and the call reports an error.
+// The 18 interpolation coefficients of one voxel (advance_p_pipeline.cc:96-
+// 109 reads them field by field; here as four float4 and a float2).
+struct Interp {
+  float4 e0, e1, e2, b0;
+  float2 b1;
+};
+
+__device__ __forceinline__ void load_interp(
+    const vpic_interpolator_t *__restrict__ f0, int ii, Interp &F) {
+  const float4 *fp = reinterpret_cast<const float4 *>(f0 + ii);
+  F.e0 = fp[0];
+  F.e1 = fp[1];
+  F.e2 = fp[2];
+  F.b0 = fp[3];
+  F.b1 = reinterpret_cast<const float2 *>(fp + 4)[0];
+}
+
+// A voxel index outside the grid (e.g. a mover left pending as
+// 8*voxel+face) is never dereferenced: the particle is left untouched and
+// counted, and the call reports an error.
+__device__ __forceinline__ int push_voxel(const float4 A, bool valid_slot,
+                                          int nv, int *__restrict__ nbad,
+                                          bool &active) {
   const int iraw = __float_as_int(A.w);
-  const bool active = valid_slot && (unsigned)iraw < (unsigned)nv;
+  active = valid_slot && (unsigned)iraw < (unsigned)nv;
   if (valid_slot && !active) atomicAdd(nbad, 1);
+  return active ? iraw : 0;
+}
+
+// STORE_CROSS: also store a crosser's new momentum (the position is left
+// for move_p); the tiled push keeps crossers in registers instead.
+template <bool STORE_CROSS = true>
+__device__ __forceinline__ void push_compute_f(
+    const float4 A, const float4 B, bool active, int ii,
+    vpic_particle_t *__restrict__ p, int n, const Interp &F,
+    const PushCoef &C, PushResult &R) {
+  const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
   float dx = A.x, dy = A.y, dz = A.z;
-  const int ii = active ? iraw : 0;
 
   // Interpolate E and B (advance_p_pipeline.cc:96-109).
-  const float4 *fp = reinterpret_cast<const float4 *>(f0 + ii);
-  const float4 fe0 = fp[0], fe1 = fp[1], fe2 = fp[2], fb0 = fp[3];
-  const float2 fb1 = reinterpret_cast<const float2 *>(fp + 4)[0];
+  const float4 fe0 = F.e0, fe1 = F.e1, fe2 = F.e2, fb0 = F.b0;
+  const float2 fb1 = F.b1;
   const float hax =
       C.qdt_2mc * ((fe0.x + dy * fe0.y) + dz * (fe0.z + dy * fe0.w));
   const float hay =
@@ -321,6 +489,7 @@ __device__ __forceinline__ void push_compute(
   R.dispx = ux;
   R.dispy = uy;
   R.dispz = uz;
+  R.Bnew = Bnew;
   if (active && inbnds) {
     q *= C.qsp;
     pp[0] = make_float4(v3, v4, v5, A.w);
@@ -335,8 +504,20 @@ __device__ __forceinline__ void push_compute(
   } else {
 #pragma unroll
     for (int k = 0; k < 12; k++) R.j[k] = 0.0f;
-    if (active) pp[1] = Bnew;  // momentum stored, position left for move_p
+    if (STORE_CROSS && active) pp[1] = Bnew;  // position left for move_p
   }
+}
+
+__device__ __forceinline__ void push_compute(
+    const float4 A, const float4 B, bool valid_slot,
+    vpic_particle_t *__restrict__ p, int n,
+    const vpic_interpolator_t *__restrict__ f0, const PushCoef &C, int nv,
+    int *__restrict__ nbad, PushResult &R) {
+  bool active;
+  const int ii = push_voxel(A, valid_slot, nv, nbad, active);
+  Interp F;
+  load_interp(f0, ii, F);
+  push_compute_f(A, B, active, ii, p, n, F, C, R);
 }
 
 // --------------------------------------------------------------------------
@@ -489,30 +670,74 @@ struct TileGeom {
   unsigned mx, my;    // fast_div multipliers for nx+2, ny+2 (0: divide)
 };
 
-constexpr int TILE_BS = 512;            // threads per workgroup
-constexpr int TILE_MAXROWS = 256;       // ty * tz
-constexpr int TILE_QCAP = 2 * TILE_BS;  // crosser queue
+constexpr int TILE_BS = 512;              // threads per workgroup
+constexpr int TILE_WAVES = TILE_BS / 64;
+constexpr int TILE_MAXROWS = 256;         // ty * tz
+constexpr int WAVE_Q = 64;                // crosser queue of one wavefront
 
-// Particle index of slot k of the tile (rows of the sort-time partition);
-// r0 is a row at or before k's.
+// Particle index of tile slot k (rows of the sort-time partition); r is a
+// row at or before k's and is advanced to k's row.
 __device__ __forceinline__ int tile_particle(const int *row_start,
                                              const int *row_pref, int nrow,
-                                             int r0, int k) {
-  int r = r0;
+                                             int &r, int k) {
   while (r + 1 < nrow && row_pref[r + 1] <= k) r++;
   return row_start[r] + (k - row_pref[r]);
 }
 
+// Order this wavefront's LDS traffic: a wavefront's LDS operations complete
+// in issue order, so waiting for them (and keeping the compiler from moving
+// memory operations across) makes one lane's queue writes visible to the
+// other lanes' reads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Move the first `cnt` crossers of a wavefront's queue, one per lane.  The
+// push stored each crosser's new momentum and left its position; the
+// particle is read back (from cache) and walked in registers.
+__device__ __forceinline__ void drain_wave_queue(
+    const vpic_particle_mover_t *q, int cnt, int lane,
+    vpic_particle_t *__restrict__ p, const WindowSink &W, const GridDev &G,
+    const VoxDiv &VD, float qsp, vpic_particle_mover_t *__restrict__ rlist,
+    int rcap, int *__restrict__ cnt_g) {
+  wave_lds_sync();
+  vpic_particle_mover_t mv;
+  if (lane < cnt) mv = q[lane];
+  wave_lds_sync();
+  if (lane < cnt) {
+    const float4 *pp = reinterpret_cast<const float4 *>(p + mv.i);
+    const float4 A = pp[0], B = pp[1];
+    if (dev_move_p_reg(p, A, B, mv, W, G, VD, qsp))
+      push_returned_mover(p, mv, rlist, rcap, cnt_g);
+  }
+}
+
+// Partition-driven push: one workgroup per tile of cells.  The tile's
+// particles (the rows of the sort-time partition, sorted by voxel) are split
+// into one contiguous run per group of GS lanes; the GS lanes of a group
+// take the run's particles in turn, so a group reads whole cache lines.  A
+// lane sums the current of its particles in registers while they stay in
+// one voxel and adds the sum to the LDS window when the voxel changes; the
+// interpolation coefficients of the last voxel are kept the same way.  The
+// window's fixed-point scale is set from the largest |q w| of the first
+// particles each lane pushes.  Cell crossers go to a queue of their
+// wavefront, drained 64 at a time by the wavefront (no workgroup barrier in
+// the particle loop).  The finished window is written, as floats, to this
+// tile's slot of `wbuf`; k_merge_windows then adds the windows into the
+// accumulator, so no tile needs atomics for the cells it shares.
+template <int GS>
 __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
-    vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
-    PushCoef C, GridDev G, TileGeom T) {
-  extern __shared__ float lds_acc[];
-  __shared__ vpic_particle_mover_t q[TILE_QCAP];
+    float *__restrict__ wbuf, int wstride, vpic_particle_mover_t *__restrict__ rlist,
+    int rcap, int *__restrict__ cnt, PushCoef C, GridDev G, TileGeom T) {
+  extern __shared__ unsigned long long lds_win[];  // 12 * ncell
+  __shared__ vpic_particle_mover_t wq[TILE_WAVES][WAVE_Q];
   __shared__ int row_start[TILE_MAXROWS];
   __shared__ int row_pref[TILE_MAXROWS + 1];
-  __shared__ int qn;
+  __shared__ unsigned qw_bits;  // largest |q w| of the first round (bits)
 
   const int t = threadIdx.x;
   const int lane = lane_id();
@@ -528,7 +753,7 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   const int nrow = rows_y * (z1 - z0 + 1);
 
   WindowSink W;
-  W.lds = lds_acc;
+  W.lds = (lds_u64 *)lds_win;
   W.acc = acc;
   W.wx0 = x0 - 1;
   W.wy0 = y0 - 1;
@@ -541,6 +766,10 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   W.mx = T.mx;
   W.my = T.my;
   const int ncell = W.wnx * W.wny * W.wnz;
+  W.ncell = ncell;
+  VoxDiv VD;
+  VD.mx = T.mx;
+  VD.my = T.my;
 
   // Rows of the tile in the sort-time partition, clamped to [0, np).
   for (int r = t; r < nrow; r += TILE_BS) {
@@ -552,8 +781,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     row_start[r] = s;
     row_pref[r + 1] = e - s;
   }
-  for (int k = t; k < 12 * ncell; k += TILE_BS) lds_acc[k] = 0.0f;
-  if (t == 0) qn = 0;
+  for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
+  if (t == 0) qw_bits = 0u;
   __syncthreads();
   if (t == 0) {
     row_pref[0] = 0;
@@ -562,75 +791,159 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   __syncthreads();
   const int total = row_pref[nrow];
 
-  int r0 = 0;  // row holding particle slot `base` (same in every lane)
-  int n = total > t ? tile_particle(row_start, row_pref, nrow, 0, t) : 0;
+  // This group's run of slots [g0, g1); lane j of the group takes slots
+  // g0 + j, g0 + j + GS, ...
+  constexpr int NGROUP = TILE_BS / GS;
+  const int per = (total + NGROUP - 1) / NGROUP;  // slots per group
+  const int steps = (per + GS - 1) / GS;          // same for every lane
+  const int grp = t / GS, j = t % GS;
+  const int g0 = min(grp * per, total), g1 = min(g0 + per, total);
+  const vpic_particle_mover_t *qr = wq[t >> 6];
+  vpic_particle_mover_t *q = wq[t >> 6];
+  int qn = 0;  // entries in this wavefront's queue (wavefront-uniform)
+
+  float jacc[12];  // this lane's current, not yet in the window
+#pragma unroll
+  for (int k = 0; k < 12; k++) jacc[k] = 0.0f;
+  int jkey = -1;   // its voxel (-1: nothing held)
+  Interp F;        // coefficients of voxel fkey
+  F.e0 = F.e1 = F.e2 = F.b0 = make_float4(0.f, 0.f, 0.f, 0.f);
+  F.b1 = make_float2(0.f, 0.f);
+  int fkey = -1;
+
+  int r = 0;
+  int k = g0 + j;
+  int n = k < g1 ? tile_particle(row_start, row_pref, nrow, r, k) : 0;
   float4 A, B;
-  load_particle(p, n, t < total, A, B);
-  for (int base = 0; base < total; base += TILE_BS) {
-    const bool slot_ok = base + t < total;
-    // prefetch the next round's particle
-    const int kn = base + TILE_BS + t;
-    int r1 = r0;
-    while (r1 + 1 < nrow && row_pref[r1 + 1] <= base + TILE_BS) r1++;
-    const int nn = kn < total ? tile_particle(row_start, row_pref, nrow, r1, kn)
-                              : 0;
+  load_particle(p, n, k < g1, A, B);
+  for (int it = 0; it < steps; it++, k += GS) {
+    const bool valid = k < g1;
+    // prefetch this lane's next particle
+    const int kn = k + GS;
+    const int nn = kn < g1 ? tile_particle(row_start, row_pref, nrow, r, kn)
+                           : 0;
     float4 An, Bn;
-    load_particle(p, nn, kn < total, An, Bn);
+    load_particle(p, nn, kn < g1, An, Bn);
 
+    bool active;
+    const int ii = push_voxel(A, valid, G.nv, cnt + 3, active);
+    if (active && ii != fkey) {
+      load_interp(f0, ii, F);
+      fkey = ii;
+    }
     PushResult R;
-    push_compute(A, B, slot_ok, p, n, f0, C, G.nv, cnt + 3, R);
-    wave_deposit(R.active ? R.voxel : -1, R.j, W);
-
-    // queue this round's crossers
+    push_compute_f(A, B, active, ii, p, n, F, C, R);
+    if (it == 0) {
+      // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^40
+      // for the first round's particles (one particle's current is below
+      // 4.4|q w|, so a lane's register sum of up to 2^11 of them, and a
+      // cell's LDS sum of up to 2^22, stay in range; a larger value takes
+      // the float path).  No deposit has been made yet (jkey was -1).
+      const float qw = active ? fabsf(C.qsp * B.w) : 0.0f;
+      atomicMax(&qw_bits, __float_as_uint(qw));
+      __syncthreads();
+      const float m = __uint_as_float(qw_bits);
+      int e = 0;
+      if (m > 0.0f) frexpf(8.0f * m, &e);  // 8|q w|max < 2^e
+      W.scale = ldexp(1.0, 40 - e);
+    }
+    if (R.active && !R.cross) {
+      if (ii != jkey) {
+        if (jkey >= 0) W.deposit(jkey, jacc);
+#pragma unroll
+        for (int c = 0; c < 12; c++) jacc[c] = R.j[c];
+        jkey = ii;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 12; c++) jacc[c] += R.j[c];
+      }
+    }
+    // queue the crossers; drain first if they would not fit
     const uint64_t cm = __ballot(R.cross);
     if (cm) {
-      const int leader = __ffsll((unsigned long long)cm) - 1;
-      int qb = 0;
-      if (lane == leader) qb = atomicAdd(&qn, __popcll(cm));
-      qb = __shfl(qb, leader);
+      const int c = __popcll(cm);
+      if (qn + c > WAVE_Q) {
+        drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt);
+        qn = 0;
+      }
       if (R.cross) {
         vpic_particle_mover_t mv;
         mv.dispx = R.dispx;
         mv.dispy = R.dispy;
         mv.dispz = R.dispz;
         mv.i = n;
-        q[qb + __popcll(cm & ((1ull << lane) - 1ull))] = mv;
+        q[qn + __popcll(cm & ((1ull << lane) - 1ull))] = mv;
       }
+      qn += c;
     }
-    __syncthreads();
-    const int nq = qn;
-    __syncthreads();  // every lane has read qn before anyone changes it
-    if (nq >= TILE_BS) {  // uniform: move the top TILE_BS queued crossers
-      vpic_particle_mover_t mv = q[nq - TILE_BS + t];
-      if (t == 0) qn = nq - TILE_BS;
-      if (dev_move_p(p, mv, W, G, C.qsp))
-        push_returned_mover(p, mv, rlist, rcap, cnt);
-      __syncthreads();
-    }
-    r0 = r1;
     n = nn;
     A = An;
     B = Bn;
   }
-  {
-    const int nq = qn;
-    if (t < nq) {
-      vpic_particle_mover_t mv = q[t];
-      if (dev_move_p(p, mv, W, G, C.qsp))
-        push_returned_mover(p, mv, rlist, rcap, cnt);
-    }
-  }
+  if (steps == 0) W.scale = 1.0;  // empty tile: nothing was deposited
+  if (jkey >= 0) W.deposit(jkey, jacc);
+  if (qn) drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt);
   __syncthreads();
-  // Add the window to the global accumulator (untouched cells skipped).
-  for (int k = t; k < 12 * ncell; k += TILE_BS) {
-    const float v = lds_acc[k];
-    if (v != 0.0f) {
-      const int c = k / 12, comp = k - 12 * c;
-      const int cx = c % W.wnx, cr = c / W.wnx;
-      const int cy = cr % W.wny, cz = cr / W.wny;
-      const int vv = vox(W.wx0 + cx, W.wy0 + cy, W.wz0 + cz, G.nx, G.ny);
-      atomic_add_f32(acc + 12 * (int64_t)vv + comp, v);
-    }
+  // The window, as floats, to this tile's slot (component-major, ncell).
+  float *wout = wbuf + (int64_t)blockIdx.x * wstride;
+  const double inv = 1.0 / W.scale;
+  for (int kk = t; kk < 12 * ncell; kk += TILE_BS)
+    wout[kk] = (float)((double)(long long)lds_win[kk] * inv);
+}
+
+// Add the tiles' windows into the accumulator: voxel v receives the
+// window of its own tile and, when it lies on a face of that tile, the
+// halo of the neighbouring tiles across that face (windows do not wrap).
+// One thread per interior voxel; every cell of acc is written by exactly
+// one thread.
+__global__ __launch_bounds__(256) void k_merge_windows(
+    float *__restrict__ acc, const float *__restrict__ wbuf, int wstride,
+    GridDev G, TileGeom T) {
+  const int64_t ncells = (int64_t)G.nx * G.ny * G.nz;
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < ncells;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    const int x = 1 + (int)(l % G.nx);
+    const int64_t lr = l / G.nx;
+    const int y = 1 + (int)(lr % G.ny);
+    const int z = 1 + (int)(lr / G.ny);
+    const int bx = (x - 1) / T.tx, by = (y - 1) / T.ty, bz = (z - 1) / T.tz;
+    // tiles whose window holds v, per axis: own, and the neighbours across
+    // the tile faces that v lies on (both, for a tile one cell wide)
+    int ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0}, oz[3] = {0, 0, 0};
+    int nox = 1, noy = 1, noz = 1;
+    if ((x - 1) % T.tx == 0 && bx > 0) ox[nox++] = -1;
+    if ((x == G.nx || (x % T.tx) == 0) && bx + 1 < T.ntx) ox[nox++] = 1;
+    if ((y - 1) % T.ty == 0 && by > 0) oy[noy++] = -1;
+    if ((y == G.ny || (y % T.ty) == 0) && by + 1 < T.nty) oy[noy++] = 1;
+    if ((z - 1) % T.tz == 0 && bz > 0) oz[noz++] = -1;
+    if ((z == G.nz || (z % T.tz) == 0) && bz + 1 < T.ntz) oz[noz++] = 1;
+    float s[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) s[c] = 0.0f;
+    for (int a = 0; a < nox; a++)
+      for (int bb = 0; bb < noy; bb++)
+        for (int cc = 0; cc < noz; cc++) {
+          const int tx = bx + ox[a], ty = by + oy[bb], tz = bz + oz[cc];
+          const int wx0 = tx * T.tx, wy0 = ty * T.ty, wz0 = tz * T.tz;
+          const int wnx = min(T.tx, G.nx - wx0) + 2;
+          const int wny = min(T.ty, G.ny - wy0) + 2;
+          const int wnz = min(T.tz, G.nz - wz0) + 2;
+          const int nc = wnx * wny * wnz;
+          const int pos = (x - wx0) + wnx * ((y - wy0) + wny * (z - wz0));
+          const float *w =
+              wbuf + (int64_t)(tx + T.ntx * (ty + T.nty * tz)) * wstride + pos;
+#pragma unroll
+          for (int c = 0; c < 12; c++) s[c] += w[c * nc];
+        }
+    float4 *a4 = reinterpret_cast<float4 *>(
+        acc + 12 * (int64_t)vox(x, y, z, G.nx, G.ny));
+    float4 v0 = a4[0], v1 = a4[1], v2 = a4[2];
+    v0.x += s[0]; v0.y += s[1]; v0.z += s[2]; v0.w += s[3];
+    v1.x += s[4]; v1.y += s[5]; v1.z += s[6]; v1.w += s[7];
+    v2.x += s[8]; v2.y += s[9]; v2.z += s[10]; v2.w += s[11];
+    a4[0] = v0;
+    a4[1] = v1;
+    a4[2] = v2;
   }
 }
 
@@ -710,7 +1023,7 @@ __global__ void k_move_p_serial(vpic_particle_t *p, vpic_particle_mover_t *pm,
 // Scratch of the push: counters, crosser list, returned-mover list and the
 // ordering buffer, one set per device.
 struct PushScratch {
-  DeviceScratch counters, xlist, rlist, sortbuf;
+  DeviceScratch counters, xlist, rlist, sortbuf, wbuf;
 };
 
 PushScratch &push_scratch() {
@@ -727,6 +1040,26 @@ int pow2_at_least(int64_t n) {
 }
 
 TileGeom g_tile = {8, 8, 4, 0, 0, 0, 0, 0};
+int g_group = 4;  // lanes sharing one run of particles in the tiled push
+
+typedef void (*TiledKernel)(vpic_particle_t *, int, const int *,
+                            const vpic_interpolator_t *, float *, float *,
+                            int, vpic_particle_mover_t *, int, int *,
+                            PushCoef, GridDev, TileGeom);
+
+int tiled_kernel_id(int gs) {
+  return gs == 1 ? 0 : gs == 2 ? 1 : gs == 4 ? 2 : gs == 8 ? 3 : 4;
+}
+
+TiledKernel tiled_kernel(int gs) {
+  switch (tiled_kernel_id(gs)) {
+    case 0: return k_advance_p_tiled<1>;
+    case 1: return k_advance_p_tiled<2>;
+    case 2: return k_advance_p_tiled<4>;
+    case 3: return k_advance_p_tiled<8>;
+    default: return k_advance_p_tiled<16>;
+  }
+}
 
 // ceil(2^32 / d) for fast_div.
 unsigned div_magic(unsigned d) {
@@ -739,11 +1072,18 @@ unsigned div_magic(unsigned d) {
 // (tx+2)(ty+2)(tz+2) cells of 48 B in LDS).
 int push_set_tile(int tx, int ty, int tz) {
   if (tx < 1 || ty < 1 || tz < 1 || ty * tz > TILE_MAXROWS) return VPIC_EINVAL;
-  if ((size_t)(tx + 2) * (ty + 2) * (tz + 2) * 48 > 96 * 1024)
+  if ((size_t)(tx + 2) * (ty + 2) * (tz + 2) * 96 > 96 * 1024)
     return VPIC_EINVAL;
   g_tile.tx = tx;
   g_tile.ty = ty;
   g_tile.tz = tz;
+  return VPIC_OK;
+}
+
+int push_set_group(int gs) {
+  if (gs != 1 && gs != 2 && gs != 4 && gs != 8 && gs != 16)
+    return VPIC_EINVAL;
+  g_group = gs;
   return VPIC_OK;
 }
 
@@ -800,20 +1140,35 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
       T.mx = magic_ok ? div_magic(sx1) : 0;
       T.my = magic_ok ? div_magic(sy1) : 0;
-      const size_t lds = sizeof(float) * 12 * (size_t)(T.tx + 2) *
-                         (T.ty + 2) * (T.tz + 2);
-      static bool attr_set = false;
-      if (!attr_set) {
-        VPIC_HIP(hipFuncSetAttribute(
-            reinterpret_cast<const void *>(k_advance_p_tiled),
-            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
-        attr_set = true;
-      }
+      const size_t wcells = (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2);
+      const size_t lds = sizeof(unsigned long long) * 12 * wcells;
       const int nblk = T.ntx * T.nty * T.ntz;
-      hipLaunchKernelGGL(k_advance_p_tiled, dim3(nblk), dim3(TILE_BS), lds, s,
-                         p, np, partition, f, (float *)a, rlist, xcap, cnt, C,
-                         G, T);
+      const int wstride = (int)(12 * wcells);
+      if ((rc = scratch_reserve(S.wbuf, sizeof(float) * (size_t)wstride *
+                                            (size_t)nblk)))
+        return rc;
+      const TiledKernel kern = tiled_kernel(g_group);
+      static bool attr_set[5] = {false};
+      const int ak = tiled_kernel_id(g_group);
+      if (!attr_set[ak]) {
+        VPIC_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(kern),
+            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+        attr_set[ak] = true;
+      }
+      hipLaunchKernelGGL(kern, dim3(nblk), dim3(TILE_BS), lds, s, p, np,
+                         partition, f, (float *)a, (float *)S.wbuf.ptr,
+                         wstride, rlist, xcap, cnt, C, G, T);
       VPIC_LAUNCHED("k_advance_p_tiled");
+      {
+        const int64_t ncells = (int64_t)g->nx * g->ny * g->nz;
+        int64_t want = (ncells + 255) / 256;
+        const int mb = (int)(want < 65536 ? want : 65536);
+        hipLaunchKernelGGL(k_merge_windows, dim3(mb), dim3(256), 0, s,
+                           (float *)a, (const float *)S.wbuf.ptr, wstride, G,
+                           T);
+        VPIC_LAUNCHED("k_merge_windows");
+      }
       // particles past the partition (added since the sort) go flat
       flat_start = partition + g->nv;
     }
