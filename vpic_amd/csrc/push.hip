@@ -508,6 +508,84 @@ __device__ __forceinline__ void push_compute_f(
   }
 }
 
+// The push of push_compute_f with the particle stored unconditionally to
+// dst (p + n, or a scratch slot for a lane without a particle): a lane that
+// moves nothing writes back what it read.  Keeping the same memory
+// operations on every path lets the compiler wait only for the loads it
+// needs (stores and prefetches stay in flight across iterations).
+__device__ __forceinline__ void push_compute_u(
+    const float4 A, const float4 B, bool active, const Interp &F,
+    const PushCoef &C, float4 *dst, PushResult &R) {
+  const float one = 1.0f, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
+  float dx = A.x, dy = A.y, dz = A.z;
+  const float4 fe0 = F.e0, fe1 = F.e1, fe2 = F.e2, fb0 = F.b0;
+  const float2 fb1 = F.b1;
+  const float hax =
+      C.qdt_2mc * ((fe0.x + dy * fe0.y) + dz * (fe0.z + dy * fe0.w));
+  const float hay =
+      C.qdt_2mc * ((fe1.x + dz * fe1.y) + dx * (fe1.z + dz * fe1.w));
+  const float haz =
+      C.qdt_2mc * ((fe2.x + dx * fe2.y) + dy * (fe2.z + dx * fe2.w));
+  const float cbx = fb0.x + dx * fb0.y;
+  const float cby = fb0.z + dy * fb0.w;
+  const float cbz = fb1.x + dz * fb1.y;
+  float ux = B.x, uy = B.y, uz = B.z, q = B.w;
+  float v0, v1, v2, v3, v4, v5;
+  ux += hax;
+  uy += hay;
+  uz += haz;
+  v0 = C.qdt_2mc / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+  v1 = cbx * cbx + (cby * cby + cbz * cbz);
+  v2 = (v0 * v0) * v1;
+  v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
+  v4 = v3 / (one + v1 * (v3 * v3));
+  v4 += v4;
+  v0 = ux + v3 * (uy * cbz - uz * cby);
+  v1 = uy + v3 * (uz * cbx - ux * cbz);
+  v2 = uz + v3 * (ux * cby - uy * cbx);
+  ux += v4 * (v1 * cbz - v2 * cby);
+  uy += v4 * (v2 * cbx - v0 * cbz);
+  uz += v4 * (v0 * cby - v1 * cbx);
+  ux += hax;
+  uy += hay;
+  uz += haz;
+  const float4 Bnew = make_float4(ux, uy, uz, q);
+  v0 = one / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+  ux *= C.cdt_dx;
+  uy *= C.cdt_dy;
+  uz *= C.cdt_dz;
+  ux *= v0;
+  uy *= v0;
+  uz *= v0;
+  v0 = dx + ux;
+  v1 = dy + uy;
+  v2 = dz + uz;
+  v3 = v0 + ux;
+  v4 = v1 + uy;
+  v5 = v2 + uz;
+  const bool inbnds = v3 <= one && v4 <= one && v5 <= one && -v3 <= one &&
+                      -v4 <= one && -v5 <= one;
+  const bool stay = active && inbnds;
+  R.active = active;
+  R.cross = active && !inbnds;
+  R.dispx = ux;
+  R.dispy = uy;
+  R.dispz = uz;
+  R.Bnew = Bnew;
+  // advance_p_pipeline.cc:172-208 for a particle that stays in its cell
+  const float4 Anew = make_float4(v3, v4, v5, A.w);
+  q *= C.qsp;
+  const float v6 = q * ux * uy * uz * one_third;
+  float j[12];
+  streak4(j + 0, q * ux, v1, v2, v6);
+  streak4(j + 4, q * uy, v2, v0, v6);
+  streak4(j + 8, q * uz, v0, v1, v6);
+#pragma unroll
+  for (int k = 0; k < 12; k++) R.j[k] = stay ? j[k] : 0.0f;
+  dst[0] = stay ? Anew : A;
+  dst[1] = active ? Bnew : B;
+}
+
 __device__ __forceinline__ void push_compute(
     const float4 A, const float4 B, bool valid_slot,
     vpic_particle_t *__restrict__ p, int n,
@@ -675,14 +753,28 @@ constexpr int TILE_WAVES = TILE_BS / 64;
 constexpr int TILE_MAXROWS = 256;         // ty * tz
 constexpr int WAVE_Q = 64;                // crosser queue of one wavefront
 
-// Particle index of tile slot k (rows of the sort-time partition); r is a
-// row at or before k's and is advanced to k's row.
-__device__ __forceinline__ int tile_particle(const int *row_start,
-                                             const int *row_pref, int nrow,
-                                             int &r, int k) {
-  while (r + 1 < nrow && row_pref[r + 1] <= k) r++;
-  return row_start[r] + (k - row_pref[r]);
-}
+// Walks tile slots in increasing order: particle index of slot k (rows of
+// the sort-time partition), with the current row's slot range and offset
+// kept in registers so that LDS is read only when a row ends.
+struct RowWalk {
+  int r, end, base;  // row, first slot past it, particle index - slot
+  __device__ __forceinline__ void start(const int *row_start,
+                                        const int *row_pref) {
+    r = 0;
+    end = row_pref[1];
+    base = row_start[0];
+  }
+  __device__ __forceinline__ int particle(const int *row_start,
+                                          const int *row_pref, int nrow,
+                                          int k) {
+    while (k >= end && r + 1 < nrow) {
+      r++;
+      end = row_pref[r + 1];
+      base = row_start[r] - row_pref[r];
+    }
+    return base + k;
+  }
+};
 
 // Order this wavefront's LDS traffic: a wavefront's LDS operations complete
 // in issue order, so waiting for them (and keeping the compiler from moving
@@ -811,17 +903,17 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   F.b1 = make_float2(0.f, 0.f);
   int fkey = -1;
 
-  int r = 0;
+  RowWalk rw;
+  rw.start(row_start, row_pref);
   int k = g0 + j;
-  int n = k < g1 ? tile_particle(row_start, row_pref, nrow, r, k) : 0;
+  int n = k < g1 ? rw.particle(row_start, row_pref, nrow, k) : 0;
   float4 A, B;
   load_particle(p, n, k < g1, A, B);
   for (int it = 0; it < steps; it++, k += GS) {
     const bool valid = k < g1;
     // prefetch this lane's next particle
     const int kn = k + GS;
-    const int nn = kn < g1 ? tile_particle(row_start, row_pref, nrow, r, kn)
-                           : 0;
+    const int nn = kn < g1 ? rw.particle(row_start, row_pref, nrow, kn) : 0;
     float4 An, Bn;
     load_particle(p, nn, kn < g1, An, Bn);
 
