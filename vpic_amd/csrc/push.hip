@@ -582,8 +582,12 @@ __device__ __forceinline__ void push_compute_u(
   streak4(j + 8, q * uz, v0, v1, v6);
 #pragma unroll
   for (int k = 0; k < 12; k++) R.j[k] = stay ? j[k] : 0.0f;
-  dst[0] = stay ? Anew : A;
-  dst[1] = active ? Bnew : B;
+  // per-component selects (a select of whole float4 values is lowered
+  // through private memory)
+  dst[0] = make_float4(stay ? Anew.x : A.x, stay ? Anew.y : A.y,
+                       stay ? Anew.z : A.z, A.w);
+  dst[1] = make_float4(active ? Bnew.x : B.x, active ? Bnew.y : B.y,
+                       active ? Bnew.z : B.z, B.w);
 }
 
 __device__ __forceinline__ void push_compute(
@@ -823,8 +827,9 @@ template <int GS>
 __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
-    float *__restrict__ wbuf, int wstride, vpic_particle_mover_t *__restrict__ rlist,
-    int rcap, int *__restrict__ cnt, PushCoef C, GridDev G, TileGeom T) {
+    float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
+    vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
+    PushCoef C, GridDev G, TileGeom T) {
   extern __shared__ unsigned long long lds_win[];  // 12 * ncell
   __shared__ vpic_particle_mover_t wq[TILE_WAVES][WAVE_Q];
   __shared__ int row_start[TILE_MAXROWS];
@@ -902,43 +907,59 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   F.e0 = F.e1 = F.e2 = F.b0 = make_float4(0.f, 0.f, 0.f, 0.f);
   F.b1 = make_float2(0.f, 0.f);
   int fkey = -1;
+  int nbad = 0;  // slots with a voxel index outside the grid
 
   RowWalk rw;
   rw.start(row_start, row_pref);
   int k = g0 + j;
   int n = k < g1 ? rw.particle(row_start, row_pref, nrow, k) : 0;
-  float4 A, B;
-  load_particle(p, n, k < g1, A, B);
-  for (int it = 0; it < steps; it++, k += GS) {
+  // One slot of this lane: push (A, B) = particle n while (An, Bn) is
+  // loaded.  The loop below alternates two register pairs, so a prefetch
+  // is waited for only one full slot after it is issued.
+  float4 A0, B0, A1, B1;
+  load_particle(p, n, k < g1, A0, B0);
+  {
+    // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^40 for
+    // the first particle of every lane (one particle's current is below
+    // 4.4|q w|, so a lane's register sum of up to 2^11 of them, and a
+    // cell's LDS sum of up to 2^22, stay in range; a larger value takes the
+    // float path).
+    const bool act0 =
+        k < g1 && (unsigned)__float_as_int(A0.w) < (unsigned)G.nv;
+    const float qw = act0 ? fabsf(C.qsp * B0.w) : 0.0f;
+    atomicMax(&qw_bits, __float_as_uint(qw));
+    __syncthreads();
+    const float m = __uint_as_float(qw_bits);
+    int e = 0;
+    if (m > 0.0f) frexpf(8.0f * m, &e);  // 8|q w|max < 2^e
+    W.scale = ldexp(1.0, 40 - e);
+  }
+  int it = 0;
+  auto slot = [&](const float4 A, const float4 B, float4 &An, float4 &Bn) {
     const bool valid = k < g1;
-    // prefetch this lane's next particle
-    const int kn = k + GS;
-    const int nn = kn < g1 ? rw.particle(row_start, row_pref, nrow, kn) : 0;
-    float4 An, Bn;
-    load_particle(p, nn, kn < g1, An, Bn);
-
-    bool active;
-    const int ii = push_voxel(A, valid, G.nv, cnt + 3, active);
+    const int iraw = __float_as_int(A.w);
+    const bool active = valid && (unsigned)iraw < (unsigned)G.nv;
+    nbad += valid && !active;
+    const int ii = active ? iraw : 0;
     if (active && ii != fkey) {
       load_interp(f0, ii, F);
       fkey = ii;
     }
+    // prefetch this lane's next particle (issued after the coefficients,
+    // so waiting for them leaves the prefetch in flight)
+    const int kn = k + GS;
+    const int nn = kn < g1 ? rw.particle(row_start, row_pref, nrow, kn) : 0;
+    load_particle(p, nn, kn < g1, An, Bn);
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch issued here
+
+    // every lane stores (an idle lane to its own trash slot), so the
+    // number of memory operations per iteration does not depend on the
+    // particle and the loop never waits for its own stores
     PushResult R;
-    push_compute_f(A, B, active, ii, p, n, F, C, R);
-    if (it == 0) {
-      // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^40
-      // for the first round's particles (one particle's current is below
-      // 4.4|q w|, so a lane's register sum of up to 2^11 of them, and a
-      // cell's LDS sum of up to 2^22, stay in range; a larger value takes
-      // the float path).  No deposit has been made yet (jkey was -1).
-      const float qw = active ? fabsf(C.qsp * B.w) : 0.0f;
-      atomicMax(&qw_bits, __float_as_uint(qw));
-      __syncthreads();
-      const float m = __uint_as_float(qw_bits);
-      int e = 0;
-      if (m > 0.0f) frexpf(8.0f * m, &e);  // 8|q w|max < 2^e
-      W.scale = ldexp(1.0, 40 - e);
-    }
+    push_compute_u(A, B, active, F, C,
+                   valid ? reinterpret_cast<float4 *>(p + n)
+                         : reinterpret_cast<float4 *>(trash + t),
+                   R);
     if (R.active && !R.cross) {
       if (ii != jkey) {
         if (jkey >= 0) W.deposit(jkey, jacc);
@@ -969,10 +990,15 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
       qn += c;
     }
     n = nn;
-    A = An;
-    B = Bn;
+    k = kn;
+    it++;
+  };
+  while (it + 1 < steps) {
+    slot(A0, B0, A1, B1);
+    slot(A1, B1, A0, B0);
   }
-  if (steps == 0) W.scale = 1.0;  // empty tile: nothing was deposited
+  if (it < steps) slot(A0, B0, A1, B1);
+  if (nbad) atomicAdd(cnt + 3, nbad);
   if (jkey >= 0) W.deposit(jkey, jacc);
   if (qn) drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt);
   __syncthreads();
@@ -1115,7 +1141,7 @@ __global__ void k_move_p_serial(vpic_particle_t *p, vpic_particle_mover_t *pm,
 // Scratch of the push: counters, crosser list, returned-mover list and the
 // ordering buffer, one set per device.
 struct PushScratch {
-  DeviceScratch counters, xlist, rlist, sortbuf, wbuf;
+  DeviceScratch counters, xlist, rlist, sortbuf, wbuf, trash;
 };
 
 PushScratch &push_scratch() {
@@ -1136,8 +1162,8 @@ int g_group = 4;  // lanes sharing one run of particles in the tiled push
 
 typedef void (*TiledKernel)(vpic_particle_t *, int, const int *,
                             const vpic_interpolator_t *, float *, float *,
-                            int, vpic_particle_mover_t *, int, int *,
-                            PushCoef, GridDev, TileGeom);
+                            int, vpic_particle_t *, vpic_particle_mover_t *,
+                            int, int *, PushCoef, GridDev, TileGeom);
 
 int tiled_kernel_id(int gs) {
   return gs == 1 ? 0 : gs == 2 ? 1 : gs == 4 ? 2 : gs == 8 ? 3 : 4;
@@ -1239,6 +1265,8 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       if ((rc = scratch_reserve(S.wbuf, sizeof(float) * (size_t)wstride *
                                             (size_t)nblk)))
         return rc;
+      if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
+        return rc;
       const TiledKernel kern = tiled_kernel(g_group);
       static bool attr_set[5] = {false};
       const int ak = tiled_kernel_id(g_group);
@@ -1250,7 +1278,8 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       }
       hipLaunchKernelGGL(kern, dim3(nblk), dim3(TILE_BS), lds, s, p, np,
                          partition, f, (float *)a, (float *)S.wbuf.ptr,
-                         wstride, rlist, xcap, cnt, C, G, T);
+                         wstride, (vpic_particle_t *)S.trash.ptr, rlist, xcap,
+                         cnt, C, G, T);
       VPIC_LAUNCHED("k_advance_p_tiled");
       {
         const int64_t ncells = (int64_t)g->nx * g->ny * g->nz;
