@@ -10,9 +10,11 @@ sort_p (every 20th step), clear/advance_p x2/reduce, unload + synchronize_jf,
 advance_b / advance_e / advance_b, load_interpolator.
 
 value = particle-steps/s over the timed steps (whole job, data resident in
-HBM).  roofline = the push kernel (k_advance_p): algorithmic 64 B per
-particle (one 32-B particle read, one 32-B write) per launch over its mean
-launch time from hipEvents on its stream.  cpu_baseline = the reference's
+HBM).  roofline = the main push kernel (k_advance_p_tiled, the partitioned
+push every timed step runs): algorithmic 64 B per particle (one 32-B
+particle read, one 32-B write) per launch over its mean launch time from
+hipEvents on its stream; traffic = its HBM bytes per launch from the
+FETCH_SIZE / WRITE_SIZE counters (profiles/, calibrated on a copy kernel).  cpu_baseline = the reference's
 own advance_p (oracle/_ref, compiled from the lanl/vpic sources; else the
 oracle's restatement) on the host cores, on a bounded sample of the same
 plasma.
@@ -257,7 +259,7 @@ def main():
         },
         "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
         "push_kernel": {
-            "name": "k_advance_p",
+            "name": "k_advance_p_tiled",
             "launches": launches.value,
             "avg_launch_ms": avg_launch_s * 1e3,
             "particle_steps_per_s": push_parts.value / (push_ms.value / 1e3),
@@ -271,7 +273,8 @@ def main():
             "traffic": None,
         },
     }
-    pmc = os.path.join(ROOT, "profiles", "k_advance_p_c2_traffic.json")
+    pmc = os.path.join(ROOT, "profiles",
+                       "k_advance_p_tiled_plasma256x128x128_traffic.json")
     if os.path.exists(pmc) and (args.nx, args.ny, args.nz, args.ppc) == (
             256, 128, 128, 100):
         with open(pmc) as fh:

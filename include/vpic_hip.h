@@ -328,9 +328,10 @@ int vpic_hip_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
 /* Timing of the push kernel (hipEvents on the launch stream)               */
 /* ------------------------------------------------------------------------ */
 
-/* When enabled, every advance_p brackets its in-cell push kernel with
- * hipEvents; vpic_hip_push_timing returns the launches and total ms since
- * the last reset (synchronises the recorded events). */
+/* When enabled, every advance_p brackets its main push kernel with
+ * hipEvents on its stream (k_advance_p_tiled when a partition is given,
+ * else k_advance_p); vpic_hip_push_timing returns the launches, total ms and
+ * particles since the last reset (synchronises the recorded events). */
 int vpic_hip_push_timing_enable(int on);
 int vpic_hip_push_timing(int64_t *host_launches, double *host_total_ms,
                          int64_t *host_particles);
