@@ -455,6 +455,62 @@ def test_tiled_push_multistep_device_sort(dev, orc):
         assert H.acc_rel_err(da.download(), ao) <= ACC_TOL, step
 
 
+BIN_MODES = [0, 1, 2]
+
+
+@pytest.mark.parametrize("mode", BIN_MODES, ids=["bins%d" % m for m in BIN_MODES])
+def test_tiled_push_tile_lists(dev, orc, mode, tile_reset):
+    """Fast particles on small tiles drift across several tiles between
+    sorts: every push (partition rows or tile lists, whichever the mode
+    picks) against the oracle, with a re-sort in the middle, a flat push, and
+    a change of the particle count."""
+    _lib.call("vpic_hip_push_bin_mode", mode)
+    _lib.call("vpic_hip_push_tile", 2, 3, 2)
+    try:
+        g = _box(9, 8, 6)
+        f = H.random_full_fields(g, 71, amp=0.3)
+        fi = H.interp_array(g)
+        orc.load_interpolator(fi, f, g)
+        p = H.thermal_particles(g, 7, 0.5, 73, shuffle=True)
+        hg = host_grid(g)
+        n0 = len(p)
+        po = p.copy()
+        dp = dev.DeviceArray.from_host(p)
+        aux = dev.particles(n0)
+        dpart = dev.DeviceArray((g.nv + 1,), np.int32)
+        dfi = dev.DeviceArray.from_host(fi)
+        da = dev.DeviceArray.from_host(H.acc_array(g))
+        dpm = dev.movers(n0)
+        G = ctypes.byref(g)
+        nm = ctypes.c_int32()
+        np_cur = n0
+        # step kinds: s = sort then tiled push, t = tiled push, f = flat push,
+        # c = tiled push with the last 37 particles dropped
+        plan = "sttttsttfttctt"
+        for step, kind in enumerate(plan):
+            if kind == "s":
+                orc.sort_p(po[:np_cur], g)
+                _lib.call("vpic_hip_sort_p", dp.ptr, aux.ptr, np_cur,
+                          dpart.ptr, G, 1, None)
+            if kind == "c":
+                np_cur -= 37
+            part = None if kind == "f" else dpart.ptr
+            ao = H.acc_array(g)
+            sub = po[:np_cur]
+            orc.advance_p(sub, -1.0, 1.0, ao, fi, hg)
+            po[:np_cur] = sub
+            _lib.call("vpic_hip_clear_accumulator_array", da.ptr, G, None)
+            _lib.call("vpic_hip_advance_p", dp.ptr, np_cur, -1.0, 1.0,
+                      dpm.ptr, n0, ctypes.byref(nm), None, da.ptr, dfi.ptr, G,
+                      part, None)
+            say("  bin mode", mode, "step", step, kind, "np", np_cur)
+            assert nm.value == 0
+            assert H.same_bits(dp.download()[:np_cur], po[:np_cur]), step
+            assert H.acc_rel_err(da.download(), ao) <= ACC_TOL, step
+    finally:
+        _lib.call("vpic_hip_push_bin_mode", 0)
+
+
 def test_tiled_push_np_changed(dev, orc):
     """Particles appended after the sort (pushed by the flat tail) and
     particles removed (partition clamped to np)."""

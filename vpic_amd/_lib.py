@@ -32,6 +32,7 @@ _SIGS = {
     "vpic_hip_push_tile": [_i32, _i32, _i32],
     "vpic_hip_push_direct_runs": [_i32],
     "vpic_hip_push_group": [_i32],
+    "vpic_hip_push_bin_mode": [_i32],
     "vpic_hip_move_p": [_vp, _vp, _i32, _vp, _pg, _f32, _vp, _vp],
     "vpic_hip_sort_p": [_vp, _vp, _i32, _vp, _pg, ctypes.c_int, _vp],
     "vpic_hip_energy_p": [_vp, _i32, _f32, _f32, _vp, _pg, _pf64, _vp],

@@ -213,6 +213,12 @@ extern "C" int vpic_hip_push_group(int32_t gs) {
   return VPIC_OK;
 }
 
+extern "C" int vpic_hip_push_bin_mode(int32_t mode) {
+  CHECK(push_set_bin_mode(mode) == VPIC_OK,
+        "push bin mode %d not one of 0, 1, 2", mode);
+  return VPIC_OK;
+}
+
 extern "C" int vpic_hip_push_direct_runs(int32_t runs) {
   CHECK(runs >= 2 && runs <= 65, "direct-deposit run threshold %d outside "
         "[2,65]", runs);

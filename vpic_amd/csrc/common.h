@@ -151,6 +151,13 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
 int push_set_tile(int tx, int ty, int tz);
 int push_set_direct_runs(int runs);
 int push_set_group(int gs);
+int push_set_bin_mode(int mode);
+// Forget the tile lists of a particle array whose order or voxels changed
+// wholesale (sort_p, loading).
+void push_invalidate_bins(const void *p);
+// Exclusive scan of n ints (in may equal out).  sums: ceil(n/2048)+1 ints.
+int device_exclusive_scan(const int *in, int *out, int64_t n, int *sums,
+                          hipStream_t s);
 int launch_move_p_list(vpic_particle_t *p, vpic_particle_mover_t *pm, int nm,
                        vpic_accumulator_t *a, const vpic_grid_t *g, float q,
                        int *dev_ret, hipStream_t s);

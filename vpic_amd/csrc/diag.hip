@@ -323,6 +323,7 @@ int launch_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
                                float udx, float udy, float udz, float w,
                                uint64_t seed, hipStream_t s) {
   const int64_t np = (int64_t)ppc * g->nx * g->ny * g->nz;
+  push_invalidate_bins(p);
   int blk = blocks_for(np, 256);
   if (blk > 262144) blk = 262144;
   hipLaunchKernelGGL(k_load_uniform_plasma, dim3(blk), dim3(256), 0, s, p, np,

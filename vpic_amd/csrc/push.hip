@@ -109,6 +109,7 @@ struct WindowSink {
   int wx0, wy0, wz0, wnx, wny, wnz;
   int sx1, sy1;     // nx+2, ny+2
   unsigned mx, my;  // fast_div multipliers for sx1, sy1 (0: divide)
+  int *oow;         // LDS count of deposits outside the window
   __device__ __forceinline__ int slot(int v) const {
     unsigned t, z;
     if (mx) {
@@ -142,6 +143,7 @@ struct WindowSink {
           atomic_add_f32(g + k, j[k]);
       }
     } else {
+      atomicAdd(oow, 1);
 #pragma unroll
       for (int k = 0; k < 12; k++) atomic_add_f32(g + k, j[k]);
     }
@@ -258,7 +260,7 @@ template <class Sink>
 __device__ int dev_move_p_reg(vpic_particle_t *__restrict__ p0, float4 A,
                               const float4 B, vpic_particle_mover_t &mv,
                               const Sink &sink, const GridDev &G,
-                              const VoxDiv &VD, float qsp) {
+                              const VoxDiv &VD, float qsp, int &vout) {
   float pd[3] = {A.x, A.y, A.z};
   int pi = __float_as_int(A.w);
   float pu[3] = {B.x, B.y, B.z};
@@ -341,6 +343,7 @@ __device__ int dev_move_p_reg(vpic_particle_t *__restrict__ p0, float4 A,
   mv.dispx = md[0];
   mv.dispy = md[1];
   mv.dispz = md[2];
+  vout = pi;
   return ret;
 }
 
@@ -793,48 +796,109 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Move the first `cnt` crossers of a wavefront's queue, one per lane.  The
 // push stored each crosser's new momentum and left its position; the
 // particle is read back (from cache) and walked in registers.
+// Tile binning.  The tile that pushes a particle records, per particle,
+// which of the 27 tiles around it holds the particle's voxel after the push
+// (offset code (dx+1) + 3(dy+1) + 9(dz+1); 13 for its own tile and for a
+// voxel further away), and counts the codes.  The next push can then give
+// every tile the particles now in its cells (lists built by k_bin_scatter),
+// so deposits stay inside the tile's window however far the particles have
+// drifted since the sort.  Which tile pushes a particle never changes the
+// result: a deposit outside the window takes the global path.
+struct TileBox {
+  int x0, x1, y0, y1, z0, z1;  // the tile's cells
+  int lx, hx, ly, hy, lz, hz;  // cells of the tile and of its neighbours
+  __device__ __forceinline__ int code(int x, int y, int z) const {
+    if (x < lx || x > hx || y < ly || y > hy || z < lz || z > hz) return 13;
+    const int dx = x < x0 ? 0 : x > x1 ? 2 : 1;
+    const int dy = y < y0 ? 0 : y > y1 ? 2 : 1;
+    const int dz = z < z0 ? 0 : z > z1 ? 2 : 1;
+    return dx + 3 * dy + 9 * dz;
+  }
+};
+
+// Binning control words (device ints):
+//   [0] codes valid: the last tiled push wrote codes and counts
+//   [2] [3] the covered range [part[0], part[nv]) of that push
+//   [4] its deposits outside the window
+//   [5] this push takes its particles from the lists
+//   [6] the last tiled push took its particles from the lists
+//   [7] lists stay on until the state is reset
+//   [8] the lists failed a consistency check (diagnostic)
+constexpr int BIN_CTL = 16;
+
+struct BinArgs {
+  const int *ctl;
+  const int *list;    // particle indices grouped by tile
+  const int *lstart;  // ntiles + 1 offsets into list
+  unsigned char *key; // per particle offset code (written); null: no binning
+  int *counts;        // per tile 27 code counts (written)
+  int *oow;           // out-of-window deposits (accumulated)
+};
+
+struct BinSink {
+  unsigned char *key;  // null: no binning
+  int *cnt;            // LDS code counts
+  TileBox box;
+};
+
 __device__ __forceinline__ void drain_wave_queue(
     const vpic_particle_mover_t *q, int cnt, int lane,
     vpic_particle_t *__restrict__ p, const WindowSink &W, const GridDev &G,
     const VoxDiv &VD, float qsp, vpic_particle_mover_t *__restrict__ rlist,
-    int rcap, int *__restrict__ cnt_g) {
+    int rcap, int *__restrict__ cnt_g, const BinSink &BS) {
   wave_lds_sync();
   vpic_particle_mover_t mv;
   if (lane < cnt) mv = q[lane];
   wave_lds_sync();
   if (lane < cnt) {
-    const float4 *pp = reinterpret_cast<const float4 *>(p + mv.i);
+    const int n = mv.i;
+    const float4 *pp = reinterpret_cast<const float4 *>(p + n);
     const float4 A = pp[0], B = pp[1];
-    if (dev_move_p_reg(p, A, B, mv, W, G, VD, qsp))
-      push_returned_mover(p, mv, rlist, rcap, cnt_g);
+    int vout;
+    const int ret = dev_move_p_reg(p, A, B, mv, W, G, VD, qsp, vout);
+    if (ret) push_returned_mover(p, mv, rlist, rcap, cnt_g);
+    if (BS.key) {
+      int o = 13;
+      if (!ret) {
+        int x, y, z;
+        VD.coords(G, vout, x, y, z);
+        o = BS.box.code(x, y, z);
+      }
+      BS.key[n] = (unsigned char)o;
+      atomicAdd(BS.cnt + o, 1);
+    }
   }
 }
 
-// Partition-driven push: one workgroup per tile of cells.  The tile's
-// particles (the rows of the sort-time partition, sorted by voxel) are split
-// into one contiguous run per group of GS lanes; the GS lanes of a group
-// take the run's particles in turn, so a group reads whole cache lines.  A
-// lane sums the current of its particles in registers while they stay in
-// one voxel and adds the sum to the LDS window when the voxel changes; the
-// interpolation coefficients of the last voxel are kept the same way.  The
-// window's fixed-point scale is set from the largest |q w| of the first
-// particles each lane pushes.  Cell crossers go to a queue of their
-// wavefront, drained 64 at a time by the wavefront (no workgroup barrier in
-// the particle loop).  The finished window is written, as floats, to this
-// tile's slot of `wbuf`; k_merge_windows then adds the windows into the
-// accumulator, so no tile needs atomics for the cells it shares.
+// Tiled push: one workgroup per tile of cells.  Its particles are either the
+// rows of the sort-time partition (sorted by voxel) or, when binning is on,
+// the tile's list (the particles the previous push left in its cells).
+// They are split into one contiguous run per group of GS lanes; the GS
+// lanes of a group take the run's particles in turn, so a group reads whole
+// cache lines.  A lane sums the current of its particles in registers while
+// they stay in one voxel and adds the sum to the LDS window when the voxel
+// changes; the interpolation coefficients of the last voxel are kept the
+// same way.  The window's fixed-point scale is set from the largest |q w| of
+// the first particles each lane pushes.  Cell crossers go to a queue of
+// their wavefront, drained 64 at a time by the wavefront (no workgroup
+// barrier in the particle loop).  The finished window is written, as
+// floats, to this tile's slot of `wbuf`; k_merge_windows then adds the
+// windows into the accumulator, so no tile needs atomics for the cells it
+// shares.
 template <int GS>
 __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
     float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
     vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
-    PushCoef C, GridDev G, TileGeom T) {
+    PushCoef C, GridDev G, TileGeom T, BinArgs Bn) {
   extern __shared__ unsigned long long lds_win[];  // 12 * ncell
   __shared__ vpic_particle_mover_t wq[TILE_WAVES][WAVE_Q];
   __shared__ int row_start[TILE_MAXROWS];
   __shared__ int row_pref[TILE_MAXROWS + 1];
   __shared__ unsigned qw_bits;  // largest |q w| of the first round (bits)
+  __shared__ int bin_cnt[27];
+  __shared__ int oow_cnt;
 
   const int t = threadIdx.x;
   const int lane = lane_id();
@@ -848,6 +912,9 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
             z1 = min(z0 + T.tz - 1, G.nz);
   const int rows_y = y1 - y0 + 1;
   const int nrow = rows_y * (z1 - z0 + 1);
+  const bool keyed = Bn.key != nullptr;
+  const bool listed =
+      keyed && __builtin_amdgcn_readfirstlane(Bn.ctl[5]) != 0;
 
   WindowSink W;
   W.lds = (lds_u64 *)lds_win;
@@ -862,11 +929,28 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   W.sy1 = G.ny + 2;
   W.mx = T.mx;
   W.my = T.my;
+  W.oow = &oow_cnt;
   const int ncell = W.wnx * W.wny * W.wnz;
   W.ncell = ncell;
   VoxDiv VD;
   VD.mx = T.mx;
   VD.my = T.my;
+  BinSink BS;
+  BS.key = Bn.key;
+  BS.cnt = bin_cnt;
+  TileBox &TB = BS.box;
+  TB.x0 = x0;
+  TB.x1 = x1;
+  TB.y0 = y0;
+  TB.y1 = y1;
+  TB.z0 = z0;
+  TB.z1 = z1;
+  TB.lx = btx > 0 ? x0 - T.tx : x0;
+  TB.hx = btx + 1 < T.ntx ? min(x1 + T.tx, G.nx) : x1;
+  TB.ly = bty > 0 ? y0 - T.ty : y0;
+  TB.hy = bty + 1 < T.nty ? min(y1 + T.ty, G.ny) : y1;
+  TB.lz = btz > 0 ? z0 - T.tz : z0;
+  TB.hz = btz + 1 < T.ntz ? min(z1 + T.tz, G.nz) : z1;
 
   // Rows of the tile in the sort-time partition, clamped to [0, np).
   for (int r = t; r < nrow; r += TILE_BS) {
@@ -880,13 +964,24 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   }
   for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
   if (t == 0) qw_bits = 0u;
+  if (t < 27) bin_cnt[t] = 0;
+  if (t == 0) oow_cnt = 0;
   __syncthreads();
   if (t == 0) {
     row_pref[0] = 0;
     for (int r = 0; r < nrow; r++) row_pref[r + 1] += row_pref[r];
   }
   __syncthreads();
-  const int total = row_pref[nrow];
+  // the slots: the partition rows, or this tile's list
+  int total = row_pref[nrow];
+  const int *L = Bn.list;
+  if (listed) {
+    int ls = Bn.lstart[blockIdx.x], le = Bn.lstart[blockIdx.x + 1];
+    ls = min(max(ls, 0), np);
+    le = min(max(le, ls), np);
+    L += ls;
+    total = le - ls;
+  }
 
   // This group's run of slots [g0, g1); lane j of the group takes slots
   // g0 + j, g0 + j + GS, ...
@@ -912,12 +1007,21 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   RowWalk rw;
   rw.start(row_start, row_pref);
   int k = g0 + j;
-  int n = k < g1 ? rw.particle(row_start, row_pref, nrow, k) : 0;
+  // particle of the current slot (a list entry outside [0, np) is skipped)
+  int n = 0;
+  bool cur_ok = k < g1;
+  if (cur_ok) {
+    n = listed ? L[k] : rw.particle(row_start, row_pref, nrow, k);
+    cur_ok = (unsigned)n < (unsigned)np;
+    if (!cur_ok) n = 0;
+  }
+  // list entry of the next slot, loaded one slot ahead
+  int nidx = (listed && k + GS < g1) ? L[k + GS] : 0;
   // One slot of this lane: push (A, B) = particle n while (An, Bn) is
   // loaded.  The loop below alternates two register pairs, so a prefetch
   // is waited for only one full slot after it is issued.
   float4 A0, B0, A1, B1;
-  load_particle(p, n, k < g1, A0, B0);
+  load_particle(p, n, cur_ok, A0, B0);
   {
     // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^40 for
     // the first particle of every lane (one particle's current is below
@@ -925,7 +1029,7 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     // cell's LDS sum of up to 2^22, stay in range; a larger value takes the
     // float path).
     const bool act0 =
-        k < g1 && (unsigned)__float_as_int(A0.w) < (unsigned)G.nv;
+        cur_ok && (unsigned)__float_as_int(A0.w) < (unsigned)G.nv;
     const float qw = act0 ? fabsf(C.qsp * B0.w) : 0.0f;
     atomicMax(&qw_bits, __float_as_uint(qw));
     __syncthreads();
@@ -935,8 +1039,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     W.scale = ldexp(1.0, 40 - e);
   }
   int it = 0;
-  auto slot = [&](const float4 A, const float4 B, float4 &An, float4 &Bn) {
-    const bool valid = k < g1;
+  auto slot = [&](const float4 A, const float4 B, float4 &An, float4 &Bn_) {
+    const bool valid = cur_ok;
     const int iraw = __float_as_int(A.w);
     const bool active = valid && (unsigned)iraw < (unsigned)G.nv;
     nbad += valid && !active;
@@ -946,10 +1050,20 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
       fkey = ii;
     }
     // prefetch this lane's next particle (issued after the coefficients,
-    // so waiting for them leaves the prefetch in flight)
+    // so waiting for them leaves the prefetch in flight), then the list
+    // entry after it
     const int kn = k + GS;
-    const int nn = kn < g1 ? rw.particle(row_start, row_pref, nrow, kn) : 0;
-    load_particle(p, nn, kn < g1, An, Bn);
+    int nn = 0;
+    bool nok = kn < g1;
+    if (listed) {
+      nn = nidx;
+      nok = nok && (unsigned)nn < (unsigned)np;
+    } else if (nok) {
+      nn = rw.particle(row_start, row_pref, nrow, kn);
+    }
+    if (!nok) nn = 0;
+    load_particle(p, nn, nok, An, Bn_);
+    if (listed) nidx = kn + GS < g1 ? L[kn + GS] : 0;
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch issued here
 
     // every lane stores (an idle lane to its own trash slot), so the
@@ -960,6 +1074,24 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
                    valid ? reinterpret_cast<float4 *>(p + n)
                          : reinterpret_cast<float4 *>(trash + t),
                    R);
+    if (keyed) {
+      // the tile holding the particle now (a crosser's is set when it is
+      // moved)
+      int o = 13;
+      if (active && !R.cross) {
+        int x, y, z;
+        VD.coords(G, ii, x, y, z);
+        o = TB.code(x, y, z);
+      }
+      unsigned char *kp = valid ? Bn.key + n
+                                : reinterpret_cast<unsigned char *>(trash) + t;
+      *kp = (unsigned char)o;
+      const bool counted = valid && !R.cross;
+      const uint64_t m13 = __ballot(counted && o == 13);
+      if (m13 && lane == __ffsll((unsigned long long)m13) - 1)
+        atomicAdd(&bin_cnt[13], __popcll(m13));
+      if (counted && o != 13) atomicAdd(&bin_cnt[o], 1);
+    }
     if (R.active && !R.cross) {
       if (ii != jkey) {
         if (jkey >= 0) W.deposit(jkey, jacc);
@@ -976,7 +1108,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     if (cm) {
       const int c = __popcll(cm);
       if (qn + c > WAVE_Q) {
-        drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt);
+        drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt,
+                         BS);
         qn = 0;
       }
       if (R.cross) {
@@ -990,6 +1123,7 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
       qn += c;
     }
     n = nn;
+    cur_ok = nok;
     k = kn;
     it++;
   };
@@ -1000,8 +1134,13 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   if (it < steps) slot(A0, B0, A1, B1);
   if (nbad) atomicAdd(cnt + 3, nbad);
   if (jkey >= 0) W.deposit(jkey, jacc);
-  if (qn) drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt);
+  if (qn)
+    drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt, BS);
   __syncthreads();
+  if (keyed) {
+    if (t < 27) Bn.counts[blockIdx.x * 27 + t] = bin_cnt[t];
+    if (t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
+  }
   // The window, as floats, to this tile's slot (component-major, ncell).
   float *wout = wbuf + (int64_t)blockIdx.x * wstride;
   const double inv = 1.0 / W.scale;
@@ -1062,6 +1201,178 @@ __global__ __launch_bounds__(256) void k_merge_windows(
     a4[0] = v0;
     a4[1] = v1;
     a4[2] = v2;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Tile lists for the next push (from the codes and counts of the last one)
+// --------------------------------------------------------------------------
+
+__global__ void k_fill_int32(int *a, int n, int v) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = v;
+}
+
+__device__ __forceinline__ void covered_range(const int *part, int nv, int np,
+                                              int &lo, int &hi) {
+  lo = min(max(part[0], 0), np);
+  hi = min(max(part[nv], lo), np);
+}
+
+// Whether this push takes its particles from the lists: the codes are
+// those of the last tiled push over the same covered range, and binning is
+// forced (mode 2) or the last push put at least 1/128 of its deposits
+// outside its windows (then it stays on until the state is reset).
+__global__ void k_bin_decide(int *ctl, const int *part, int nv, int np,
+                             int mode) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int lo, hi;
+  covered_range(part, nv, np, lo, hi);
+  const bool ok = ctl[0] != 0 && ctl[2] == lo && ctl[3] == hi;
+  const bool want =
+      ok && mode > 0 &&
+      (mode == 2 || ctl[7] != 0 || (int64_t)ctl[4] * 128 >= (int64_t)(hi - lo));
+  ctl[5] = want ? 1 : 0;
+  ctl[7] = want ? 1 : 0;
+  ctl[4] = 0;
+  ctl[8] = 0;
+}
+
+// After the push: its codes are valid for its covered range.
+__global__ void k_bin_post(int *ctl, const int *part, int nv, int np) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int lo, hi;
+  covered_range(part, nv, np, lo, hi);
+  ctl[0] = 1;
+  ctl[2] = lo;
+  ctl[3] = hi;
+  ctl[6] = ctl[5];
+}
+
+__device__ __forceinline__ bool tile_src(const TileGeom &T, int d, int o,
+                                         int &s) {
+  const int dtx = d % T.ntx, dty = (d / T.ntx) % T.nty,
+            dtz = d / (T.ntx * T.nty);
+  const int sx = dtx - (o % 3 - 1), sy = dty - ((o / 3) % 3 - 1),
+            sz = dtz - (o / 9 - 1);
+  if (sx < 0 || sx >= T.ntx || sy < 0 || sy >= T.nty || sz < 0 ||
+      sz >= T.ntz)
+    return false;
+  s = sx + T.ntx * (sy + T.nty * sz);
+  return true;
+}
+
+// total[d] = particles the last push left in tile d (total[ntiles] = 0).
+__global__ __launch_bounds__(256) void k_bin_total(const int *ctl,
+                                                   const int *counts,
+                                                   int *total, TileGeom T) {
+  const int ntiles = T.ntx * T.nty * T.ntz;
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d > ntiles) return;
+  int sum = 0;
+  if (d < ntiles && ctl[5]) {
+    for (int o = 0; o < 27; o++) {
+      int s;
+      if (tile_src(T, d, o, s)) sum += counts[s * 27 + o];
+    }
+  }
+  total[d] = sum;
+}
+
+// base[s*27+o] = where the particles tile s sends to tile d = s + off(o)
+// start in d's list (sources in code order).
+__global__ __launch_bounds__(256) void k_bin_base(const int *ctl,
+                                                  const int *counts,
+                                                  const int *start, int *base,
+                                                  TileGeom T) {
+  const int ntiles = T.ntx * T.nty * T.ntz;
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= ntiles || !ctl[5]) return;
+  int run = start[d];
+  for (int o = 0; o < 27; o++) {
+    int s;
+    if (tile_src(T, d, o, s)) {
+      base[s * 27 + o] = run;
+      run += counts[s * 27 + o];
+    }
+  }
+}
+
+// One wavefront per source tile s: walk the particles the last push gave to
+// s, in the order it took them (its partition rows, or its list), and write
+// each index to the list of the tile its code names.  The order within one
+// (source, code) pair is kept.  A count that does not match the push's (the
+// partition or the particles changed in between) switches the lists off
+// for this push.
+__global__ __launch_bounds__(256) void k_bin_scatter(
+    int *ctl, const int *__restrict__ part, int np,
+    const int *__restrict__ lcur, const int *__restrict__ scur,
+    const unsigned char *__restrict__ key, const int *__restrict__ counts,
+    const int *__restrict__ base, int *__restrict__ lnext, GridDev G,
+    TileGeom T) {
+  if (!ctl[5]) return;
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ntiles = T.ntx * T.nty * T.ntz;
+  if (s >= ntiles) return;
+  const int b27 = lane < 27 ? lane : 0;
+  int runv = base[s * 27 + b27];
+  const int want = runv + (lane < 27 ? counts[s * 27 + b27] : 0);
+  bool bad = false;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  auto emit = [&](int n, bool v) {
+    int o = v ? (int)key[n] : 0;
+    if (o > 26) {
+      bad = true;
+      v = false;
+    }
+    uint64_t rem = __ballot(v);
+    while (rem) {
+      const int l = __ffsll((unsigned long long)rem) - 1;
+      const int bo = __builtin_amdgcn_readlane(o, l);
+      const uint64_t m = __ballot(v && o == bo);
+      const int r0 = __builtin_amdgcn_readlane(runv, bo);
+      if (v && o == bo) {
+        const int pos = r0 + __popcll(m & lt);
+        if ((unsigned)pos < (unsigned)np)
+          lnext[pos] = n;
+        else
+          bad = true;
+      }
+      if (lane == bo) runv += __popcll(m);
+      rem &= ~m;
+    }
+  };
+  if (ctl[6]) {
+    int ls = scur[s], le = scur[s + 1];
+    ls = min(max(ls, 0), np);
+    le = min(max(le, ls), np);
+    for (int e0 = ls; e0 < le; e0 += 64) {
+      const int e = e0 + lane;
+      const bool v = e < le;
+      const int n = v ? lcur[e] : 0;
+      emit(n, v && (unsigned)n < (unsigned)np);
+    }
+  } else {
+    const int btx = s % T.ntx, bty = (s / T.ntx) % T.nty,
+              btz = s / (T.ntx * T.nty);
+    const int x0 = 1 + btx * T.tx, y0 = 1 + bty * T.ty, z0 = 1 + btz * T.tz;
+    const int x1 = min(x0 + T.tx - 1, G.nx), y1 = min(y0 + T.ty - 1, G.ny),
+              z1 = min(z0 + T.tz - 1, G.nz);
+    for (int z = z0; z <= z1; z++)
+      for (int y = y0; y <= y1; y++) {
+        int rs = part[vox(x0, y, z, G.nx, G.ny)];
+        int re = part[vox(x1 + 1, y, z, G.nx, G.ny)];
+        rs = min(max(rs, 0), np);
+        re = min(max(re, rs), np);
+        for (int e0 = rs; e0 < re; e0 += 64) {
+          const int e = e0 + lane;
+          emit(e, e < re);
+        }
+      }
+  }
+  if (__ballot(bad || (lane < 27 && runv != want)) && lane == 0) {
+    ctl[5] = 0;
+    ctl[8] = 1;
   }
 }
 
@@ -1140,8 +1451,19 @@ __global__ void k_move_p_serial(vpic_particle_t *p, vpic_particle_mover_t *pm,
 
 // Scratch of the push: counters, crosser list, returned-mover list and the
 // ordering buffer, one set per device.
+// Tile lists of one particle array (keyed by its address).
+struct BinState {
+  const void *p = nullptr;
+  int np = -1, ntiles = 0;
+  int64_t used = 0;
+  int cur = 0;  // list[cur] / start[cur]: the lists the last push used
+  DeviceScratch key, counts, base, total, sums, ctl, list[2], start[2];
+};
+
 struct PushScratch {
   DeviceScratch counters, xlist, rlist, sortbuf, wbuf, trash;
+  BinState bins[8];
+  int64_t bin_clock = 0;
 };
 
 PushScratch &push_scratch() {
@@ -1159,11 +1481,27 @@ int pow2_at_least(int64_t n) {
 
 TileGeom g_tile = {8, 8, 4, 0, 0, 0, 0, 0};
 int g_group = 4;  // lanes sharing one run of particles in the tiled push
+int g_bin_mode = 0;  // tile lists: 0 off, 1 when drift warrants, 2 always
+
+BinState &bin_state(PushScratch &S, const void *p) {
+  BinState *hit = nullptr, *lru = &S.bins[0];
+  for (BinState &b : S.bins) {
+    if (b.p == p) hit = &b;
+    if (b.used < lru->used) lru = &b;
+  }
+  if (!hit) {
+    hit = lru;
+    hit->p = p;
+    hit->np = -1;
+  }
+  hit->used = ++S.bin_clock;
+  return *hit;
+}
 
 typedef void (*TiledKernel)(vpic_particle_t *, int, const int *,
                             const vpic_interpolator_t *, float *, float *,
                             int, vpic_particle_t *, vpic_particle_mover_t *,
-                            int, int *, PushCoef, GridDev, TileGeom);
+                            int, int *, PushCoef, GridDev, TileGeom, BinArgs);
 
 int tiled_kernel_id(int gs) {
   return gs == 1 ? 0 : gs == 2 ? 1 : gs == 4 ? 2 : gs == 8 ? 3 : 4;
@@ -1203,6 +1541,21 @@ int push_set_group(int gs) {
     return VPIC_EINVAL;
   g_group = gs;
   return VPIC_OK;
+}
+
+int push_set_bin_mode(int mode) {
+  if (mode < 0 || mode > 2) return VPIC_EINVAL;
+  g_bin_mode = mode;
+  // lists resume only from codes written while binning was on
+  PushScratch &S = push_scratch();
+  for (BinState &b : S.bins) b.np = -1;
+  return VPIC_OK;
+}
+
+void push_invalidate_bins(const void *p) {
+  PushScratch &S = push_scratch();
+  for (BinState &b : S.bins)
+    if (b.p == p) b.np = -1;
 }
 
 int push_set_direct_runs(int runs) {
@@ -1245,6 +1598,8 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
 
   hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
   VPIC_LAUNCHED("k_clear_counters");
+  // a flat push moves particles without recording their tiles
+  if (!partition) push_invalidate_bins(p);
   if (np > 0) {
     // the timing brackets the main push kernel only: k_advance_p_tiled
     // when a partition is given, else k_advance_p
@@ -1277,13 +1632,80 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
             hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
         attr_set[ak] = true;
       }
+      // Tile lists from the codes of the last tiled push of this array.
+      BinArgs BA = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+      BinState *BSt = nullptr;
+      if (g_bin_mode > 0) {
+        BinState &B = bin_state(S, p);
+        const int nt = nblk;
+        const int nsum = (nt + 1 + 2047) / 2048 + 1;
+        if ((rc = scratch_reserve(B.key, (size_t)np)) ||
+            (rc = scratch_reserve(B.counts, sizeof(int) * 27 * (size_t)nt)) ||
+            (rc = scratch_reserve(B.base, sizeof(int) * 27 * (size_t)nt)) ||
+            (rc = scratch_reserve(B.total, sizeof(int) * (size_t)(nt + 1))) ||
+            (rc = scratch_reserve(B.sums, sizeof(int) * (size_t)nsum)) ||
+            (rc = scratch_reserve(B.ctl, sizeof(int) * BIN_CTL)) ||
+            (rc = scratch_reserve(B.list[0], sizeof(int) * (size_t)np)) ||
+            (rc = scratch_reserve(B.list[1], sizeof(int) * (size_t)np)) ||
+            (rc = scratch_reserve(B.start[0], sizeof(int) * (size_t)(nt + 1))) ||
+            (rc = scratch_reserve(B.start[1], sizeof(int) * (size_t)(nt + 1))))
+          return rc;
+        int *ctl = (int *)B.ctl.ptr;
+        if (B.np != np || B.ntiles != nt) {
+          hipLaunchKernelGGL(k_fill_int32, dim3(1), dim3(64), 0, s, ctl,
+                             BIN_CTL, 0);
+          VPIC_LAUNCHED("k_fill_int32");
+          B.np = np;
+          B.ntiles = nt;
+          B.cur = 0;
+        }
+        const int nxt = B.cur ^ 1;
+        int *counts = (int *)B.counts.ptr, *base = (int *)B.base.ptr;
+        int *total = (int *)B.total.ptr;
+        int *snext = (int *)B.start[nxt].ptr;
+        hipLaunchKernelGGL(k_bin_decide, dim3(1), dim3(64), 0, s, ctl,
+                           partition, g->nv, np, g_bin_mode);
+        VPIC_LAUNCHED("k_bin_decide");
+        hipLaunchKernelGGL(k_bin_total, dim3((nt + 1 + 255) / 256), dim3(256),
+                           0, s, (const int *)ctl, (const int *)counts, total,
+                           T);
+        VPIC_LAUNCHED("k_bin_total");
+        if ((rc = device_exclusive_scan(total, snext, nt + 1,
+                                        (int *)B.sums.ptr, s)))
+          return rc;
+        hipLaunchKernelGGL(k_bin_base, dim3((nt + 255) / 256), dim3(256), 0,
+                           s, (const int *)ctl, (const int *)counts,
+                           (const int *)snext, base, T);
+        VPIC_LAUNCHED("k_bin_base");
+        hipLaunchKernelGGL(k_bin_scatter, dim3((nt + 3) / 4), dim3(256), 0, s,
+                           ctl, partition, np,
+                           (const int *)B.list[B.cur].ptr,
+                           (const int *)B.start[B.cur].ptr,
+                           (const unsigned char *)B.key.ptr,
+                           (const int *)counts, (const int *)base,
+                           (int *)B.list[nxt].ptr, G, T);
+        VPIC_LAUNCHED("k_bin_scatter");
+        BA.ctl = ctl;
+        BA.list = (const int *)B.list[nxt].ptr;
+        BA.lstart = snext;
+        BA.key = (unsigned char *)B.key.ptr;
+        BA.counts = counts;
+        BA.oow = ctl + 4;
+        BSt = &B;
+      }
       if (timed) push_timing_record(s, np, true);
       hipLaunchKernelGGL(kern, dim3(nblk), dim3(TILE_BS), lds, s, p, np,
                          partition, f, (float *)a, (float *)S.wbuf.ptr,
                          wstride, (vpic_particle_t *)S.trash.ptr, rlist, xcap,
-                         cnt, C, G, T);
+                         cnt, C, G, T, BA);
       VPIC_LAUNCHED("k_advance_p_tiled");
       if (timed) push_timing_record(s, np, false);
+      if (BSt) {
+        hipLaunchKernelGGL(k_bin_post, dim3(1), dim3(64), 0, s,
+                           (int *)BSt->ctl.ptr, partition, g->nv, np);
+        VPIC_LAUNCHED("k_bin_post");
+        BSt->cur ^= 1;
+      }
       {
         const int64_t ncells = (int64_t)g->nx * g->ny * g->nz;
         int64_t want = (ncells + 255) / 256;

@@ -276,6 +276,8 @@ int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
   const int nv = g->nv;
   const int64_t nslots = (int64_t)nv + 1;
   int rc;
+  push_invalidate_bins(p);
+  push_invalidate_bins(aux);
   if ((rc = scratch_reserve(S.keys, sizeof(int) * (size_t)(np > 0 ? np : 1))) ||
       (rc = scratch_reserve(S.perm, sizeof(int) * (size_t)(np > 0 ? np : 1))) ||
       (rc = scratch_reserve(S.count, sizeof(int) * (size_t)nslots)) ||
