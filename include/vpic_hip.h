@@ -181,22 +181,21 @@ int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
 int vpic_hip_push_direct_runs(int32_t runs);
 
 /* Lanes that share one run of particles in the partition-driven push (1,
- * 2, 4, 8 or 16; default 4): a group reads its run's particles in turn, so
- * larger groups read wider contiguous spans.  A tuning knob; no reference
- * counterpart. */
+ * 2, 4, 8, 16, 32 or 64; default 16): a group reads its run's particles in
+ * turn, so larger groups read wider contiguous spans.  A tuning knob; no
+ * reference counterpart. */
 int vpic_hip_push_group(int32_t gs);
 
-/* Tile lists for the partition-driven push (default 0).  Each tiled push
+/* Tile lists for the partition-driven push (default 1).  Each tiled push
  * records which tile holds every particle it pushed; with lists on, the next
  * push of the same particle array gives each tile the particles now in its
  * cells, so particles that drifted since the sort still deposit into their
  * tile's LDS window.  0: off (always the sort-time partition); 1: on once
- * the last push put at least 1/128 of its deposits outside the windows,
- * until the next sort_p of that array; 2: on whenever the last push's
- * records match.  Results do not depend on the mode.  Off by default: on
- * gfx950 the per-particle records and list reads cost more than the
- * out-of-window deposits they remove.  A tuning knob; no reference
- * counterpart. */
+ * the last push made out-of-window deposits for at least 1/20 of its
+ * particles, until the next sort_p of that array (below that the records
+ * and list reads cost more than the global atomics they remove); 2: on
+ * whenever the last push's records match.  Results do not depend on the
+ * mode.  A tuning knob; no reference counterpart. */
 int vpic_hip_push_bin_mode(int32_t mode);
 
 /* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),

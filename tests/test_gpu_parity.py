@@ -369,10 +369,10 @@ TILES = [(8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
 def tile_reset():
     yield
     _lib.call("vpic_hip_push_tile", 8, 8, 4)
-    _lib.call("vpic_hip_push_group", 4)
+    _lib.call("vpic_hip_push_group", 16)
 
 
-GROUPS = [1, 2, 4, 8, 16]
+GROUPS = [1, 2, 4, 8, 16, 32, 64]
 
 
 @pytest.mark.parametrize("gs", GROUPS, ids=["gs%d" % gs for gs in GROUPS])
@@ -508,7 +508,7 @@ def test_tiled_push_tile_lists(dev, orc, mode, tile_reset):
             assert H.same_bits(dp.download()[:np_cur], po[:np_cur]), step
             assert H.acc_rel_err(da.download(), ao) <= ACC_TOL, step
     finally:
-        _lib.call("vpic_hip_push_bin_mode", 0)
+        _lib.call("vpic_hip_push_bin_mode", 1)
 
 
 def test_tiled_push_np_changed(dev, orc):

@@ -209,7 +209,7 @@ extern "C" int vpic_hip_advance_p_async(
 
 extern "C" int vpic_hip_push_group(int32_t gs) {
   CHECK(push_set_group(gs) == VPIC_OK,
-        "push lane group %d not one of 1, 2, 4, 8, 16", gs);
+        "push lane group %d not one of 1, 2, 4, 8, 16, 32, 64", gs);
   return VPIC_OK;
 }
 

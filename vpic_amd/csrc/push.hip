@@ -105,6 +105,7 @@ struct WindowSink {
   lds_u64 *lds;  // 12 * ncell
   int ncell;
   double scale;  // 2^k, set once per workgroup before the first deposit
+  float fscale;  // the same power of two
   float *acc;    // global accumulator
   int wx0, wy0, wz0, wnx, wny, wnz;
   int sx1, sy1;     // nx+2, ny+2
@@ -132,6 +133,21 @@ struct WindowSink {
     float *g = acc + 12 * (int64_t)voxel;
     if (s >= 0) {
       lds_u64 *a = lds + s;
+      // Scaling by a power of two is exact in float, so when every scaled
+      // component is below 2^31 the rounding to an integer is done in FP32
+      // (the same integer the FP64 path gives).
+      const float fs = fscale;
+      bool small = true;
+#pragma unroll
+      for (int k = 0; k < 12; k++) small = small && fabsf(j[k] * fs) < 2147483520.0f;
+      if (small) {
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+          __hip_atomic_fetch_add(a + k * ncell,
+                                 (unsigned long long)(long long)__float2int_rn(j[k] * fs),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+      }
 #pragma unroll
       for (int k = 0; k < 12; k++) {
         const double y = (double)j[k] * scale;
@@ -824,6 +840,7 @@ struct TileBox {
 //   [6] the last tiled push took its particles from the lists
 //   [7] lists stay on until the state is reset
 //   [8] the lists failed a consistency check (diagnostic)
+//   [9] this push records codes and counts
 constexpr int BIN_CTL = 16;
 
 struct BinArgs {
@@ -885,8 +902,10 @@ __device__ __forceinline__ void drain_wave_queue(
 // floats, to this tile's slot of `wbuf`; k_merge_windows then adds the
 // windows into the accumulator, so no tile needs atomics for the cells it
 // shares.
+// (launch bounds: at least 4 waves per SIMD, i.e. two 512-lane workgroups
+// per CU, which caps the kernel at 128 VGPRs)
 template <int GS>
-__global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
+__global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
     float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
@@ -912,7 +931,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
             z1 = min(z0 + T.tz - 1, G.nz);
   const int rows_y = y1 - y0 + 1;
   const int nrow = rows_y * (z1 - z0 + 1);
-  const bool keyed = Bn.key != nullptr;
+  const bool keyed =
+      Bn.key != nullptr && __builtin_amdgcn_readfirstlane(Bn.ctl[9]) != 0;
   const bool listed =
       keyed && __builtin_amdgcn_readfirstlane(Bn.ctl[5]) != 0;
 
@@ -936,7 +956,7 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   VD.mx = T.mx;
   VD.my = T.my;
   BinSink BS;
-  BS.key = Bn.key;
+  BS.key = keyed ? Bn.key : nullptr;
   BS.cnt = bin_cnt;
   TileBox &TB = BS.box;
   TB.x0 = x0;
@@ -1023,11 +1043,13 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   float4 A0, B0, A1, B1;
   load_particle(p, n, cur_ok, A0, B0);
   {
-    // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^40 for
-    // the first particle of every lane (one particle's current is below
-    // 4.4|q w|, so a lane's register sum of up to 2^11 of them, and a
-    // cell's LDS sum of up to 2^22, stay in range; a larger value takes the
-    // float path).
+    // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
+    // the first particle of every lane.  One particle's current is below
+    // 4.4|q w|, so its components scale below 2^31 and convert in FP32; a
+    // register sum of many of them may exceed 2^31 and converts in FP64
+    // (exact below 2^51: sums of up to 2^20 particles; a larger value takes
+    // the float path).  The resolution, 8|q w|max / 2^31 per deposit, is far
+    // below FP32 rounding of the cell sums.
     const bool act0 =
         cur_ok && (unsigned)__float_as_int(A0.w) < (unsigned)G.nv;
     const float qw = act0 ? fabsf(C.qsp * B0.w) : 0.0f;
@@ -1036,7 +1058,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
     const float m = __uint_as_float(qw_bits);
     int e = 0;
     if (m > 0.0f) frexpf(8.0f * m, &e);  // 8|q w|max < 2^e
-    W.scale = ldexp(1.0, 40 - e);
+    W.scale = ldexp(1.0, 31 - e);
+    W.fscale = (float)W.scale;
   }
   int it = 0;
   auto slot = [&](const float4 A, const float4 B, float4 &An, float4 &Bn_) {
@@ -1137,10 +1160,8 @@ __global__ __launch_bounds__(TILE_BS) void k_advance_p_tiled(
   if (qn)
     drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt, BS);
   __syncthreads();
-  if (keyed) {
-    if (t < 27) Bn.counts[blockIdx.x * 27 + t] = bin_cnt[t];
-    if (t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
-  }
+  if (keyed && t < 27) Bn.counts[blockIdx.x * 27 + t] = bin_cnt[t];
+  if (Bn.oow && t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
   // The window, as floats, to this tile's slot (component-major, ncell).
   float *wout = wbuf + (int64_t)blockIdx.x * wstride;
   const double inv = 1.0 / W.scale;
@@ -1220,29 +1241,34 @@ __device__ __forceinline__ void covered_range(const int *part, int nv, int np,
 
 // Whether this push takes its particles from the lists: the codes are
 // those of the last tiled push over the same covered range, and binning is
-// forced (mode 2) or the last push put at least 1/128 of its deposits
-// outside its windows (then it stays on until the state is reset).
+// forced (mode 2) or the last push made out-of-window deposits for at least
+// 1/20 of its particles (then it stays on until the state is reset; below
+// that the lists cost more than the global atomics they save).  Codes are
+// recorded from 1/40 on, so that they are ready when the lists are wanted.
 __global__ void k_bin_decide(int *ctl, const int *part, int nv, int np,
                              int mode) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int lo, hi;
   covered_range(part, nv, np, lo, hi);
   const bool ok = ctl[0] != 0 && ctl[2] == lo && ctl[3] == hi;
+  const int64_t oow = ctl[4], n = hi - lo;
   const bool want =
-      ok && mode > 0 &&
-      (mode == 2 || ctl[7] != 0 || (int64_t)ctl[4] * 128 >= (int64_t)(hi - lo));
+      ok && mode > 0 && (mode == 2 || ctl[7] != 0 || oow * 20 >= n);
+  const bool keys = mode == 2 || want || (mode > 0 && oow * 40 >= n);
   ctl[5] = want ? 1 : 0;
   ctl[7] = want ? 1 : 0;
+  ctl[9] = keys ? 1 : 0;
   ctl[4] = 0;
   ctl[8] = 0;
 }
 
-// After the push: its codes are valid for its covered range.
+// After the push: its codes (if it recorded them) are valid for its
+// covered range.
 __global__ void k_bin_post(int *ctl, const int *part, int nv, int np) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int lo, hi;
   covered_range(part, nv, np, lo, hi);
-  ctl[0] = 1;
+  ctl[0] = ctl[9];
   ctl[2] = lo;
   ctl[3] = hi;
   ctl[6] = ctl[5];
@@ -1480,8 +1506,8 @@ int pow2_at_least(int64_t n) {
 }
 
 TileGeom g_tile = {8, 8, 4, 0, 0, 0, 0, 0};
-int g_group = 4;  // lanes sharing one run of particles in the tiled push
-int g_bin_mode = 0;  // tile lists: 0 off, 1 when drift warrants, 2 always
+int g_group = 16;  // lanes sharing one run of particles in the tiled push
+int g_bin_mode = 1;  // tile lists: 0 off, 1 when drift warrants, 2 always
 
 BinState &bin_state(PushScratch &S, const void *p) {
   BinState *hit = nullptr, *lru = &S.bins[0];
@@ -1504,7 +1530,9 @@ typedef void (*TiledKernel)(vpic_particle_t *, int, const int *,
                             int, int *, PushCoef, GridDev, TileGeom, BinArgs);
 
 int tiled_kernel_id(int gs) {
-  return gs == 1 ? 0 : gs == 2 ? 1 : gs == 4 ? 2 : gs == 8 ? 3 : 4;
+  int id = 0;
+  while ((1 << id) < gs && id < 6) id++;
+  return id;  // log2 of the group size (1 .. 64)
 }
 
 TiledKernel tiled_kernel(int gs) {
@@ -1513,7 +1541,9 @@ TiledKernel tiled_kernel(int gs) {
     case 1: return k_advance_p_tiled<2>;
     case 2: return k_advance_p_tiled<4>;
     case 3: return k_advance_p_tiled<8>;
-    default: return k_advance_p_tiled<16>;
+    case 4: return k_advance_p_tiled<16>;
+    case 5: return k_advance_p_tiled<32>;
+    default: return k_advance_p_tiled<64>;
   }
 }
 
@@ -1537,8 +1567,7 @@ int push_set_tile(int tx, int ty, int tz) {
 }
 
 int push_set_group(int gs) {
-  if (gs != 1 && gs != 2 && gs != 4 && gs != 8 && gs != 16)
-    return VPIC_EINVAL;
+  if (gs < 1 || gs > 64 || (gs & (gs - 1))) return VPIC_EINVAL;
   g_group = gs;
   return VPIC_OK;
 }
@@ -1624,7 +1653,7 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
         return rc;
       const TiledKernel kern = tiled_kernel(g_group);
-      static bool attr_set[5] = {false};
+      static bool attr_set[7] = {false};
       const int ak = tiled_kernel_id(g_group);
       if (!attr_set[ak]) {
         VPIC_HIP(hipFuncSetAttribute(
