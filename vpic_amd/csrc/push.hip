@@ -1172,32 +1172,35 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
 // Add the tiles' windows into the accumulator: voxel v receives the
 // window of its own tile and, when it lies on a face of that tile, the
 // halo of the neighbouring tiles across that face (windows do not wrap).
-// One thread per interior voxel; every cell of acc is written by exactly
-// one thread.
+// One workgroup per tile, one thread per voxel of the tile, so a workgroup
+// reads its own window and the faces of its neighbours' windows; every
+// cell of acc is written by exactly one thread.
 __global__ __launch_bounds__(256) void k_merge_windows(
     float *__restrict__ acc, const float *__restrict__ wbuf, int wstride,
     GridDev G, TileGeom T) {
-  const int64_t ncells = (int64_t)G.nx * G.ny * G.nz;
-  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < ncells;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    const int x = 1 + (int)(l % G.nx);
-    const int64_t lr = l / G.nx;
-    const int y = 1 + (int)(lr % G.ny);
-    const int z = 1 + (int)(lr / G.ny);
-    const int bx = (x - 1) / T.tx, by = (y - 1) / T.ty, bz = (z - 1) / T.tz;
+  int b = blockIdx.x;
+  const int bx = b % T.ntx;
+  b /= T.ntx;
+  const int by = b % T.nty;
+  const int bz = b / T.nty;
+  const int x0 = 1 + bx * T.tx, y0 = 1 + by * T.ty, z0 = 1 + bz * T.tz;
+  const int cx = min(T.tx, G.nx - bx * T.tx), cy = min(T.ty, G.ny - by * T.ty),
+            cz = min(T.tz, G.nz - bz * T.tz);
+  for (int c = threadIdx.x; c < cx * cy * cz; c += blockDim.x) {
+    const int x = x0 + c % cx, y = y0 + (c / cx) % cy, z = z0 + c / (cx * cy);
     // tiles whose window holds v, per axis: own, and the neighbours across
     // the tile faces that v lies on (both, for a tile one cell wide)
     int ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0}, oz[3] = {0, 0, 0};
     int nox = 1, noy = 1, noz = 1;
-    if ((x - 1) % T.tx == 0 && bx > 0) ox[nox++] = -1;
-    if ((x == G.nx || (x % T.tx) == 0) && bx + 1 < T.ntx) ox[nox++] = 1;
-    if ((y - 1) % T.ty == 0 && by > 0) oy[noy++] = -1;
-    if ((y == G.ny || (y % T.ty) == 0) && by + 1 < T.nty) oy[noy++] = 1;
-    if ((z - 1) % T.tz == 0 && bz > 0) oz[noz++] = -1;
-    if ((z == G.nz || (z % T.tz) == 0) && bz + 1 < T.ntz) oz[noz++] = 1;
+    if (x == x0 && bx > 0) ox[nox++] = -1;
+    if (x == x0 + cx - 1 && bx + 1 < T.ntx) ox[nox++] = 1;
+    if (y == y0 && by > 0) oy[noy++] = -1;
+    if (y == y0 + cy - 1 && by + 1 < T.nty) oy[noy++] = 1;
+    if (z == z0 && bz > 0) oz[noz++] = -1;
+    if (z == z0 + cz - 1 && bz + 1 < T.ntz) oz[noz++] = 1;
     float s[12];
 #pragma unroll
-    for (int c = 0; c < 12; c++) s[c] = 0.0f;
+    for (int k = 0; k < 12; k++) s[k] = 0.0f;
     for (int a = 0; a < nox; a++)
       for (int bb = 0; bb < noy; bb++)
         for (int cc = 0; cc < noz; cc++) {
@@ -1211,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_merge_windows(
           const float *w =
               wbuf + (int64_t)(tx + T.ntx * (ty + T.nty * tz)) * wstride + pos;
 #pragma unroll
-          for (int c = 0; c < 12; c++) s[c] += w[c * nc];
+          for (int k = 0; k < 12; k++) s[k] += w[k * nc];
         }
     float4 *a4 = reinterpret_cast<float4 *>(
         acc + 12 * (int64_t)vox(x, y, z, G.nx, G.ny));
@@ -1735,15 +1738,9 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         VPIC_LAUNCHED("k_bin_post");
         BSt->cur ^= 1;
       }
-      {
-        const int64_t ncells = (int64_t)g->nx * g->ny * g->nz;
-        int64_t want = (ncells + 255) / 256;
-        const int mb = (int)(want < 65536 ? want : 65536);
-        hipLaunchKernelGGL(k_merge_windows, dim3(mb), dim3(256), 0, s,
-                           (float *)a, (const float *)S.wbuf.ptr, wstride, G,
-                           T);
-        VPIC_LAUNCHED("k_merge_windows");
-      }
+      hipLaunchKernelGGL(k_merge_windows, dim3(nblk), dim3(256), 0, s,
+                         (float *)a, (const float *)S.wbuf.ptr, wstride, G, T);
+      VPIC_LAUNCHED("k_merge_windows");
       // particles past the partition (added since the sort) go flat
       flat_start = partition + g->nv;
     }

@@ -240,8 +240,131 @@ __global__ __launch_bounds__(256) void k_gather(
   d[1] = b;
 }
 
+// --------------------------------------------------------------------------
+// Axis passes: the stable sort by voxel v = x + sx(y + sy z) as three stable
+// counting sorts of (key, index) pairs, by x, then y, then z (least
+// significant first).  Particles sorted at the last sort_p and moved a few
+// cells since are still nearly in voxel order, so each pass sees long runs
+// of equal digits: one single-wavefront block counts and places them with
+// LDS counters, no global atomics, and lanes of a run write consecutive
+// slots.
+// --------------------------------------------------------------------------
+constexpr int RS_BLK = 16384;  // elements per block (one wavefront)
+constexpr int RS_MAXB = 2048;  // digit values per pass (LDS counters)
+
+struct SortDig {
+  unsigned sx1, sy1;  // nx + 2, ny + 2
+  int nv;
+};
+
+__device__ __forceinline__ int axis_digit(int v, int axis, const SortDig &D) {
+  const unsigned t = (unsigned)v / D.sx1;
+  if (axis == 0) return v - (int)(t * D.sx1);
+  const unsigned z = t / D.sy1;
+  if (axis == 1) return (int)(t - z * D.sy1);
+  return (int)z;
+}
+
+// Per block, the count of each digit: hist[d * nblocks + block].  The first
+// pass reads the voxel indices from the particles and keeps them in keys.
+template <bool FIRST>
+__global__ __launch_bounds__(64) void k_rs_hist(
+    const vpic_particle_t *__restrict__ p, const int *__restrict__ keys_in,
+    int *__restrict__ keys_out, int np, SortDig D, int axis, int nbins,
+    int *__restrict__ hist, int nblocks, int *__restrict__ nbad) {
+  __shared__ int h[RS_MAXB];
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;
+  for (int d = lane; d < nbins; d += 64) h[d] = 0;
+  __syncthreads();
+  const int base = b * RS_BLK, end = min(base + RS_BLK, np);
+  int bad = 0;
+  for (int i0 = base; i0 < end; i0 += 64) {
+    const int i = i0 + lane;
+    const bool valid = i < end;
+    int key = 0;
+    if (valid) {
+      key = FIRST ? p[i].i : keys_in[i];
+      if (FIRST) {
+        if ((unsigned)key >= (unsigned)D.nv) {
+          bad++;  // never used as an index; the sort reports it
+          key = 0;
+        }
+        keys_out[i] = key;
+      }
+    }
+    const int d = valid ? axis_digit(key, axis, D) : -1;
+    // runs of equal digits over consecutive lanes: the last lane of a run
+    // adds its length
+    const int dn = __shfl_down(d, 1);
+    const int dp = __shfl_up(d, 1);
+    const bool head = lane == 0 || dp != d;
+    const bool tail = lane == 63 || dn != d;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h0 = 63 - __clzll(heads & upto);
+    if (tail && d >= 0) atomicAdd(&h[d], lane - h0 + 1);
+  }
+  if (FIRST && bad) atomicAdd(nbad, bad);
+  __syncthreads();
+  for (int d = lane; d < nbins; d += 64) hist[(int64_t)d * nblocks + b] = h[d];
+}
+
+// Place every element at offs[d * nblocks + block] + (its rank among the
+// block's elements of digit d), in index order: stable.
+template <bool FIRST>
+__global__ __launch_bounds__(64) void k_rs_scatter(
+    const int *__restrict__ keys_in, const int *__restrict__ idx_in, int np,
+    SortDig D, int axis, int nbins, int nbits, const int *__restrict__ offs,
+    int nblocks, int *__restrict__ keys_out, int *__restrict__ idx_out) {
+  __shared__ int pos[RS_MAXB];
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;
+  for (int d = lane; d < nbins; d += 64) pos[d] = offs[(int64_t)d * nblocks + b];
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int base = b * RS_BLK, end = min(base + RS_BLK, np);
+  for (int i0 = base; i0 < end; i0 += 64) {
+    const int i = i0 + lane;
+    const bool valid = i < end;
+    const int key = valid ? keys_in[i] : 0;
+    const int idx = FIRST ? i : (valid ? idx_in[i] : 0);
+    const int d = valid ? axis_digit(key, axis, D) : 0;
+    // lanes holding the same digit
+    uint64_t m = __ballot(valid);
+    for (int bit = 0; bit < nbits; bit++) {
+      const bool on = (d >> bit) & 1;
+      const uint64_t bm = __ballot(on);
+      m &= on ? bm : ~bm;
+    }
+    const int rank = __popcll(m & lt);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int p0 = 0;
+    if (valid && lane == leader) {
+      p0 = pos[d];
+      pos[d] = p0 + __popcll(m);
+    }
+    p0 = __shfl(p0, leader < 0 ? 0 : leader);
+    if (valid) {
+      keys_out[p0 + rank] = key;
+      idx_out[p0 + rank] = idx;
+    }
+  }
+}
+
+// partition[v] = first slot whose key is >= v; partition[nv] = np.
+__global__ __launch_bounds__(256) void k_rs_partition(const int *__restrict__ keys,
+                                                      int np, int nv,
+                                                      int *__restrict__ part) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j > np) return;
+  const int prev = j > 0 ? keys[j - 1] : -1;
+  const int cur = j < np ? keys[j] : nv;
+  for (int v = prev + 1; v <= cur && v <= nv; v++) part[v] = (int)j;
+}
+
 struct SortScratch {
-  DeviceScratch keys, perm, count, sums, big, nbig, tmp;
+  DeviceScratch keys, perm, count, sums, big, nbig, tmp, keys2, idx2, hist;
 };
 SortScratch &sort_scratch() {
   static thread_local SortScratch s[16];
@@ -269,15 +392,101 @@ int device_exclusive_scan(const int *in, int *out, int64_t n, int *sums,
   return VPIC_OK;
 }
 
-int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
-                  int *partition, const vpic_grid_t *g, int in_place,
-                  hipStream_t s) {
+namespace {
+
+int nbits_for(int nbins) {
+  int b = 0;
+  while ((1 << b) < nbins) b++;
+  return b;
+}
+
+// The axis-pass sort (grids up to RS_MAXB - 2 cells per axis).
+int sort_p_axes(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                int *partition, const vpic_grid_t *g, int in_place,
+                hipStream_t s) {
+  SortScratch &S = sort_scratch();
+  const int nv = g->nv;
+  const int nblocks = blocks_for(np, RS_BLK);
+  const int bins[3] = {g->nx + 2, g->ny + 2, g->nz + 2};
+  const int maxb = bins[0] > bins[1] ? (bins[0] > bins[2] ? bins[0] : bins[2])
+                                     : (bins[1] > bins[2] ? bins[1] : bins[2]);
+  const int64_t nh = (int64_t)maxb * nblocks;
+  int rc;
+  if ((rc = scratch_reserve(S.keys, sizeof(int) * (size_t)np)) ||
+      (rc = scratch_reserve(S.perm, sizeof(int) * (size_t)np)) ||
+      (rc = scratch_reserve(S.keys2, sizeof(int) * (size_t)np)) ||
+      (rc = scratch_reserve(S.idx2, sizeof(int) * (size_t)np)) ||
+      (rc = scratch_reserve(S.hist, sizeof(int) * (size_t)nh)) ||
+      (rc = scratch_reserve(S.sums,
+                            sizeof(int) * (size_t)(blocks_for(nh, SCAN_ITEMS) + 1))) ||
+      (rc = scratch_reserve(S.nbig, 64)))
+    return rc;
+  int *ka = (int *)S.keys.ptr, *ia = (int *)S.perm.ptr;
+  int *kb = (int *)S.keys2.ptr, *ib = (int *)S.idx2.ptr;
+  int *hist = (int *)S.hist.ptr, *sums = (int *)S.sums.ptr;
+  int *nbad = (int *)S.nbig.ptr;
+  SortDig D;
+  D.sx1 = (unsigned)(g->nx + 2);
+  D.sy1 = (unsigned)(g->ny + 2);
+  D.nv = nv;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, nbad, (int64_t)1, 0);
+  VPIC_LAUNCHED("k_fill_int");
+  // pass x: keys from the particles into kb, then (kb, index) -> (ka, ia)
+  hipLaunchKernelGGL(k_rs_hist<true>, dim3(nblocks), dim3(64), 0, s, p,
+                     (const int *)nullptr, kb, np, D, 0, bins[0], hist,
+                     nblocks, nbad);
+  VPIC_LAUNCHED("k_rs_hist");
+  int h_nbad = 0;
+  VPIC_HIP(hipMemcpyAsync(&h_nbad, nbad, sizeof(int), hipMemcpyDeviceToHost,
+                          s));
+  VPIC_HIP(hipStreamSynchronize(s));
+  if (h_nbad > 0)
+    return vpic_set_error(VPIC_EINVAL,
+                          "sort_p: %d particles have a voxel index outside "
+                          "[0,nv); particles left unsorted", h_nbad);
+  if ((rc = device_exclusive_scan(hist, hist, (int64_t)bins[0] * nblocks,
+                                  sums, s)))
+    return rc;
+  hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblocks), dim3(64), 0, s,
+                     (const int *)kb, (const int *)nullptr, np, D, 0, bins[0],
+                     nbits_for(bins[0]), (const int *)hist, nblocks, ka, ia);
+  VPIC_LAUNCHED("k_rs_scatter");
+  // passes y and z: (ka, ia) -> (kb, ib) -> (ka, ia)
+  for (int axis = 1; axis < 3; axis++) {
+    const int *kin = axis == 1 ? ka : kb, *iin = axis == 1 ? ia : ib;
+    int *kout = axis == 1 ? kb : ka, *iout = axis == 1 ? ib : ia;
+    hipLaunchKernelGGL(k_rs_hist<false>, dim3(nblocks), dim3(64), 0, s,
+                       (const vpic_particle_t *)nullptr, kin, (int *)nullptr,
+                       np, D, axis, bins[axis], hist, nblocks, nbad);
+    VPIC_LAUNCHED("k_rs_hist");
+    if ((rc = device_exclusive_scan(hist, hist,
+                                    (int64_t)bins[axis] * nblocks, sums, s)))
+      return rc;
+    hipLaunchKernelGGL(k_rs_scatter<false>, dim3(nblocks), dim3(64), 0, s,
+                       kin, iin, np, D, axis, bins[axis], nbits_for(bins[axis]),
+                       (const int *)hist, nblocks, kout, iout);
+    VPIC_LAUNCHED("k_rs_scatter");
+  }
+  hipLaunchKernelGGL(k_rs_partition, dim3(blocks_for((int64_t)np + 1, 256)),
+                     dim3(256), 0, s, (const int *)ka, np, nv, partition);
+  VPIC_LAUNCHED("k_rs_partition");
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for(np, 256)), dim3(256), 0, s, p,
+                     (const int *)ia, np, aux);
+  VPIC_LAUNCHED("k_gather");
+  if (in_place)
+    VPIC_HIP(hipMemcpyAsync(p, aux, sizeof(vpic_particle_t) * (size_t)np,
+                            hipMemcpyDeviceToDevice, s));
+  return VPIC_OK;
+}
+
+// The counting sort with global atomics (any grid).
+int sort_p_counting(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                    int *partition, const vpic_grid_t *g, int in_place,
+                    hipStream_t s) {
   SortScratch &S = sort_scratch();
   const int nv = g->nv;
   const int64_t nslots = (int64_t)nv + 1;
   int rc;
-  push_invalidate_bins(p);
-  push_invalidate_bins(aux);
   if ((rc = scratch_reserve(S.keys, sizeof(int) * (size_t)(np > 0 ? np : 1))) ||
       (rc = scratch_reserve(S.perm, sizeof(int) * (size_t)(np > 0 ? np : 1))) ||
       (rc = scratch_reserve(S.count, sizeof(int) * (size_t)nslots)) ||
@@ -344,3 +553,19 @@ int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
                             hipMemcpyDeviceToDevice, s));
   return VPIC_OK;
 }
+
+}  // namespace
+
+// sort_p: the axis passes when every axis fits their LDS counters, else the
+// counting sort.
+int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                  int *partition, const vpic_grid_t *g, int in_place,
+                  hipStream_t s) {
+  push_invalidate_bins(p);
+  push_invalidate_bins(aux);
+  if (np > 0 && g->nx + 2 <= RS_MAXB && g->ny + 2 <= RS_MAXB &&
+      g->nz + 2 <= RS_MAXB)
+    return sort_p_axes(p, aux, np, partition, g, in_place, s);
+  return sort_p_counting(p, aux, np, partition, g, in_place, s);
+}
+
