@@ -279,19 +279,25 @@ __global__ __launch_bounds__(64) void k_rs_hist(
   __syncthreads();
   const int base = b * RS_BLK, end = min(base + RS_BLK, np);
   int bad = 0;
-  for (int i0 = base; i0 < end; i0 += 64) {
-    const int i = i0 + lane;
+  constexpr int U = 4;  // rounds whose loads are in flight together
+  for (int i0 = base; i0 < end; i0 += 64 * U) {
+    int kk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * 64 + lane;
+      kk[u] = i < end ? (FIRST ? p[i].i : keys_in[i]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+    const int i = i0 + u * 64 + lane;
     const bool valid = i < end;
-    int key = 0;
-    if (valid) {
-      key = FIRST ? p[i].i : keys_in[i];
-      if (FIRST) {
-        if ((unsigned)key >= (unsigned)D.nv) {
-          bad++;  // never used as an index; the sort reports it
-          key = 0;
-        }
-        keys_out[i] = key;
+    int key = kk[u];
+    if (valid && FIRST) {
+      if ((unsigned)key >= (unsigned)D.nv) {
+        bad++;  // never used as an index; the sort reports it
+        key = 0;
       }
+      keys_out[i] = key;
     }
     const int d = valid ? axis_digit(key, axis, D) : -1;
     // runs of equal digits over consecutive lanes: the last lane of a run
@@ -304,6 +310,7 @@ __global__ __launch_bounds__(64) void k_rs_hist(
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const int h0 = 63 - __clzll(heads & upto);
     if (tail && d >= 0) atomicAdd(&h[d], lane - h0 + 1);
+    }
   }
   if (FIRST && bad) atomicAdd(nbad, bad);
   __syncthreads();
@@ -324,11 +331,21 @@ __global__ __launch_bounds__(64) void k_rs_scatter(
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
   const int base = b * RS_BLK, end = min(base + RS_BLK, np);
-  for (int i0 = base; i0 < end; i0 += 64) {
-    const int i = i0 + lane;
+  constexpr int U = 4;  // rounds whose loads are in flight together
+  for (int i0 = base; i0 < end; i0 += 64 * U) {
+    int kk[U], ii[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * 64 + lane;
+      kk[u] = i < end ? keys_in[i] : 0;
+      ii[u] = FIRST ? i : (i < end ? idx_in[i] : 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+    const int i = i0 + u * 64 + lane;
     const bool valid = i < end;
-    const int key = valid ? keys_in[i] : 0;
-    const int idx = FIRST ? i : (valid ? idx_in[i] : 0);
+    const int key = kk[u];
+    const int idx = ii[u];
     const int d = valid ? axis_digit(key, axis, D) : 0;
     // lanes holding the same digit
     uint64_t m = __ballot(valid);
@@ -349,18 +366,42 @@ __global__ __launch_bounds__(64) void k_rs_scatter(
       keys_out[p0 + rank] = key;
       idx_out[p0 + rank] = idx;
     }
+    }
   }
 }
 
-// partition[v] = first slot whose key is >= v; partition[nv] = np.
-__global__ __launch_bounds__(256) void k_rs_partition(const int *__restrict__ keys,
-                                                      int np, int nv,
-                                                      int *__restrict__ part) {
-  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (j > np) return;
-  const int prev = j > 0 ? keys[j - 1] : -1;
-  const int cur = j < np ? keys[j] : nv;
-  for (int v = prev + 1; v <= cur && v <= nv; v++) part[v] = (int)j;
+// out[j] = p[perm[j]] for the 4 x 256 slots of a block (each load and
+// store instruction covers consecutive slots; four in flight per thread),
+// and partition[v] = first slot whose key is >= v (partition[nv] = np),
+// from the sorted keys.
+__global__ __launch_bounds__(256) void k_gather_part(
+    const vpic_particle_t *__restrict__ src, const int *__restrict__ perm,
+    const int *__restrict__ keys, int np, int nv,
+    vpic_particle_t *__restrict__ dst, int *__restrict__ part) {
+  const int64_t j0 = 1024 * (int64_t)blockIdx.x + threadIdx.x;
+  float4 a[4], b[4];
+  int kc[4], kp[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int64_t j = j0 + 256 * u;
+    const int q = j < np ? perm[j] : 0;
+    kc[u] = j < np ? keys[j] : nv;
+    kp[u] = j == 0 ? -1 : (j <= np ? keys[j - 1] : nv);
+    const float4 *sp = reinterpret_cast<const float4 *>(src + q);
+    a[u] = sp[0];
+    b[u] = sp[1];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int64_t j = j0 + 256 * u;
+    if (j < np) {
+      float4 *d = reinterpret_cast<float4 *>(dst + j);
+      d[0] = a[u];
+      d[1] = b[u];
+    }
+    if (j <= np)
+      for (int v = kp[u] + 1; v <= kc[u] && v <= nv; v++) part[v] = (int)j;
+  }
 }
 
 struct SortScratch {
@@ -467,12 +508,10 @@ int sort_p_axes(vpic_particle_t *p, vpic_particle_t *aux, int np,
                        (const int *)hist, nblocks, kout, iout);
     VPIC_LAUNCHED("k_rs_scatter");
   }
-  hipLaunchKernelGGL(k_rs_partition, dim3(blocks_for((int64_t)np + 1, 256)),
-                     dim3(256), 0, s, (const int *)ka, np, nv, partition);
-  VPIC_LAUNCHED("k_rs_partition");
-  hipLaunchKernelGGL(k_gather, dim3(blocks_for(np, 256)), dim3(256), 0, s, p,
-                     (const int *)ia, np, aux);
-  VPIC_LAUNCHED("k_gather");
+  hipLaunchKernelGGL(k_gather_part, dim3(blocks_for((int64_t)np + 1, 1024)),
+                     dim3(256), 0, s, p, (const int *)ia, (const int *)ka, np,
+                     nv, aux, partition);
+  VPIC_LAUNCHED("k_gather_part");
   if (in_place)
     VPIC_HIP(hipMemcpyAsync(p, aux, sizeof(vpic_particle_t) * (size_t)np,
                             hipMemcpyDeviceToDevice, s));
