@@ -260,9 +260,11 @@ void unload_accumulator_array(field_array_t *fa,
 /* (sfa.cc:191-216).  These are the HIP versions of the vacuum kernels.     */
 /* ------------------------------------------------------------------------ */
 
-/* Replace the entries of fa->kernel that this library implements
- * (advance_b, advance_e, energy_f, clear_jf, synchronize_jf, clear_rhof,
- * synchronize_rho) with the functions below.  fa must come from
+/* Replace the entries of fa->kernel with the functions below (every entry
+ * but delete_fa: advance_b, advance_e, energy_f, clear_jf, synchronize_jf,
+ * clear_rhof, synchronize_rho, compute_rhob, compute_curl_b,
+ * synchronize_tang_e_norm_b, compute_div_e_err, compute_rms_div_e_err,
+ * clean_div_e, compute_div_b_err, compute_rms_div_b_err, clean_div_b).  fa must come from
  * new_standard_field_array with ONE material (the vacuum kernels,
  * sfa.cc:205-214) and fa->params must be its sfa_params_t.  The other
  * entries are left as they are.  Returns 0, or < 0 with
@@ -282,6 +284,25 @@ void vpic_hip_fa_clear_rhof(field_array_t *fa);
  * 298-334. */
 void vpic_hip_fa_synchronize_jf(field_array_t *fa);
 void vpic_hip_fa_synchronize_rho(field_array_t *fa);
+/* vacuum_compute_rhob.cc -> vacuum_compute_rhob_pipeline.cc:41-175. */
+void vpic_hip_fa_vacuum_compute_rhob(field_array_t *fa);
+/* vacuum_compute_curl_b.cc -> vacuum_compute_curl_b_pipeline.cc:50-326. */
+void vpic_hip_fa_vacuum_compute_curl_b(field_array_t *fa);
+/* remote.cc:298-416 (one domain). */
+double vpic_hip_fa_synchronize_tang_e_norm_b(field_array_t *fa);
+/* vacuum_compute_div_e_err.cc -> vacuum_compute_div_e_err_pipeline.cc:
+ * 61-174. */
+void vpic_hip_fa_vacuum_compute_div_e_err(field_array_t *fa);
+/* compute_rms_div_e_err.cc -> compute_rms_div_e_err_pipeline.cc:64-185. */
+double vpic_hip_fa_compute_rms_div_e_err(const field_array_t *fa);
+/* vacuum_clean_div_e.cc -> vacuum_clean_div_e_pipeline.cc:72-147. */
+void vpic_hip_fa_vacuum_clean_div_e(field_array_t *fa);
+/* compute_div_b_err.cc -> compute_div_b_err_pipeline.cc:260-296. */
+void vpic_hip_fa_compute_div_b_err(field_array_t *fa);
+/* compute_rms_div_b_err.cc -> compute_rms_div_b_err_pipeline.cc:53-111. */
+double vpic_hip_fa_compute_rms_div_b_err(const field_array_t *fa);
+/* clean_div_b.cc -> clean_div_b_pipeline.cc:230-328. */
+void vpic_hip_fa_clean_div_b(field_array_t *fa);
 
 /* ------------------------------------------------------------------------ */
 /* Residency control (no reference counterpart: the reference's arrays     */

@@ -112,6 +112,7 @@ typedef struct vpic_field_params {
   float rmux, rmuy, rmuz;
   float epsx, epsy, epsz;
   float damp;
+  float nonconductive; /* 1 when sigma = 0 (divergence cleaning weight) */
 } vpic_field_params_t;
 
 /* ------------------------------------------------------------------------ */
@@ -304,6 +305,57 @@ int vpic_hip_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
 int vpic_hip_energy_f(double host_en[6], const vpic_field_t *f,
                       const vpic_grid_t *g, const vpic_field_params_t *mp,
                       void *stream);
+
+/* Divergence cleaning and initialization kernels of the vacuum field table
+ * (standard/sfa.cc:203-210; field_advance.h:199-216).  Field results are
+ * bit-identical to the reference's; the FP64 sums (rms, err) are summed in
+ * another order.  The rms and err entry points synchronise. */
+
+/* vacuum_compute_div_e_err: normal-E ghosts, div_e_err on every node, then
+ * local_adjust_div_e (vacuum_compute_div_e_err_pipeline.h:17-50,
+ * .cc:61-174; standard/local.cc:129-179,299-334; remote.cc:137-207). */
+int vpic_hip_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
+                               const vpic_field_params_t *mp, void *stream);
+
+/* compute_rms_div_e_err (compute_rms_div_e_err_pipeline.cc:16-185). */
+int vpic_hip_compute_rms_div_e_err(double *host_rms, const vpic_field_t *f,
+                                   const vpic_grid_t *g, void *stream);
+
+/* vacuum_clean_div_e: Marder pass on every E edge, then
+ * local_adjust_tang_e (vacuum_clean_div_e_pipeline.h:16-55, .cc:43-147). */
+int vpic_hip_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
+                         const vpic_field_params_t *mp, void *stream);
+
+/* compute_div_b_err (compute_div_b_err_pipeline.cc:15-57,260-296). */
+int vpic_hip_compute_div_b_err(vpic_field_t *f, const vpic_grid_t *g,
+                               void *stream);
+
+/* compute_rms_div_b_err (compute_rms_div_b_err_pipeline.cc:14-111). */
+int vpic_hip_compute_rms_div_b_err(double *host_rms, const vpic_field_t *f,
+                                   const vpic_grid_t *g, void *stream);
+
+/* clean_div_b: div_b_err ghosts, Marder pass on every B face, then
+ * local_adjust_norm_b (clean_div_b_pipeline.h:17-27, .cc:230-328;
+ * standard/local.cc:182-221,266-296; remote.cc:209-281). */
+int vpic_hip_clean_div_b(vpic_field_t *f, const vpic_grid_t *g, void *stream);
+
+/* vacuum_compute_rhob: normal-E ghosts, rhob on every node, then
+ * local_adjust_rhob (vacuum_compute_rhob_pipeline.h:17-52, .cc:41-175;
+ * standard/local.cc:415-440). */
+int vpic_hip_compute_rhob(vpic_field_t *f, const vpic_grid_t *g,
+                          const vpic_field_params_t *mp, void *stream);
+
+/* vacuum_compute_curl_b: tangential-B ghosts, tca = curl B on every edge,
+ * then local_adjust_tang_e (vacuum_compute_curl_b_pipeline.h:17-66,
+ * .cc:50-326). */
+int vpic_hip_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
+                            const vpic_field_params_t *mp, void *stream);
+
+/* synchronize_tang_e_norm_b for one domain: local adjusts, then each
+ * periodic face pair averages cb normal, e and tca tangential; *host_err
+ * receives the summed squared differences (standard/remote.cc:298-416). */
+int vpic_hip_synchronize_tang_e_norm_b(double *host_err, vpic_field_t *f,
+                                       const vpic_grid_t *g, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Set-up helpers (no reference counterpart on the device; they fill the   */

@@ -899,6 +899,7 @@ void orc_field_params_vacuum(vpic_field_params_t *mp, float damp) {
   mp->rmux = mp->rmuy = mp->rmuz = (float)(1. / 1.0);
   mp->epsx = mp->epsy = mp->epsz = 1.0f;
   mp->damp = damp;
+  mp->nonconductive = 1.0f; /* sfa.cc:144 with ax = ay = az = 0 */
 }
 
 void orc_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
@@ -1133,4 +1134,326 @@ void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
     f[v + sz + sy].rhof += w6;
     f[v + sz + sy + 1].rhof += w7;
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Divergence cleaning and initialization kernels of the vacuum field table */
+/* (sfa.cc:203-210)                                                         */
+/* ------------------------------------------------------------------------ */
+
+static float *fslot(vpic_field_t *f, int k) { return (float *)f + k; }
+
+/* Normal-E ghosts (and tca for local faces) of every face: periodic faces
+ * take the opposite interior node plane with rw = 1, lw = 0
+ * (remote.cc:137-207); local faces copy (anti-symmetric) or negate
+ * (symmetric, pmc) the adjacent plane (local.cc:129-179).  Ghost node
+ * planes only read interior planes, so the order does not matter. */
+static void ghost_norm_e(vpic_field_t *f, const vpic_grid_t *g) {
+  for (int k = 0; k < 6; k++) {
+    const int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const int hi = FACE_HIGH(k), bc = g->fbc[k];
+    const int ghost = hi ? nd(g, X) + 1 : 0;
+    int c[3], s[3];
+    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+        c[X] = ghost; c[Y] = yy; c[Z] = zz;
+        s[0] = c[0]; s[1] = c[1]; s[2] = c[2];
+        vpic_field_t *f0 = f + fvox(g, c);
+        if (bc == VPIC_BC_PERIODIC) {
+          /* the opposite interior plane; lw multiplies the adjacent one */
+          s[X] = hi ? 1 : nd(g, X);
+          int a[3] = {c[0], c[1], c[2]};
+          a[X] = hi ? nd(g, X) : 1;
+          const float rw = 1.0f, lw = 0.0f;
+          *fcomp(f0, 0, X) = rw * *fcomp(f + fvox(g, s), 0, X) +
+                             lw * *fcomp(f + fvox(g, a), 0, X);
+        } else {
+          s[X] = hi ? nd(g, X) : 1;
+          vpic_field_t *f1 = f + fvox(g, s);
+          const float e = *fcomp(f1, 0, X), t = *fcomp(f1, 2, X);
+          if (bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS) {
+            *fcomp(f0, 0, X) = e;
+            *fcomp(f0, 2, X) = t;
+          } else if (bc == VPIC_BC_ABSORB_FIELDS) {
+            s[X] = hi ? nd(g, X) - 1 : 2;
+            vpic_field_t *f2 = f + fvox(g, s);
+            *fcomp(f0, 0, X) = 2 * e - *fcomp(f2, 0, X);
+            *fcomp(f0, 2, X) = 2 * t - *fcomp(f2, 2, X);
+          } else {
+            *fcomp(f0, 0, X) = -e;
+            *fcomp(f0, 2, X) = -t;
+          }
+        }
+      }
+  }
+}
+
+/* Zero field slot `slot` (floats into field_t) on the nodes of every
+ * anti-symmetric face, and of every absorbing face when `absorb`
+ * (local_adjust_div_e, local.cc:299-334; local_adjust_rhob,
+ * local.cc:415-440). */
+static void zero_node_faces(vpic_field_t *f, const vpic_grid_t *g, int slot,
+                            int absorb) {
+  for (int k = 0; k < 6; k++) {
+    if (g->fbc[k] != VPIC_BC_ANTI_SYMMETRIC_FIELDS &&
+        !(absorb && g->fbc[k] == VPIC_BC_ABSORB_FIELDS))
+      continue;
+    const int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
+    int c[3];
+    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+        c[X] = face; c[Y] = yy; c[Z] = zz;
+        *fslot(f + fvox(g, c), slot) = 0;
+      }
+  }
+}
+
+void orc_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
+                           const vpic_field_params_t *m) {
+  /* vacuum_compute_div_e_err_pipeline.h:17-50, .cc:61-174 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const float nc = m->nonconductive;
+  const float px = ((nx > 1) ? g->rdx : 0) * m->epsx;
+  const float py = ((ny > 1) ? g->rdy : 0) * m->epsy;
+  const float pz = ((nz > 1) ? g->rdz : 0) * m->epsz;
+  const float cj = 1. / g->eps0;
+  ghost_norm_e(f, g);
+  for (int z = 1; z <= nz + 1; z++)
+    for (int y = 1; y <= ny + 1; y++)
+      for (int x = 1; x <= nx + 1; x++) {
+        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
+        const vpic_field_t *fx = f + VOX(x - 1, y, z, nx, ny);
+        const vpic_field_t *fy = f + VOX(x, y - 1, z, nx, ny);
+        const vpic_field_t *fz = f + VOX(x, y, z - 1, nx, ny);
+        f0->div_e_err = nc * (px * (f0->ex - fx->ex) + py * (f0->ey - fy->ey) +
+                              pz * (f0->ez - fz->ez) - cj * (f0->rhof + f0->rhob));
+      }
+  zero_node_faces(f, g, 3, 1);
+}
+
+void orc_compute_rhob(vpic_field_t *f, const vpic_grid_t *g,
+                      const vpic_field_params_t *m) {
+  /* vacuum_compute_rhob_pipeline.h:17-52, .cc:41-175 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const float nc = m->nonconductive;
+  const float px = (nx > 1) ? g->eps0 * m->epsx * g->rdx : 0;
+  const float py = (ny > 1) ? g->eps0 * m->epsy * g->rdy : 0;
+  const float pz = (nz > 1) ? g->eps0 * m->epsz * g->rdz : 0;
+  ghost_norm_e(f, g);
+  for (int z = 1; z <= nz + 1; z++)
+    for (int y = 1; y <= ny + 1; y++)
+      for (int x = 1; x <= nx + 1; x++) {
+        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
+        const vpic_field_t *fx = f + VOX(x - 1, y, z, nx, ny);
+        const vpic_field_t *fy = f + VOX(x, y - 1, z, nx, ny);
+        const vpic_field_t *fz = f + VOX(x, y, z - 1, nx, ny);
+        f0->rhob = nc * (px * (f0->ex - fx->ex) + py * (f0->ey - fy->ey) +
+                         pz * (f0->ez - fz->ez) - f0->rhof);
+      }
+  zero_node_faces(f, g, 11, 0);
+}
+
+/* Weighted sum of div_e_err^2 over the nodes 1..n+1: interior nodes in
+ * FP32 products, boundary nodes in FP64 with 1/2, 1/4, 1/8 on faces,
+ * edges and corners (compute_rms_div_e_err_pipeline.cc:16-180). */
+double orc_compute_rms_div_e_err(const vpic_field_t *f, const vpic_grid_t *g) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  double err = 0;
+  for (int z = 2; z <= nz; z++)
+    for (int y = 2; y <= ny; y++)
+      for (int x = 2; x <= nx; x++) {
+        const float d = f[VOX(x, y, z, nx, ny)].div_e_err;
+        err += d * d;
+      }
+  for (int z = 1; z <= nz + 1; z++)
+    for (int y = 1; y <= ny + 1; y++)
+      for (int x = 1; x <= nx + 1; x++) {
+        const int bx = x == 1 || x == nx + 1, by = y == 1 || y == ny + 1,
+                  bz = z == 1 || z == nz + 1;
+        const int nb = bx + by + bz;
+        if (!nb) continue;
+        const double w = nb == 1 ? 0.5 : nb == 2 ? 0.25 : 0.125;
+        const double d = f[VOX(x, y, z, nx, ny)].div_e_err;
+        err += w * d * d;
+      }
+  double local0 = err * g->dV;
+  double local1 = (g->nx * g->ny * g->nz) * g->dV;
+  return g->eps0 * sqrt(local0 / local1);
+}
+
+void orc_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
+                     const vpic_field_params_t *m) {
+  /* vacuum_clean_div_e_pipeline.h:16-55, .cc:43-145 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const float _rdx = (nx > 1) ? g->rdx : 0;
+  const float _rdy = (ny > 1) ? g->rdy : 0;
+  const float _rdz = (nz > 1) ? g->rdz : 0;
+  const float alphadt = 0.3888889 / (_rdx * _rdx + _rdy * _rdy + _rdz * _rdz);
+  const float px = (alphadt * _rdx) * m->drivex;
+  const float py = (alphadt * _rdy) * m->drivey;
+  const float pz = (alphadt * _rdz) * m->drivez;
+  for (int z = 1; z <= nz + 1; z++)
+    for (int y = 1; y <= ny + 1; y++)
+      for (int x = 1; x <= nx + 1; x++) {
+        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
+        if (x <= nx)
+          f0->ex += px * (f[VOX(x + 1, y, z, nx, ny)].div_e_err - f0->div_e_err);
+        if (y <= ny)
+          f0->ey += py * (f[VOX(x, y + 1, z, nx, ny)].div_e_err - f0->div_e_err);
+        if (z <= nz)
+          f0->ez += pz * (f[VOX(x, y, z + 1, nx, ny)].div_e_err - f0->div_e_err);
+      }
+  local_adjust_tang_e(f, g);
+}
+
+void orc_compute_div_b_err(vpic_field_t *f, const vpic_grid_t *g) {
+  /* compute_div_b_err_pipeline.cc:15-57 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const float px = (nx > 1) ? g->rdx : 0;
+  const float py = (ny > 1) ? g->rdy : 0;
+  const float pz = (nz > 1) ? g->rdz : 0;
+  for (int z = 1; z <= nz; z++)
+    for (int y = 1; y <= ny; y++)
+      for (int x = 1; x <= nx; x++) {
+        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
+        f0->div_b_err = px * (f[VOX(x + 1, y, z, nx, ny)].cbx - f0->cbx) +
+                        py * (f[VOX(x, y + 1, z, nx, ny)].cby - f0->cby) +
+                        pz * (f[VOX(x, y, z + 1, nx, ny)].cbz - f0->cbz);
+      }
+}
+
+double orc_compute_rms_div_b_err(const vpic_field_t *f, const vpic_grid_t *g) {
+  /* compute_rms_div_b_err_pipeline.cc:14-111 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  double err = 0;
+  for (int z = 1; z <= nz; z++)
+    for (int y = 1; y <= ny; y++)
+      for (int x = 1; x <= nx; x++) {
+        const float d = f[VOX(x, y, z, nx, ny)].div_b_err;
+        err += d * d;
+      }
+  double local0 = err * g->dV;
+  double local1 = (g->nx * g->ny * g->nz) * g->dV;
+  return g->eps0 * sqrt(local0 / local1);
+}
+
+/* div_b_err ghosts (cell centred): periodic faces take the opposite
+ * interior cell plane (remote.cc:209-281); local faces copy (anti-
+ * symmetric), negate (symmetric, pmc) or zero (absorbing) the adjacent
+ * plane (local.cc:182-221). */
+static void ghost_div_b(vpic_field_t *f, const vpic_grid_t *g) {
+  for (int k = 0; k < 6; k++) {
+    const int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const int hi = FACE_HIGH(k), bc = g->fbc[k];
+    const int ghost = hi ? nd(g, X) + 1 : 0;
+    int c[3], s[3], a[3];
+    for (int zz = 1; zz <= nd(g, Z); zz++)
+      for (int yy = 1; yy <= nd(g, Y); yy++) {
+        c[X] = ghost; c[Y] = yy; c[Z] = zz;
+        s[0] = a[0] = c[0]; s[1] = a[1] = c[1]; s[2] = a[2] = c[2];
+        vpic_field_t *f0 = f + fvox(g, c);
+        if (bc == VPIC_BC_PERIODIC) {
+          s[X] = hi ? 1 : nd(g, X);
+          a[X] = hi ? nd(g, X) : 1;
+          const float rw = 1.0f, lw = 0.0f;
+          f0->div_b_err = rw * f[fvox(g, s)].div_b_err + lw * f[fvox(g, a)].div_b_err;
+        } else {
+          s[X] = hi ? nd(g, X) : 1;
+          const float d = f[fvox(g, s)].div_b_err;
+          f0->div_b_err = bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS ? d
+                          : bc == VPIC_BC_ABSORB_FIELDS      ? 0.0f
+                                                             : -d;
+        }
+      }
+  }
+}
+
+void orc_clean_div_b(vpic_field_t *f, const vpic_grid_t *g) {
+  /* clean_div_b_pipeline.h:17-27, clean_div_b_pipeline.cc:230-328 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  float px = (nx > 1) ? g->rdx : 0;
+  float py = (ny > 1) ? g->rdy : 0;
+  float pz = (nz > 1) ? g->rdz : 0;
+  const float alphadt = 0.3888889 / (px * px + py * py + pz * pz);
+  px *= alphadt;
+  py *= alphadt;
+  pz *= alphadt;
+  ghost_div_b(f, g);
+  for (int z = 1; z <= nz + 1; z++)
+    for (int y = 1; y <= ny + 1; y++)
+      for (int x = 1; x <= nx + 1; x++) {
+        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
+        if (y <= ny && z <= nz)
+          f0->cbx += px * (f0->div_b_err - f[VOX(x - 1, y, z, nx, ny)].div_b_err);
+        if (x <= nx && z <= nz)
+          f0->cby += py * (f0->div_b_err - f[VOX(x, y - 1, z, nx, ny)].div_b_err);
+        if (x <= nx && y <= ny)
+          f0->cbz += pz * (f0->div_b_err - f[VOX(x, y, z - 1, nx, ny)].div_b_err);
+      }
+  local_adjust_norm_b(f, g);
+}
+
+void orc_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
+                        const vpic_field_params_t *m) {
+  /* vacuum_compute_curl_b_pipeline.h:17-66, .cc:50-326 */
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const float px_muz = ((nx > 1) ? g->cvac * g->dt * g->rdx : 0) * m->rmuz;
+  const float px_muy = ((nx > 1) ? g->cvac * g->dt * g->rdx : 0) * m->rmuy;
+  const float py_mux = ((ny > 1) ? g->cvac * g->dt * g->rdy : 0) * m->rmux;
+  const float py_muz = ((ny > 1) ? g->cvac * g->dt * g->rdy : 0) * m->rmuz;
+  const float pz_muy = ((nz > 1) ? g->cvac * g->dt * g->rdz : 0) * m->rmuy;
+  const float pz_mux = ((nz > 1) ? g->cvac * g->dt * g->rdz : 0) * m->rmux;
+  remote_ghost_tang_b(f, g);
+  local_ghost_tang_b(f, g);
+  for (int z = 1; z <= nz + 1; z++)
+    for (int y = 1; y <= ny + 1; y++)
+      for (int x = 1; x <= nx + 1; x++) {
+        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
+        const vpic_field_t *fx = f + VOX(x - 1, y, z, nx, ny);
+        const vpic_field_t *fy = f + VOX(x, y - 1, z, nx, ny);
+        const vpic_field_t *fz = f + VOX(x, y, z - 1, nx, ny);
+        if (x <= nx)
+          f0->tcax = (py_muz * (f0->cbz - fy->cbz) - pz_muy * (f0->cby - fz->cby));
+        if (y <= ny)
+          f0->tcay = (pz_mux * (f0->cbx - fz->cbx) - px_muz * (f0->cbz - fx->cbz));
+        if (z <= nz)
+          f0->tcaz = (px_muy * (f0->cby - fx->cby) - py_mux * (f0->cbx - fy->cbx));
+      }
+  local_adjust_tang_e(f, g);
+}
+
+/* synchronize_tang_e_norm_b for one domain (remote.cc:298-416): local
+ * adjusts, then for the x, y and z periodic face pairs in turn both faces
+ * take 0.5*(w1+w2) in FP64 of cb normal, e and tca tangential; err sums
+ * (w1-w2)^2 of cb and e, once from each face. */
+double orc_synchronize_tang_e_norm_b(vpic_field_t *f, const vpic_grid_t *g) {
+  double err = 0;
+  local_adjust_tang_e(f, g);
+  local_adjust_norm_b(f, g);
+  for (int X = 0; X < 3; X++) {
+    if (g->fbc[X] != VPIC_BC_PERIODIC || g->fbc[X + 3] != VPIC_BC_PERIODIC)
+      continue;
+    const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const int lo = 1, hi = nd(g, X) + 1;
+    int a[3], b[3];
+    /* (slot block, component, Y extra, Z extra, counts in err) */
+    const int parts[5][5] = {{1, X, 0, 0, 1}, {0, Y, 0, 1, 1}, {2, Y, 0, 1, 0},
+                             {0, Z, 1, 0, 1}, {2, Z, 1, 0, 0}};
+    for (int q = 0; q < 5; q++) {
+      const int blk = parts[q][0], comp = parts[q][1];
+      for (int zz = 1; zz <= nd(g, Z) + parts[q][3]; zz++)
+        for (int yy = 1; yy <= nd(g, Y) + parts[q][2]; yy++) {
+          a[X] = lo; a[Y] = yy; a[Z] = zz;
+          b[X] = hi; b[Y] = yy; b[Z] = zz;
+          float *pa = fcomp(f + fvox(g, a), blk, comp);
+          float *pb = fcomp(f + fvox(g, b), blk, comp);
+          const double w1 = *pa, w2 = *pb;
+          *pa = 0.5 * (w1 + w2);
+          *pb = 0.5 * (w2 + w1);
+          if (parts[q][4]) err += 2 * ((w1 - w2) * (w1 - w2));
+        }
+    }
+  }
+  return err;
 }

@@ -117,6 +117,33 @@ void orc_center_p(vpic_particle_t *p, int np, float q, float m,
 void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
                           float q, const vpic_grid_t *g);
 
+/* The divergence-cleaning and initialization kernels of the vacuum field
+ * table (sfa.cc:203-210), each over the whole local domain with its
+ * ghosts and local adjusts:
+ *   compute_div_e_err  vacuum_compute_div_e_err_pipeline.{h:17-50,cc:61-174}
+ *   compute_rhob       vacuum_compute_rhob_pipeline.{h:17-52,cc:41-175}
+ *   compute_rms_div_e_err  compute_rms_div_e_err_pipeline.cc:16-180
+ *   clean_div_e        vacuum_clean_div_e_pipeline.{h:16-55,cc:43-145}
+ *   compute_div_b_err  compute_div_b_err_pipeline.cc:15-57
+ *   compute_rms_div_b_err  compute_rms_div_b_err_pipeline.cc:14-111
+ *   clean_div_b        clean_div_b_pipeline.{h:17-27,cc:230-328}
+ *   compute_curl_b     vacuum_compute_curl_b_pipeline.{h:17-66,cc:50-326}
+ *   synchronize_tang_e_norm_b  remote.cc:298-416 (one domain)
+ * with the ghosts of local.cc:129-221 and remote.cc:137-281. */
+void orc_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
+                           const vpic_field_params_t *mp);
+void orc_compute_rhob(vpic_field_t *f, const vpic_grid_t *g,
+                      const vpic_field_params_t *mp);
+double orc_compute_rms_div_e_err(const vpic_field_t *f, const vpic_grid_t *g);
+void orc_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
+                     const vpic_field_params_t *mp);
+void orc_compute_div_b_err(vpic_field_t *f, const vpic_grid_t *g);
+double orc_compute_rms_div_b_err(const vpic_field_t *f, const vpic_grid_t *g);
+void orc_clean_div_b(vpic_field_t *f, const vpic_grid_t *g);
+void orc_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
+                        const vpic_field_params_t *mp);
+double orc_synchronize_tang_e_norm_b(vpic_field_t *f, const vpic_grid_t *g);
+
 /* The reference's CPU parallel scheme for advance_p, used only as the
  * timed CPU baseline: n_thread pthreads each run one DISTRIBUTE(np,16)
  * block into a private accumulator copy, the caller runs the np%16 tail,

@@ -85,6 +85,18 @@ class Oracle:
                                      ctypes.POINTER(T.FieldParams)]
         lib.orc_accumulate_rho_p.argtypes = [_vp, _vp, _i32, _f32, _pg]
         lib.orc_synchronize_rho.argtypes = [_vp, _pg]
+        for name in ("orc_compute_div_e_err", "orc_compute_rhob",
+                     "orc_clean_div_e", "orc_compute_curl_b"):
+            getattr(lib, name).argtypes = [_vp, _pg,
+                                           ctypes.POINTER(T.FieldParams)]
+            getattr(lib, name).restype = None
+        for name in ("orc_compute_div_b_err", "orc_clean_div_b"):
+            getattr(lib, name).argtypes = [_vp, _pg]
+            getattr(lib, name).restype = None
+        for name in ("orc_compute_rms_div_e_err", "orc_compute_rms_div_b_err",
+                     "orc_synchronize_tang_e_norm_b"):
+            getattr(lib, name).argtypes = [_vp, _pg]
+            getattr(lib, name).restype = ctypes.c_double
 
     @classmethod
     def get(cls):
@@ -167,6 +179,37 @@ class Oracle:
                               ctypes.byref(mp))
         return en
 
+    # divergence cleaning / initialization kernels of the vacuum table
+    def compute_div_e_err(self, f, g, mp):
+        self.lib.orc_compute_div_e_err(_ptr(f), ctypes.byref(g),
+                                       ctypes.byref(mp))
+
+    def compute_rhob(self, f, g, mp):
+        self.lib.orc_compute_rhob(_ptr(f), ctypes.byref(g), ctypes.byref(mp))
+
+    def clean_div_e(self, f, g, mp):
+        self.lib.orc_clean_div_e(_ptr(f), ctypes.byref(g), ctypes.byref(mp))
+
+    def compute_curl_b(self, f, g, mp):
+        self.lib.orc_compute_curl_b(_ptr(f), ctypes.byref(g),
+                                    ctypes.byref(mp))
+
+    def compute_div_b_err(self, f, g):
+        self.lib.orc_compute_div_b_err(_ptr(f), ctypes.byref(g))
+
+    def clean_div_b(self, f, g):
+        self.lib.orc_clean_div_b(_ptr(f), ctypes.byref(g))
+
+    def compute_rms_div_e_err(self, f, g):
+        return self.lib.orc_compute_rms_div_e_err(_ptr(f), ctypes.byref(g))
+
+    def compute_rms_div_b_err(self, f, g):
+        return self.lib.orc_compute_rms_div_b_err(_ptr(f), ctypes.byref(g))
+
+    def synchronize_tang_e_norm_b(self, f, g):
+        return self.lib.orc_synchronize_tang_e_norm_b(_ptr(f),
+                                                      ctypes.byref(g))
+
 
 class Ref:
     """The reference's own scalar kernels (oracle/_ref)."""
@@ -192,6 +235,17 @@ class Ref:
         lib.ref_center_p.argtypes = [_vp, ctypes.c_int, _f32, _f32, _vp, _pg,
                                      ctypes.c_int]
         lib.ref_center_p.restype = None
+        for name in ("ref_div_e_err_bulk", "ref_rhob_bulk",
+                     "ref_clean_div_e_bulk", "ref_curl_b_bulk"):
+            getattr(lib, name).argtypes = [_vp, _pg,
+                                           ctypes.POINTER(T.FieldParams)]
+            getattr(lib, name).restype = None
+        for name in ("ref_div_b_err", "ref_clean_div_b_bulk"):
+            getattr(lib, name).argtypes = [_vp, _pg]
+            getattr(lib, name).restype = None
+        for name in ("ref_rms_div_b_err", "ref_rms_div_e_err_bulk"):
+            getattr(lib, name).argtypes = [_vp, _pg]
+            getattr(lib, name).restype = ctypes.c_double
         lib.ref_sort_p.argtypes = [_vp, ctypes.c_int, _vp, _pg]
         lib.ref_load_interpolator.argtypes = [_vp, _vp, _pg]
         lib.ref_unload_accumulator.argtypes = [_vp, _vp, _pg]
@@ -264,3 +318,29 @@ class Ref:
         self.lib.ref_energy_f(_ptr(en), _ptr(f), ctypes.byref(g),
                               ctypes.byref(mp))
         return en
+
+    # bulk (pipelined interior) parts of the divergence-cleaning kernels
+    def div_e_err_bulk(self, f, g, mp):
+        self.lib.ref_div_e_err_bulk(_ptr(f), ctypes.byref(g), ctypes.byref(mp))
+
+    def rhob_bulk(self, f, g, mp):
+        self.lib.ref_rhob_bulk(_ptr(f), ctypes.byref(g), ctypes.byref(mp))
+
+    def clean_div_e_bulk(self, f, g, mp):
+        self.lib.ref_clean_div_e_bulk(_ptr(f), ctypes.byref(g),
+                                      ctypes.byref(mp))
+
+    def curl_b_bulk(self, f, g, mp):
+        self.lib.ref_curl_b_bulk(_ptr(f), ctypes.byref(g), ctypes.byref(mp))
+
+    def div_b_err(self, f, g):
+        self.lib.ref_div_b_err(_ptr(f), ctypes.byref(g))
+
+    def clean_div_b_bulk(self, f, g):
+        self.lib.ref_clean_div_b_bulk(_ptr(f), ctypes.byref(g))
+
+    def rms_div_b_err(self, f, g):
+        return self.lib.ref_rms_div_b_err(_ptr(f), ctypes.byref(g))
+
+    def rms_div_e_err_bulk(self, f, g):
+        return self.lib.ref_rms_div_e_err_bulk(_ptr(f), ctypes.byref(g))

@@ -467,6 +467,32 @@ static void check_field_kernels(int x_faces, const char *name) {
     worst = fmax(worst, fabs(en[c] - ren[c]) / fmax(1e-300, fabs(ren[c])));
   report(okf && worst <= 1e-12, name, "fields %s, energy_f rel err %.3g",
          okf ? "same" : "DIFFER", worst);
+  // the divergence-cleaning / initialization entries of the table
+  k->compute_curl_b(&A.fa);
+  orc_compute_curl_b(o, &b.G, &mp);
+  k->compute_rhob(&A.fa);
+  orc_compute_rhob(o, &b.G, &mp);
+  k->compute_div_e_err(&A.fa);
+  orc_compute_div_e_err(o, &b.G, &mp);
+  const double re = k->compute_rms_div_e_err(&A.fa);
+  const double oe = orc_compute_rms_div_e_err(o, &b.G);
+  k->clean_div_e(&A.fa);
+  orc_clean_div_e(o, &b.G, &mp);
+  k->compute_div_b_err(&A.fa);
+  orc_compute_div_b_err(o, &b.G);
+  const double rb = k->compute_rms_div_b_err(&A.fa);
+  const double ob = orc_compute_rms_div_b_err(o, &b.G);
+  k->clean_div_b(&A.fa);
+  orc_clean_div_b(o, &b.G);
+  const double rs = k->synchronize_tang_e_norm_b(&A.fa);
+  const double os = orc_synchronize_tang_e_norm_b(o, &b.G);
+  const bool okc = same(A.f.data(), f0.data(), sizeof(field_t) * b.G.nv);
+  const double wc = fmax(fmax(fabs(re - oe) / fmax(1e-300, fabs(oe)),
+                              fabs(rb - ob) / fmax(1e-300, fabs(ob))),
+                         fabs(rs - os) / fmax(1e-300, fabs(os)));
+  report(okc && wc <= 1e-12, name,
+         "cleaning kernels: fields %s, rms/err rel err %.3g",
+         okc ? "same" : "DIFFER", wc);
   vpic_hip_dropin_release_all();
 }
 

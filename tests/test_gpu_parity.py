@@ -257,6 +257,47 @@ def test_field_kernels_bitexact(dev, orc):
         assert np.allclose(np.array(en[:]), eo, rtol=1e-12, atol=0)
 
 
+CLEANING = ["compute_div_e_err", "compute_rhob", "clean_div_e",
+            "compute_curl_b", "compute_div_b_err", "clean_div_b",
+            "synchronize_tang_e_norm_b", "rms"]
+
+
+@pytest.mark.parametrize("op", CLEANING)
+def test_divergence_cleaning_bitexact(dev, orc, op):
+    """The divergence-cleaning / initialization kernels of the vacuum field
+    table against the oracle on every boundary mix: fields bit-exact, the
+    FP64 sums (rms, err) to 1e-12 relative."""
+    mp = T.vacuum_params(damp=0.0)
+    for name, g in _field_bc_grids():
+        f = H.random_full_fields(g, 11)
+        fo = f.copy()
+        df = dev.DeviceArray.from_host(f)
+        G = ctypes.byref(g)
+        say(op + ",", name)
+        if op == "rms":
+            for kind in ("e", "b"):
+                r = ctypes.c_double()
+                _lib.call("vpic_hip_compute_rms_div_%s_err" % kind,
+                          ctypes.byref(r), df.ptr, G, None)
+                want = getattr(orc, "compute_rms_div_%s_err" % kind)(fo, g)
+                assert abs(r.value - want) <= 1e-12 * abs(want), (name, kind)
+            continue
+        if op == "synchronize_tang_e_norm_b":
+            err = ctypes.c_double()
+            _lib.call("vpic_hip_" + op, ctypes.byref(err), df.ptr, G, None)
+            want = orc.synchronize_tang_e_norm_b(fo, g)
+            assert abs(err.value - want) <= 1e-12 * max(abs(want), 1e-300)
+        elif op in ("compute_div_b_err", "clean_div_b"):
+            _lib.call("vpic_hip_" + op, df.ptr, G, None)
+            getattr(orc, op)(fo, g)
+        else:
+            _lib.call("vpic_hip_" + op, df.ptr, G, ctypes.byref(mp), None)
+            getattr(orc, op)(fo, g, mp)
+        got = df.download()
+        for c in T.FIELD_FLOATS:
+            assert H.same_bits(got[c], fo[c]), (name, op, c)
+
+
 def test_synchronize_rho_bitexact(dev, orc):
     for name, g in _field_bc_grids():
         f = H.random_full_fields(g, 5)

@@ -293,3 +293,155 @@ def test_energy_f(orc, ref):
     f = H.random_full_fields(g, 13)
     mp = T.vacuum_params()
     assert np.array_equal(orc.energy_f(f, g, mp), ref.energy_f(f, g, mp))
+
+
+# -- divergence cleaning / initialization kernels of the vacuum table -------
+
+def _mat(damp=0.0):
+    mp = T.vacuum_params(damp)
+    return mp
+
+
+def test_div_e_err_and_rhob_bulk(orc, ref):
+    """Every interior node (2..n) against the reference's own pipelines."""
+    g = _box(7, 5, 6)
+    mp = _mat()
+    f = H.random_full_fields(g, 21)
+    m = _interior_mask(g, 2)
+    fa, fb = f.copy(), f.copy()
+    orc.compute_div_e_err(fa, g, mp)
+    ref.div_e_err_bulk(fb, g, mp)
+    assert H.same_bits(fa["div_e_err"][m], fb["div_e_err"][m])
+    fa, fb = f.copy(), f.copy()
+    orc.compute_rhob(fa, g, mp)
+    ref.rhob_bulk(fb, g, mp)
+    assert H.same_bits(fa["rhob"][m], fb["rhob"][m])
+
+
+def test_clean_div_e_curl_b_bulk(orc, ref):
+    g = _box(6, 7, 5)
+    mp = _mat()
+    f = H.random_full_fields(g, 22)
+    fa, fb = f.copy(), f.copy()
+    orc.clean_div_e(fa, g, mp)
+    ref.clean_div_e_bulk(fb, g, mp)
+    m = _interior_mask(g, 1)
+    for c in ("ex", "ey", "ez"):
+        assert H.same_bits(fa[c][m], fb[c][m]), c
+    fa, fb = f.copy(), f.copy()
+    orc.compute_curl_b(fa, g, mp)
+    ref.curl_b_bulk(fb, g, mp)
+    m = _interior_mask(g, 2)
+    for c in ("tcax", "tcay", "tcaz"):
+        assert H.same_bits(fa[c][m], fb[c][m]), c
+
+
+def test_div_b_err_and_clean_div_b(orc, ref):
+    g = _box(5, 6, 7)
+    f = H.random_full_fields(g, 23)
+    fa, fb = f.copy(), f.copy()
+    orc.compute_div_b_err(fa, g)
+    ref.div_b_err(fb, g)
+    m = _interior_mask(g, 1)
+    assert H.same_bits(fa["div_b_err"][m], fb["div_b_err"][m])
+    fa, fb = f.copy(), f.copy()
+    orc.clean_div_b(fa, g)
+    ref.clean_div_b_bulk(fb, g)
+    m = _interior_mask(g, 2)
+    for c in ("cbx", "cby", "cbz"):
+        assert H.same_bits(fa[c][m], fb[c][m]), c
+
+
+def test_rms_div_errors(orc, ref):
+    g = _box(6, 5, 7)
+    f = H.random_full_fields(g, 24)
+    ro, rr = orc.compute_rms_div_b_err(f, g), ref.rms_div_b_err(f, g)
+    assert abs(ro - rr) <= 1e-12 * abs(rr)
+    # with div_e_err zero on the boundary nodes only the pipelined interior
+    # sum remains
+    a = f.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+    for sl in ((slice(None), slice(None), 1), (slice(None), slice(None), g.nx + 1),
+               (slice(None), 1, slice(None)), (slice(None), g.ny + 1, slice(None)),
+               (1, slice(None), slice(None)), (g.nz + 1, slice(None), slice(None))):
+        a["div_e_err"][sl] = 0
+    bulk = ref.rms_div_e_err_bulk(f, g)
+    n = g.nx * g.ny * g.nz
+    want = float(g.eps0) * np.sqrt(bulk * float(g.dV) /
+                                   float(np.float32(n) * g.dV))
+    assert abs(orc.compute_rms_div_e_err(f, g) - want) <= 1e-12 * want
+
+
+def test_rms_closed_forms(orc):
+    """A uniform error c gives eps0*|c|: the node weights (1, 1/2, 1/4, 1/8)
+    sum to nx*ny*nz, as the cell count does for div_b_err."""
+    g = _box(5, 4, 6)
+    g.eps0 = np.float32(2.0)
+    f = np.zeros(g.nv, dtype=T.FIELD)
+    f["div_e_err"] = np.float32(0.75)
+    f["div_b_err"] = np.float32(-1.25)
+    assert abs(orc.compute_rms_div_e_err(f, g) - 1.5) <= 1e-12
+    assert abs(orc.compute_rms_div_b_err(f, g) - 2.5) <= 1e-12
+
+
+def test_cleaning_translation_invariance(orc):
+    """On a periodic box the face/leftover parts of each kernel equal the
+    bulk part on the translated field (the bulk is pinned to the reference
+    above), so every node, edge and face is pinned."""
+    g = _box(6, 5, 4)
+    mp = _mat()
+    f = _periodize(g, H.random_full_fields(g, 25))
+    ops = {
+        "div_e_err": (lambda x: orc.compute_div_e_err(x, g, mp), ("div_e_err",)),
+        "rhob": (lambda x: orc.compute_rhob(x, g, mp), ("rhob",)),
+        "clean_div_e": (lambda x: orc.clean_div_e(x, g, mp), ("ex", "ey", "ez")),
+        "curl_b": (lambda x: orc.compute_curl_b(x, g, mp),
+                   ("tcax", "tcay", "tcaz")),
+        "div_b_err": (lambda x: orc.compute_div_b_err(x, g), ("div_b_err",)),
+        "clean_div_b": (lambda x: orc.clean_div_b(x, g), ("cbx", "cby", "cbz")),
+    }
+    m = _interior_mask(g, 1)
+    for name, (op, comps) in ops.items():
+        fa = f.copy()
+        fr = _periodize(g, _roll_field(g, f, 2, 3, 1))
+        op(fa)
+        op(fr)
+        ra = _roll_field(g, fa, 2, 3, 1)
+        for c in comps:
+            assert H.same_bits(ra[c][m], fr[c][m]), (name, c)
+
+
+def test_discrete_divergence(orc):
+    """div_e_err is the backward difference of E minus rho/eps0 on the
+    periodic node lattice (numpy roll as the independent statement)."""
+    g = _box(6, 5, 4)
+    g.eps0 = np.float32(1.0)
+    mp = _mat()
+    f = _periodize(g, H.random_full_fields(g, 26))
+    fa = f.copy()
+    orc.compute_div_e_err(fa, g, mp)
+    a = f.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+    inn = (slice(1, g.nz + 1), slice(1, g.ny + 1), slice(1, g.nx + 1))
+    ex, ey, ez = (a[c][inn].astype(np.float64) for c in ("ex", "ey", "ez"))
+    rho = (a["rhof"][inn].astype(np.float64) + a["rhob"][inn])
+    d = ((ex - np.roll(ex, 1, axis=2)) * g.rdx + (ey - np.roll(ey, 1, axis=1)) *
+         g.rdy + (ez - np.roll(ez, 1, axis=0)) * g.rdz - rho)
+    got = fa.reshape(a.shape)["div_e_err"][inn]
+    assert np.max(np.abs(got - d)) <= 1e-5 * np.max(np.abs(d))
+
+
+def test_synchronize_tang_e_norm_b_closed_forms(orc):
+    g = _box(5, 4, 3)
+    f = _periodize(g, H.random_full_fields(g, 27))
+    fa = f.copy()
+    assert orc.synchronize_tang_e_norm_b(fa, g) == 0.0
+    assert H.same_bits(fa, f)
+    # one face value off by d: err = 2 d^2 (seen from both faces), both
+    # faces take the FP64 average
+    a = fa.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+    d = np.float32(0.5)
+    lo = a["cbx"][2, 2, 1]
+    a["cbx"][2, 2, g.nx + 1] = lo + d
+    err = orc.synchronize_tang_e_norm_b(fa, g)
+    assert abs(err - 2 * float(d) ** 2) <= 1e-12
+    want = np.float32(0.5 * (float(lo) + float(lo + d)))
+    assert a["cbx"][2, 2, 1] == want and a["cbx"][2, 2, g.nx + 1] == want

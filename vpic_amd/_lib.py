@@ -49,6 +49,15 @@ _SIGS = {
     "vpic_hip_advance_b": [_vp, _pg, _f32, _vp],
     "vpic_hip_vacuum_advance_e": [_vp, _pg, _pmp, _f32, _vp],
     "vpic_hip_energy_f": [_pf64, _vp, _pg, _pmp, _vp],
+    "vpic_hip_compute_div_e_err": [_vp, _pg, _pmp, _vp],
+    "vpic_hip_compute_rms_div_e_err": [_pf64, _vp, _pg, _vp],
+    "vpic_hip_clean_div_e": [_vp, _pg, _pmp, _vp],
+    "vpic_hip_compute_div_b_err": [_vp, _pg, _vp],
+    "vpic_hip_compute_rms_div_b_err": [_pf64, _vp, _pg, _vp],
+    "vpic_hip_clean_div_b": [_vp, _pg, _vp],
+    "vpic_hip_compute_rhob": [_vp, _pg, _pmp, _vp],
+    "vpic_hip_compute_curl_b": [_vp, _pg, _pmp, _vp],
+    "vpic_hip_synchronize_tang_e_norm_b": [_pf64, _vp, _pg, _vp],
     "vpic_hip_grid_periodic": [_pg, _f64, _f64, _f64, _f64, _f64, _f64, _i32,
                                _i32, _i32, _f32, _f32, _f32],
     "vpic_hip_field_params_vacuum": [_pmp, _f32],

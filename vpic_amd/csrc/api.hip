@@ -422,6 +422,89 @@ extern "C" int vpic_hip_energy_f(double host_en[6], const vpic_field_t *f,
   return launch_energy_f(host_en, f, g, mp, S(stream));
 }
 
+// Divergence cleaning and initialization kernels of the vacuum field table.
+extern "C" int vpic_hip_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
+                                          const vpic_field_params_t *mp,
+                                          void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(f && mp && aligned16(f), "bad argument");
+  return launch_compute_div_e_err(f, g, mp, 0, S(stream));
+}
+
+extern "C" int vpic_hip_compute_rhob(vpic_field_t *f, const vpic_grid_t *g,
+                                     const vpic_field_params_t *mp,
+                                     void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(f && mp && aligned16(f), "bad argument");
+  return launch_compute_div_e_err(f, g, mp, 1, S(stream));
+}
+
+extern "C" int vpic_hip_compute_rms_div_e_err(double *host_rms,
+                                              const vpic_field_t *f,
+                                              const vpic_grid_t *g,
+                                              void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(host_rms && f, "NULL argument");
+  return launch_compute_rms_div_err(host_rms, f, g, 0, S(stream));
+}
+
+extern "C" int vpic_hip_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
+                                    const vpic_field_params_t *mp,
+                                    void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(f && mp && aligned16(f), "bad argument");
+  return launch_clean_div_e(f, g, mp, S(stream));
+}
+
+extern "C" int vpic_hip_compute_div_b_err(vpic_field_t *f, const vpic_grid_t *g,
+                                          void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(f && aligned16(f), "bad argument");
+  return launch_compute_div_b_err(f, g, S(stream));
+}
+
+extern "C" int vpic_hip_compute_rms_div_b_err(double *host_rms,
+                                              const vpic_field_t *f,
+                                              const vpic_grid_t *g,
+                                              void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(host_rms && f, "NULL argument");
+  return launch_compute_rms_div_err(host_rms, f, g, 1, S(stream));
+}
+
+extern "C" int vpic_hip_clean_div_b(vpic_field_t *f, const vpic_grid_t *g,
+                                    void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(f && aligned16(f), "bad argument");
+  return launch_clean_div_b(f, g, S(stream));
+}
+
+extern "C" int vpic_hip_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
+                                       const vpic_field_params_t *mp,
+                                       void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(f && mp && aligned16(f), "bad argument");
+  return launch_compute_curl_b(f, g, mp, S(stream));
+}
+
+extern "C" int vpic_hip_synchronize_tang_e_norm_b(double *host_err,
+                                                  vpic_field_t *f,
+                                                  const vpic_grid_t *g,
+                                                  void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(host_err && f && aligned16(f), "bad argument");
+  return launch_synchronize_tang_e_norm_b(host_err, f, g, S(stream));
+}
+
 // ---------------------------------------------------------------------------
 // set-up helpers
 // ---------------------------------------------------------------------------
@@ -477,6 +560,7 @@ extern "C" int vpic_hip_field_params_vacuum(vpic_field_params_t *mp,
   mp->rmux = mp->rmuy = mp->rmuz = (float)(1. / 1.0);
   mp->epsx = mp->epsy = mp->epsz = 1.0f;
   mp->damp = damp;
+  mp->nonconductive = 1.0f;  // sfa.cc:144 with ax = ay = az = 0
   return VPIC_OK;
 }
 

@@ -190,6 +190,20 @@ int launch_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
                             const vpic_field_params_t *mp, hipStream_t s);
 int launch_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
                     const vpic_field_params_t *mp, hipStream_t s);
+int launch_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
+                             const vpic_field_params_t *m, int rhob,
+                             hipStream_t s);
+int launch_compute_rms_div_err(double *out, const vpic_field_t *f,
+                               const vpic_grid_t *g, int divb, hipStream_t s);
+int launch_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
+                       const vpic_field_params_t *m, hipStream_t s);
+int launch_compute_div_b_err(vpic_field_t *f, const vpic_grid_t *g,
+                             hipStream_t s);
+int launch_clean_div_b(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
+int launch_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
+                          const vpic_field_params_t *m, hipStream_t s);
+int launch_synchronize_tang_e_norm_b(double *out, vpic_field_t *f,
+                                     const vpic_grid_t *g, hipStream_t s);
 int launch_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
                                int ppc, float uthx, float uthy, float uthz,
                                float udx, float udy, float udz, float w,

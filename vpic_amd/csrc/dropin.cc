@@ -261,6 +261,7 @@ bool field_params(const char *fn, const field_array_t *fa,
   mp->epsy = c.epsy;
   mp->epsz = c.epsz;
   mp->damp = sp->damp;
+  mp->nonconductive = c.nonconductive;
   return true;
 }
 
@@ -674,6 +675,61 @@ FIELD_SIMPLE(clear_jf, vpic_hip_clear_jf)
 FIELD_SIMPLE(clear_rhof, vpic_hip_clear_rhof)
 FIELD_SIMPLE(synchronize_jf, vpic_hip_synchronize_jf)
 FIELD_SIMPLE(synchronize_rho, vpic_hip_synchronize_rho)
+FIELD_SIMPLE(compute_div_b_err, vpic_hip_compute_div_b_err)
+FIELD_SIMPLE(clean_div_b, vpic_hip_clean_div_b)
+
+#define FIELD_WITH_PARAMS(NAME, ABI_FN)                                 \
+  extern "C" void vpic_hip_fa_##NAME(field_array_t *fa) {               \
+    const char *fn = #NAME;                                             \
+    if (!fa) {                                                          \
+      fail(fn, "Bad args");                                             \
+      return;                                                           \
+    }                                                                   \
+    vpic_field_params_t mp;                                             \
+    vpic_field_t *f;                                                    \
+    vpic_grid_t G;                                                      \
+    if (!field_params(fn, fa, &mp) || !field_call(fn, fa, &f, &G))      \
+      return;                                                           \
+    if (ABI_FN(f, &G, &mp, nullptr) != 0) {                             \
+      fail(fn, "%s", vpic_hip_last_error());                            \
+      return;                                                           \
+    }                                                                   \
+    dev_wrote(fn, fa->f, fbytes(fa->g));                                \
+  }
+
+FIELD_WITH_PARAMS(vacuum_compute_div_e_err, vpic_hip_compute_div_e_err)
+FIELD_WITH_PARAMS(vacuum_clean_div_e, vpic_hip_clean_div_e)
+FIELD_WITH_PARAMS(vacuum_compute_rhob, vpic_hip_compute_rhob)
+FIELD_WITH_PARAMS(vacuum_compute_curl_b, vpic_hip_compute_curl_b)
+
+#define FIELD_RMS(NAME, ABI_FN)                                         \
+  extern "C" double vpic_hip_fa_##NAME(const field_array_t *fa) {       \
+    const char *fn = #NAME;                                             \
+    vpic_field_t *f;                                                    \
+    vpic_grid_t G;                                                      \
+    double r = 0;                                                       \
+    if (!field_call(fn, (field_array_t *)fa, &f, &G)) return 0;         \
+    if (ABI_FN(&r, f, &G, nullptr) != 0)                                \
+      fail(fn, "%s", vpic_hip_last_error());                            \
+    return r;                                                           \
+  }
+
+FIELD_RMS(compute_rms_div_e_err, vpic_hip_compute_rms_div_e_err)
+FIELD_RMS(compute_rms_div_b_err, vpic_hip_compute_rms_div_b_err)
+
+extern "C" double vpic_hip_fa_synchronize_tang_e_norm_b(field_array_t *fa) {
+  const char *fn = "synchronize_tang_e_norm_b";
+  vpic_field_t *f;
+  vpic_grid_t G;
+  double err = 0;
+  if (!field_call(fn, fa, &f, &G)) return 0;
+  if (vpic_hip_synchronize_tang_e_norm_b(&err, f, &G, nullptr) != 0) {
+    fail(fn, "%s", vpic_hip_last_error());
+    return 0;
+  }
+  dev_wrote(fn, fa->f, fbytes(fa->g));
+  return err;
+}
 
 static void fa_advance_e_entry(field_array_t *fa, float frac) {
   vpic_hip_fa_advance_e(fa, frac);
@@ -699,6 +755,15 @@ extern "C" int vpic_hip_install_field_kernels(field_array_t *fa) {
   k->synchronize_jf = vpic_hip_fa_synchronize_jf;
   k->clear_rhof = vpic_hip_fa_clear_rhof;
   k->synchronize_rho = vpic_hip_fa_synchronize_rho;
+  k->compute_rhob = vpic_hip_fa_vacuum_compute_rhob;
+  k->compute_curl_b = vpic_hip_fa_vacuum_compute_curl_b;
+  k->synchronize_tang_e_norm_b = vpic_hip_fa_synchronize_tang_e_norm_b;
+  k->compute_div_e_err = vpic_hip_fa_vacuum_compute_div_e_err;
+  k->compute_rms_div_e_err = vpic_hip_fa_compute_rms_div_e_err;
+  k->clean_div_e = vpic_hip_fa_vacuum_clean_div_e;
+  k->compute_div_b_err = vpic_hip_fa_compute_div_b_err;
+  k->compute_rms_div_b_err = vpic_hip_fa_compute_rms_div_b_err;
+  k->clean_div_b = vpic_hip_fa_clean_div_b;
   return 0;
 }
 

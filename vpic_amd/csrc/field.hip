@@ -456,6 +456,273 @@ __global__ __launch_bounds__(256) void k_fold_partials(const double *part,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Divergence cleaning and initialization kernels of the vacuum field table
+// (sfa.cc:203-210).  Same expressions and association as the reference, so
+// the field results are bit-identical; the FP64 sums differ only in order.
+// ---------------------------------------------------------------------------
+
+// Normal-E ghosts of axis X (both sides) on the node planes 0 and nX+1
+// (Y in 1..nY+1, Z in 1..nZ+1): periodic faces take the opposite interior
+// plane with rw = 1, lw = 0 (remote.cc:137-207); local faces copy
+// (anti-symmetric) or negate (symmetric, pmc) e and tca of the adjacent
+// plane (local.cc:129-179).
+__global__ void k_ghost_norm_e(vpic_field_t *f, FaceGeom G, int X, int mlo,
+                               int mhi) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int nX = G.n[X];
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const float rw = 1.0f, lw = 0.0f;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    const int g0 = face_vox(G, X, 0, yy, zz);
+    const int g1 = face_vox(G, X, nX + 1, yy, zz);
+    const int i1 = face_vox(G, X, 1, yy, zz);
+    const int iN = face_vox(G, X, nX, yy, zz);
+    const float e1 = fcomp(f, i1, 0, X), eN = fcomp(f, iN, 0, X);
+    const float t1 = fcomp(f, i1, 2, X), tN = fcomp(f, iN, 2, X);
+    if (mlo == VPIC_BC_PERIODIC) {
+      fcomp(f, g0, 0, X) = rw * eN + lw * e1;
+    } else if (mlo == VPIC_BC_ANTI_SYMMETRIC_FIELDS) {
+      fcomp(f, g0, 0, X) = e1;
+      fcomp(f, g0, 2, X) = t1;
+    } else {
+      fcomp(f, g0, 0, X) = -e1;
+      fcomp(f, g0, 2, X) = -t1;
+    }
+    if (mhi == VPIC_BC_PERIODIC) {
+      fcomp(f, g1, 0, X) = rw * e1 + lw * eN;
+    } else if (mhi == VPIC_BC_ANTI_SYMMETRIC_FIELDS) {
+      fcomp(f, g1, 0, X) = eN;
+      fcomp(f, g1, 2, X) = tN;
+    } else {
+      fcomp(f, g1, 0, X) = -eN;
+      fcomp(f, g1, 2, X) = -tN;
+    }
+  }
+}
+
+// div_b_err ghosts of axis X (cell planes 0 and nX+1, Y in 1..nY, Z in
+// 1..nZ): periodic as above (remote.cc:209-281); local faces copy (anti-
+// symmetric) or negate (symmetric, pmc) (local.cc:182-221).
+__global__ void k_ghost_div_b(vpic_field_t *f, FaceGeom G, int X, int mlo,
+                              int mhi) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int nX = G.n[X];
+  const int wy = G.n[Y], wz = G.n[Z];
+  const float rw = 1.0f, lw = 0.0f;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    vpic_field_t &g0 = f[face_vox(G, X, 0, yy, zz)];
+    vpic_field_t &g1 = f[face_vox(G, X, nX + 1, yy, zz)];
+    const float d1 = f[face_vox(G, X, 1, yy, zz)].div_b_err;
+    const float dN = f[face_vox(G, X, nX, yy, zz)].div_b_err;
+    g0.div_b_err = mlo == VPIC_BC_PERIODIC ? rw * dN + lw * d1
+                   : mlo == VPIC_BC_ANTI_SYMMETRIC_FIELDS ? d1
+                                                          : -d1;
+    g1.div_b_err = mhi == VPIC_BC_PERIODIC ? rw * d1 + lw * dN
+                   : mhi == VPIC_BC_ANTI_SYMMETRIC_FIELDS ? dN
+                                                          : -dN;
+  }
+}
+
+// Zero float slot `slot` of field_t on the node plane of face k
+// (local_adjust_div_e, local.cc:299-334; local_adjust_rhob, 415-440).
+__global__ void k_zero_node_face(vpic_field_t *f, FaceGeom G, int k,
+                                 int slot) {
+  const int X = k % 3, Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int plane = k >= 3 ? G.n[X] + 1 : 1;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    reinterpret_cast<float *>(f + face_vox(G, X, plane, yy, zz))[slot] = 0;
+  }
+}
+
+// Nodes 1..n+1: div_e_err (RHOB false: vacuum_compute_div_e_err_pipeline.h:
+// 17-50) or rhob (RHOB true: vacuum_compute_rhob_pipeline.h:17-52).
+template <bool RHOB>
+__global__ __launch_bounds__(256) void k_div_e_nodes(vpic_field_t *__restrict__ f,
+                                                     Box b, int nx, int ny,
+                                                     float nc, float px,
+                                                     float py, float pz,
+                                                     float cj) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  const float4 e0 = F4(f, v)[0];
+  const float4 t0 = F4(f, v)[2];
+  const float4 j0 = F4(f, v)[3];
+  const float exm = F4(f, v - 1)[0].x, eym = F4(f, v - sy)[0].y,
+              ezm = F4(f, v - sz)[0].z;
+  if (RHOB) {
+    reinterpret_cast<float *>(f + v)[11] =
+        nc * (px * (e0.x - exm) + py * (e0.y - eym) + pz * (e0.z - ezm) -
+              j0.w);
+  } else {
+    reinterpret_cast<float *>(f + v)[3] =
+        nc * (px * (e0.x - exm) + py * (e0.y - eym) + pz * (e0.z - ezm) -
+              cj * (j0.w + t0.w));
+  }
+}
+
+// Edges: E += p (div_e_err(+1) - div_e_err) (vacuum_clean_div_e_pipeline.h:
+// 45-47 over every edge of vacuum_clean_div_e_pipeline.cc).
+__global__ __launch_bounds__(256) void k_clean_div_e(vpic_field_t *__restrict__ f,
+                                                     Box b, int nx, int ny,
+                                                     int nz, float px, float py,
+                                                     float pz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  float4 e0 = F4(f, v)[0];
+  const float d0 = e0.w;
+  if (x <= nx) e0.x += px * (F4(f, v + 1)[0].w - d0);
+  if (y <= ny) e0.y += py * (F4(f, v + sy)[0].w - d0);
+  if (z <= nz) e0.z += pz * (F4(f, v + sz)[0].w - d0);
+  F4(f, v)[0] = e0;
+}
+
+// Cells 1..n: div_b_err (compute_div_b_err_pipeline.cc:44-46).
+__global__ __launch_bounds__(256) void k_div_b_cells(vpic_field_t *__restrict__ f,
+                                                     Box b, int nx, int ny,
+                                                     float px, float py,
+                                                     float pz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  const float4 b0 = F4(f, v)[1];
+  reinterpret_cast<float *>(f + v)[7] =
+      px * (F4(f, v + 1)[1].x - b0.x) + py * (F4(f, v + sy)[1].y - b0.y) +
+      pz * (F4(f, v + sz)[1].z - b0.z);
+}
+
+// Faces: cb += p (div_b_err - div_b_err(-1)) (clean_div_b_pipeline.h:
+// 19-21 over every face of clean_div_b_pipeline.cc).
+__global__ __launch_bounds__(256) void k_clean_div_b(vpic_field_t *__restrict__ f,
+                                                     Box b, int nx, int ny,
+                                                     int nz, float px, float py,
+                                                     float pz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  float4 b0 = F4(f, v)[1];
+  const float d0 = b0.w;
+  if (y <= ny && z <= nz) b0.x += px * (d0 - F4(f, v - 1)[1].w);
+  if (x <= nx && z <= nz) b0.y += py * (d0 - F4(f, v - sy)[1].w);
+  if (x <= nx && y <= ny) b0.z += pz * (d0 - F4(f, v - sz)[1].w);
+  F4(f, v)[1] = b0;
+}
+
+// Edges: tca = curl B (vacuum_compute_curl_b_pipeline.h:52-57).
+__global__ __launch_bounds__(256) void k_curl_b(vpic_field_t *__restrict__ f,
+                                                Box b, int nx, int ny, int nz,
+                                                ECoef c) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  const int v = vox(x, y, z, nx, ny);
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  const float4 b0 = F4(f, v)[1];
+  float4 t0 = F4(f, v)[2];
+  const float4 bx = F4(f, v - 1)[1], by = F4(f, v - sy)[1],
+               bz = F4(f, v - sz)[1];
+  if (x <= nx) t0.x = (c.py_muz * (b0.z - by.z) - c.pz_muy * (b0.y - bz.y));
+  if (y <= ny) t0.y = (c.pz_mux * (b0.x - bz.x) - c.px_muz * (b0.z - bx.z));
+  if (z <= nz) t0.z = (c.px_muy * (b0.y - bx.y) - c.py_mux * (b0.x - by.x));
+  F4(f, v)[2] = t0;
+}
+
+// Per-block FP64 partial sums of the squared divergence errors: div_e_err
+// on nodes 1..n+1 (interior products in FP32, boundary nodes in FP64 with
+// weights 1/2, 1/4, 1/8; compute_rms_div_e_err_pipeline.cc:29-172) or
+// div_b_err on cells 1..n (compute_rms_div_b_err_pipeline.cc:29-45).
+template <bool DIVB>
+__global__ __launch_bounds__(256) void k_sq_partials(const vpic_field_t *__restrict__ f,
+                                                     Box b, int nx, int ny,
+                                                     int nz,
+                                                     double *__restrict__ part) {
+  __shared__ double red[256];
+  int x, y, z;
+  double e = 0;
+  if (box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z)) {
+    const float d = reinterpret_cast<const float *>(
+        f + vox(x, y, z, nx, ny))[DIVB ? 7 : 3];
+    if (DIVB) {
+      e = d * d;
+    } else {
+      const int nb = (x == 1 || x == nx + 1) + (y == 1 || y == ny + 1) +
+                     (z == 1 || z == nz + 1);
+      if (nb == 0) {
+        e = d * d;
+      } else {
+        const double w = nb == 1 ? 0.5 : nb == 2 ? 0.25 : 0.125;
+        e = w * (double)d * (double)d;
+      }
+    }
+  }
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// synchronize_tang_e_norm_b on the periodic face pair of axis X
+// (remote.cc:298-416): both faces take 0.5*(w1+w2) in FP64 of cb_X, e and
+// tca tangential; the per-block sums of 2(w1-w2)^2 over cb and e (once
+// from each face) go to part.
+__global__ __launch_bounds__(256) void k_sync_tang_e_norm_b(vpic_field_t *f,
+                                                            FaceGeom G, int X,
+                                                            double *part) {
+  __shared__ double red[256];
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const int hi = G.n[X] + 1;
+  double err = 0;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < wy * wz) {
+    const int yy = 1 + l % wy, zz = 1 + l / wy;
+    const int va = face_vox(G, X, 1, yy, zz), vb = face_vox(G, X, hi, yy, zz);
+    // (block, component, valid on this edge/face, counts in err)
+    const bool onY = yy <= G.n[Y], onZ = zz <= G.n[Z];
+    const int blk[5] = {1, 0, 2, 0, 2};
+    const int cmp[5] = {X, Y, Y, Z, Z};
+    const bool ok[5] = {onY && onZ, onY, onY, onZ, onZ};
+    const bool cnt[5] = {true, true, false, true, false};
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      if (!ok[q]) continue;
+      float &a = fcomp(f, va, blk[q], cmp[q]);
+      float &bb = fcomp(f, vb, blk[q], cmp[q]);
+      const double w1 = a, w2 = bb;
+      a = 0.5 * (w1 + w2);
+      bb = 0.5 * (w2 + w1);
+      if (cnt[q]) err += 2 * ((w1 - w2) * (w1 - w2));
+    }
+  }
+  red[threadIdx.x] = err;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 FaceGeom face_geom(const vpic_grid_t *g) {
   FaceGeom G;
   G.n[0] = g->nx;
@@ -673,4 +940,236 @@ int launch_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
   const double v0 = 0.5 * g->eps0 * g->dV;
   for (int k = 0; k < 6; k++) en[k] = out[k] * v0;
   return VPIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Divergence cleaning and initialization (launchers)
+// ---------------------------------------------------------------------------
+namespace {
+
+int launch_adjust_tang_e_if(vpic_field_t *f, const vpic_grid_t *g,
+                            hipStream_t s) {
+  bool pec = false;
+  for (int k = 0; k < 6; k++) pec |= g->fbc[k] == VPIC_BC_ANTI_SYMMETRIC_FIELDS;
+  if (!pec) return VPIC_OK;
+  int4 m0;
+  int2 m1;
+  face_modes(g, m0, m1);
+  hipLaunchKernelGGL(k_adjust_tang_e, dim3(face_blocks(g), 6), dim3(256), 0,
+                     s, f, face_geom(g), m0, m1);
+  VPIC_LAUNCHED("k_adjust_tang_e");
+  return VPIC_OK;
+}
+
+int launch_adjust_norm_b_if(vpic_field_t *f, const vpic_grid_t *g,
+                            hipStream_t s) {
+  bool sym = false;
+  for (int k = 0; k < 6; k++) sym |= g->fbc[k] == VPIC_BC_SYMMETRIC_FIELDS;
+  if (!sym) return VPIC_OK;
+  int4 m0;
+  int2 m1;
+  face_modes(g, m0, m1);
+  hipLaunchKernelGGL(k_adjust_norm_b, dim3(face_blocks(g), 6), dim3(256), 0,
+                     s, f, face_geom(g), m0, m1);
+  VPIC_LAUNCHED("k_adjust_norm_b");
+  return VPIC_OK;
+}
+
+Box node_box(const vpic_grid_t *g) {
+  Box b = {1, 1, 1, g->nx + 1, g->ny + 1, g->nz + 1};
+  return b;
+}
+
+int64_t node_count(const vpic_grid_t *g) {
+  return (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
+}
+
+// FP64 sum of per-block partials (one fold block, fixed order) to the host.
+int fold_to_host(double *out, const double *part, int nblk, hipStream_t s) {
+  EnergyScratch &S = energy_scratch();
+  int rc;
+  if ((rc = scratch_reserve(S.out, sizeof(double) * 8))) return rc;
+  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(256), 0, s, part, nblk, 1,
+                     (double *)S.out.ptr);
+  VPIC_LAUNCHED("k_fold_partials");
+  VPIC_HIP(hipMemcpyAsync(out, S.out.ptr, sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  VPIC_HIP(hipStreamSynchronize(s));
+  return VPIC_OK;
+}
+
+}  // namespace
+
+int launch_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
+                             const vpic_field_params_t *m, int rhob,
+                             hipStream_t s) {
+  const FaceGeom G = face_geom(g);
+  const int fb = face_blocks(g);
+  for (int X = 0; X < 3; X++) {
+    hipLaunchKernelGGL(k_ghost_norm_e, dim3(fb), dim3(256), 0, s, f, G, X,
+                       g->fbc[X], g->fbc[X + 3]);
+    VPIC_LAUNCHED("k_ghost_norm_e");
+  }
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const Box b = node_box(g);
+  const int nblk = blocks_for(node_count(g), 256);
+  if (rhob) {
+    // vacuum_compute_rhob_pipeline.h:23-26
+    const float px = (nx > 1) ? g->eps0 * m->epsx * g->rdx : 0;
+    const float py = (ny > 1) ? g->eps0 * m->epsy * g->rdy : 0;
+    const float pz = (nz > 1) ? g->eps0 * m->epsz * g->rdz : 0;
+    hipLaunchKernelGGL(k_div_e_nodes<true>, dim3(nblk), dim3(256), 0, s, f, b,
+                       nx, ny, m->nonconductive, px, py, pz, 0.0f);
+  } else {
+    // vacuum_compute_div_e_err_pipeline.h:23-27
+    const float px = ((nx > 1) ? g->rdx : 0) * m->epsx;
+    const float py = ((ny > 1) ? g->rdy : 0) * m->epsy;
+    const float pz = ((nz > 1) ? g->rdz : 0) * m->epsz;
+    const float cj = 1. / g->eps0;
+    hipLaunchKernelGGL(k_div_e_nodes<false>, dim3(nblk), dim3(256), 0, s, f, b,
+                       nx, ny, m->nonconductive, px, py, pz, cj);
+  }
+  VPIC_LAUNCHED("k_div_e_nodes");
+  for (int k = 0; k < 6; k++) {
+    if (g->fbc[k] != VPIC_BC_ANTI_SYMMETRIC_FIELDS) continue;
+    hipLaunchKernelGGL(k_zero_node_face, dim3(fb), dim3(256), 0, s, f, G, k,
+                       rhob ? 11 : 3);
+    VPIC_LAUNCHED("k_zero_node_face");
+  }
+  return VPIC_OK;
+}
+
+int launch_compute_rms_div_err(double *out, const vpic_field_t *f,
+                               const vpic_grid_t *g, int divb, hipStream_t s) {
+  EnergyScratch &S = energy_scratch();
+  const Box b = divb ? Box{1, 1, 1, g->nx, g->ny, g->nz} : node_box(g);
+  const int64_t n = divb ? (int64_t)g->nx * g->ny * g->nz : node_count(g);
+  const int nblk = blocks_for(n, 256);
+  int rc;
+  if ((rc = scratch_reserve(S.part, sizeof(double) * (size_t)nblk))) return rc;
+  if (divb)
+    hipLaunchKernelGGL(k_sq_partials<true>, dim3(nblk), dim3(256), 0, s, f, b,
+                       g->nx, g->ny, g->nz, (double *)S.part.ptr);
+  else
+    hipLaunchKernelGGL(k_sq_partials<false>, dim3(nblk), dim3(256), 0, s, f, b,
+                       g->nx, g->ny, g->nz, (double *)S.part.ptr);
+  VPIC_LAUNCHED("k_sq_partials");
+  double err = 0;
+  if ((rc = fold_to_host(&err, (const double *)S.part.ptr, nblk, s))) return rc;
+  // compute_rms_div_e_err_pipeline.cc:177-182 (the volume in FP32 as there)
+  const double local0 = err * g->dV;
+  const double local1 = (g->nx * g->ny * g->nz) * g->dV;
+  *out = g->eps0 * sqrt(local0 / local1);
+  return VPIC_OK;
+}
+
+int launch_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
+                       const vpic_field_params_t *m, hipStream_t s) {
+  // vacuum_clean_div_e_pipeline.h:20-27
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const float _rdx = (nx > 1) ? g->rdx : 0;
+  const float _rdy = (ny > 1) ? g->rdy : 0;
+  const float _rdz = (nz > 1) ? g->rdz : 0;
+  const float alphadt = 0.3888889 / (_rdx * _rdx + _rdy * _rdy + _rdz * _rdz);
+  const float px = (alphadt * _rdx) * m->drivex;
+  const float py = (alphadt * _rdy) * m->drivey;
+  const float pz = (alphadt * _rdz) * m->drivez;
+  hipLaunchKernelGGL(k_clean_div_e, dim3(blocks_for(node_count(g), 256)),
+                     dim3(256), 0, s, f, node_box(g), nx, ny, nz, px, py, pz);
+  VPIC_LAUNCHED("k_clean_div_e");
+  return launch_adjust_tang_e_if(f, g, s);
+}
+
+int launch_compute_div_b_err(vpic_field_t *f, const vpic_grid_t *g,
+                             hipStream_t s) {
+  // compute_div_b_err_pipeline.cc:27-29
+  const float px = (g->nx > 1) ? g->rdx : 0;
+  const float py = (g->ny > 1) ? g->rdy : 0;
+  const float pz = (g->nz > 1) ? g->rdz : 0;
+  const Box b = {1, 1, 1, g->nx, g->ny, g->nz};
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  hipLaunchKernelGGL(k_div_b_cells, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     f, b, g->nx, g->ny, px, py, pz);
+  VPIC_LAUNCHED("k_div_b_cells");
+  return VPIC_OK;
+}
+
+int launch_clean_div_b(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s) {
+  // clean_div_b_pipeline.cc:245-251
+  float px = (g->nx > 1) ? g->rdx : 0;
+  float py = (g->ny > 1) ? g->rdy : 0;
+  float pz = (g->nz > 1) ? g->rdz : 0;
+  const float alphadt = 0.3888889 / (px * px + py * py + pz * pz);
+  px *= alphadt;
+  py *= alphadt;
+  pz *= alphadt;
+  const FaceGeom G = face_geom(g);
+  const int fb = face_blocks(g);
+  for (int X = 0; X < 3; X++) {
+    hipLaunchKernelGGL(k_ghost_div_b, dim3(fb), dim3(256), 0, s, f, G, X,
+                       g->fbc[X], g->fbc[X + 3]);
+    VPIC_LAUNCHED("k_ghost_div_b");
+  }
+  hipLaunchKernelGGL(k_clean_div_b, dim3(blocks_for(node_count(g), 256)),
+                     dim3(256), 0, s, f, node_box(g), g->nx, g->ny, g->nz, px,
+                     py, pz);
+  VPIC_LAUNCHED("k_clean_div_b");
+  return launch_adjust_norm_b_if(f, g, s);
+}
+
+int launch_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
+                          const vpic_field_params_t *m, hipStream_t s) {
+  const FaceGeom G = face_geom(g);
+  const int fb = face_blocks(g);
+  for (int X = 0; X < 3; X++) {
+    hipLaunchKernelGGL(k_ghost_tang_b, dim3(fb), dim3(256), 0, s, f, G, X,
+                       g->fbc[X], g->fbc[X + 3]);
+    VPIC_LAUNCHED("k_ghost_tang_b");
+  }
+  // vacuum_compute_curl_b_pipeline.h:22-27
+  ECoef c = {};
+  c.px_muz = ((g->nx > 1) ? g->cvac * g->dt * g->rdx : 0) * m->rmuz;
+  c.px_muy = ((g->nx > 1) ? g->cvac * g->dt * g->rdx : 0) * m->rmuy;
+  c.py_mux = ((g->ny > 1) ? g->cvac * g->dt * g->rdy : 0) * m->rmux;
+  c.py_muz = ((g->ny > 1) ? g->cvac * g->dt * g->rdy : 0) * m->rmuz;
+  c.pz_muy = ((g->nz > 1) ? g->cvac * g->dt * g->rdz : 0) * m->rmuy;
+  c.pz_mux = ((g->nz > 1) ? g->cvac * g->dt * g->rdz : 0) * m->rmux;
+  hipLaunchKernelGGL(k_curl_b, dim3(blocks_for(node_count(g), 256)), dim3(256),
+                     0, s, f, node_box(g), g->nx, g->ny, g->nz, c);
+  VPIC_LAUNCHED("k_curl_b");
+  return launch_adjust_tang_e_if(f, g, s);
+}
+
+int launch_synchronize_tang_e_norm_b(double *out, vpic_field_t *f,
+                                     const vpic_grid_t *g, hipStream_t s) {
+  EnergyScratch &S = energy_scratch();
+  int rc;
+  if ((rc = launch_adjust_tang_e_if(f, g, s)) ||
+      (rc = launch_adjust_norm_b_if(f, g, s)))
+    return rc;
+  const FaceGeom G = face_geom(g);
+  int nb[3], total = 0;
+  for (int X = 0; X < 3; X++) {
+    const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const bool per = g->fbc[X] == VPIC_BC_PERIODIC &&
+                     g->fbc[X + 3] == VPIC_BC_PERIODIC;
+    nb[X] = per ? blocks_for((int64_t)(G.n[Y] + 1) * (G.n[Z] + 1), 256) : 0;
+    total += nb[X];
+  }
+  if ((rc = scratch_reserve(S.part, sizeof(double) * (size_t)(total + 1))))
+    return rc;
+  double *part = (double *)S.part.ptr;
+  int off = 0;
+  for (int X = 0; X < 3; X++) {
+    if (!nb[X]) continue;
+    hipLaunchKernelGGL(k_sync_tang_e_norm_b, dim3(nb[X]), dim3(256), 0, s, f,
+                       G, X, part + off);
+    VPIC_LAUNCHED("k_sync_tang_e_norm_b");
+    off += nb[X];
+  }
+  if (total == 0) {
+    *out = 0;
+    return VPIC_OK;
+  }
+  return fold_to_host(out, part, total, s);
 }

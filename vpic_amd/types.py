@@ -74,7 +74,8 @@ class FieldParams(ctypes.Structure):
     """vpic_field_params_t."""
     _fields_ = [(n, ctypes.c_float) for n in
                 ("decayx", "drivex", "decayy", "drivey", "decayz", "drivez",
-                 "rmux", "rmuy", "rmuz", "epsx", "epsy", "epsz", "damp")]
+                 "rmux", "rmuy", "rmuz", "epsx", "epsy", "epsz", "damp",
+                 "nonconductive")]
 
 
 def grid_periodic(gx0, gy0, gz0, gx1, gy1, gz1, gnx, gny, gnz, dt, cvac=1.0,
@@ -129,6 +130,7 @@ def vacuum_params(damp=0.0):
     mp.rmux = mp.rmuy = mp.rmuz = 1.0
     mp.epsx = mp.epsy = mp.epsz = 1.0
     mp.damp = damp
+    mp.nonconductive = 1.0
     return mp
 
 
