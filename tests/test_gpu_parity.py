@@ -403,13 +403,13 @@ def test_loader_statistics(dev):
 # ---------------------------------------------------------------------------
 # The partition-driven (tiled, LDS-accumulating) push
 # ---------------------------------------------------------------------------
-TILES = [(8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
+TILES = [(8, 8, 8), (8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
 
 
 @pytest.fixture
 def tile_reset():
     yield
-    _lib.call("vpic_hip_push_tile", 8, 8, 4)
+    _lib.call("vpic_hip_push_tile", 8, 8, 8)
     _lib.call("vpic_hip_push_group", 16)
 
 

@@ -771,9 +771,11 @@ struct TileGeom {
   unsigned mx, my;    // fast_div multipliers for nx+2, ny+2 (0: divide)
 };
 
-constexpr int TILE_BS = 512;              // threads per workgroup
+constexpr int TILE_BS = 1024;             // threads per workgroup
 constexpr int TILE_WAVES = TILE_BS / 64;
 constexpr int TILE_MAXROWS = 256;         // ty * tz
+constexpr int TILE_MAXCELLS = 512;        // tx * ty * tz (LDS coefficients)
+constexpr int TILE_WIN_BYTES = 104 * 1024;  // largest LDS window
 constexpr int WAVE_Q = 64;                // crosser queue of one wavefront
 
 // Walks tile slots in increasing order: particle index of slot k (rows of
@@ -902,7 +904,7 @@ __device__ __forceinline__ void drain_wave_queue(
 // floats, to this tile's slot of `wbuf`; k_merge_windows then adds the
 // windows into the accumulator, so no tile needs atomics for the cells it
 // shares.
-// (launch bounds: at least 4 waves per SIMD, i.e. two 512-lane workgroups
+// (launch bounds: at least 4 waves per SIMD, i.e. one 1024-lane workgroup
 // per CU, which caps the kernel at 128 VGPRs)
 template <int GS>
 __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
@@ -918,6 +920,12 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   __shared__ unsigned qw_bits;  // largest |q w| of the first round (bits)
   __shared__ int bin_cnt[27];
   __shared__ int oow_cnt;
+  // interpolation coefficients of the tile's cells (gathers from LDS cost
+  // far fewer cache requests than per-lane global gathers once the
+  // particles have drifted out of voxel order)
+  __shared__ float4 ic_e0[TILE_MAXCELLS], ic_e1[TILE_MAXCELLS],
+      ic_e2[TILE_MAXCELLS], ic_b0[TILE_MAXCELLS];
+  __shared__ float2 ic_b1[TILE_MAXCELLS];
 
   const int t = threadIdx.x;
   const int lane = lane_id();
@@ -983,6 +991,17 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     row_pref[r + 1] = e - s;
   }
   for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
+  const int cx = x1 - x0 + 1, cxy = cx * rows_y;
+  for (int c = t; c < cxy * (z1 - z0 + 1); c += TILE_BS) {
+    const int v = vox(x0 + c % cx, y0 + (c / cx) % rows_y, z0 + c / cxy,
+                      G.nx, G.ny);
+    const float4 *fp = reinterpret_cast<const float4 *>(f0 + v);
+    ic_e0[c] = fp[0];
+    ic_e1[c] = fp[1];
+    ic_e2[c] = fp[2];
+    ic_b0[c] = fp[3];
+    ic_b1[c] = reinterpret_cast<const float2 *>(fp + 4)[0];
+  }
   if (t == 0) qw_bits = 0u;
   if (t < 27) bin_cnt[t] = 0;
   if (t == 0) oow_cnt = 0;
@@ -1069,7 +1088,18 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     nbad += valid && !active;
     const int ii = active ? iraw : 0;
     if (active && ii != fkey) {
-      load_interp(f0, ii, F);
+      int x, y, z;
+      VD.coords(G, ii, x, y, z);
+      if (x >= x0 && x <= x1 && y >= y0 && y <= y1 && z >= z0 && z <= z1) {
+        const int c = (x - x0) + cx * (y - y0) + cxy * (z - z0);
+        F.e0 = ic_e0[c];
+        F.e1 = ic_e1[c];
+        F.e2 = ic_e2[c];
+        F.b0 = ic_b0[c];
+        F.b1 = ic_b1[c];
+      } else {
+        load_interp(f0, ii, F);
+      }
       fkey = ii;
     }
     // prefetch this lane's next particle (issued after the coefficients,
@@ -1508,7 +1538,7 @@ int pow2_at_least(int64_t n) {
   return (int)p;
 }
 
-TileGeom g_tile = {8, 8, 4, 0, 0, 0, 0, 0};
+TileGeom g_tile = {8, 8, 8, 0, 0, 0, 0, 0};
 int g_group = 16;  // lanes sharing one run of particles in the tiled push
 int g_bin_mode = 1;  // tile lists: 0 off, 1 when drift warrants, 2 always
 
@@ -1560,8 +1590,10 @@ unsigned div_magic(unsigned d) {
 // Tile shape of the tiled push (set by the tuner; the window of a tile is
 // (tx+2)(ty+2)(tz+2) cells of 48 B in LDS).
 int push_set_tile(int tx, int ty, int tz) {
-  if (tx < 1 || ty < 1 || tz < 1 || ty * tz > TILE_MAXROWS) return VPIC_EINVAL;
-  if ((size_t)(tx + 2) * (ty + 2) * (tz + 2) * 96 > 96 * 1024)
+  if (tx < 1 || ty < 1 || tz < 1 || ty * tz > TILE_MAXROWS ||
+      tx * ty * tz > TILE_MAXCELLS)
+    return VPIC_EINVAL;
+  if ((size_t)(tx + 2) * (ty + 2) * (tz + 2) * 96 > TILE_WIN_BYTES)
     return VPIC_EINVAL;
   g_tile.tx = tx;
   g_tile.ty = ty;
@@ -1661,7 +1693,7 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       if (!attr_set[ak]) {
         VPIC_HIP(hipFuncSetAttribute(
             reinterpret_cast<const void *>(kern),
-            hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+            hipFuncAttributeMaxDynamicSharedMemorySize, TILE_WIN_BYTES));
         attr_set[ak] = true;
       }
       // Tile lists from the codes of the last tiled push of this array.
