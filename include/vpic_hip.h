@@ -104,6 +104,15 @@ typedef struct vpic_grid {
   int32_t fbc[6];
 } vpic_grid_t;
 
+/* hydro_t (src/sf_interface/sf_interface.h:185-192): 16 floats, 64 B. */
+typedef struct vpic_hydro {
+  float jx, jy, jz, rho;
+  float px, py, pz, ke;
+  float txx, tyy, tzz;
+  float tyz, tzx, txy;
+  float pad_[2];
+} vpic_hydro_t;
+
 /* Field-solve material coefficients for the one vacuum material and the
  * TCA damping, src/field_advance/standard/sfa_private.h:14-30 and
  * src/field_advance/standard/sfa.cc:100-136. */
@@ -356,6 +365,34 @@ int vpic_hip_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
  * receives the summed squared differences (standard/remote.cc:298-416). */
 int vpic_hip_synchronize_tang_e_norm_b(double *host_err, vpic_field_t *f,
                                        const vpic_grid_t *g, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Hydro moments (src/species_advance/standard/hydro_p.cc,                  */
+/* src/sf_interface/hydro_array.cc)                                         */
+/* ------------------------------------------------------------------------ */
+
+/* clear_hydro_array (sf_interface/clear_array.cc:29-38): h[0..nv) = 0. */
+int vpic_hip_clear_hydro_array(vpic_hydro_t *h, const vpic_grid_t *g,
+                               void *stream);
+
+/* accumulate_hydro_p (hydro_p.cc:12-19 -> hydro_p_pipeline.cc:17-242):
+ * adds the 14 moments of p[0..np) to the nodes, trilinear weights.  With
+ * `partition` (from the last vpic_hip_sort_p of these particles) the
+ * tiles of the partition are summed in LDS, the particles past it one by
+ * one; without it every particle one by one.  The per-particle arithmetic
+ * is the reference's; the node sums differ only in order.  Synchronous;
+ * particles whose voxel is not interior are skipped and reported as an
+ * error.  reduce_hydro_array has nothing to do (one device copy). */
+int vpic_hip_accumulate_hydro_p(vpic_hydro_t *h, const vpic_particle_t *p,
+                                int32_t np, float q, float m,
+                                const vpic_interpolator_t *f,
+                                const vpic_grid_t *g, const int32_t *partition,
+                                void *stream);
+
+/* synchronize_hydro_array for one domain (hydro_array.cc:131-311): local
+ * faces doubled, then the periodic node planes summed. */
+int vpic_hip_synchronize_hydro_array(vpic_hydro_t *h, const vpic_grid_t *g,
+                                     void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Set-up helpers (no reference counterpart on the device; they fill the   */

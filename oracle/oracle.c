@@ -1457,3 +1457,164 @@ double orc_synchronize_tang_e_norm_b(vpic_field_t *f, const vpic_grid_t *g) {
   }
   return err;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Hydro moments (hydro_p_pipeline.cc, hydro_array.cc)                      */
+/* ------------------------------------------------------------------------ */
+
+/* accumulate_hydro_p_pipeline_scalar for particles [n0, n1) into h
+ * (hydro_p_pipeline.cc:17-210). */
+static void hydro_range(vpic_hydro_t *h, const vpic_particle_t *p, int n0,
+                        int n1, float qsp, float msp,
+                        const vpic_interpolator_t *f, const vpic_grid_t *g) {
+  const float qdt_2mc = (qsp * g->dt) / (2 * msp * g->cvac);
+  const float qdt_4mc2 = qdt_2mc / (2 * g->cvac);
+  const float mspc = g->cvac * msp;
+  const float c = g->cvac;
+  const float r8V = g->r8V;
+  const float one = 1.0, one_third = 1.0 / 3.0;
+  const int stride_10 = 1, stride_21 = g->sy - 1, stride_43 = g->sz - g->sy - 1;
+  for (int n = n0; n < n1; n++) {
+    float dx = p[n].dx, dy = p[n].dy, dz = p[n].dz;
+    int i = p[n].i;
+    float ux = p[n].ux, uy = p[n].uy, uz = p[n].uz, w = p[n].w;
+    const vpic_interpolator_t *fi = f + i;
+    float vx, vy, vz, ke_mc, t, w0, w1, w2, w3, w4, w5, w6, w7;
+    ux += qdt_2mc * ((fi->ex + dy * fi->dexdy) + dz * (fi->dexdz + dy * fi->d2exdydz));
+    uy += qdt_2mc * ((fi->ey + dz * fi->deydz) + dx * (fi->deydx + dz * fi->d2eydzdx));
+    uz += qdt_2mc * ((fi->ez + dx * fi->dezdx) + dy * (fi->dezdy + dx * fi->d2ezdxdy));
+    w5 = fi->cbx + dx * fi->dcbxdx;
+    w6 = fi->cby + dy * fi->dcbydy;
+    w7 = fi->cbz + dz * fi->dcbzdz;
+    ke_mc = ux * ux + uy * uy + uz * uz;
+    vz = sqrtf(one + ke_mc);
+    ke_mc *= c / (vz + one);
+    vz = c / vz;
+    w0 = qdt_4mc2 * vz;
+    w1 = w5 * w5 + w6 * w6 + w7 * w7;
+    w2 = w0 * w0 * w1;
+    w3 = w0 * (one + (one_third) * w2 * (one + 0.4f * w2));
+    w4 = w3 / (one + w1 * w3 * w3);
+    w4 += w4;
+    w0 = ux + w3 * (uy * w7 - uz * w6);
+    w1 = uy + w3 * (uz * w5 - ux * w7);
+    w2 = uz + w3 * (ux * w6 - uy * w5);
+    ux += w4 * (w1 * w7 - w2 * w6);
+    uy += w4 * (w2 * w5 - w0 * w7);
+    uz += w4 * (w0 * w6 - w1 * w5);
+    vx = ux * vz;
+    vy = uy * vz;
+    vz = uz * vz;
+    w0 = r8V * w;
+    dx *= w0;
+    w1 = w0 + dx;
+    w0 -= dx;
+    w3 = one + dy;
+    w2 = w0 * w3;
+    w3 *= w1;
+    dy = one - dy;
+    w0 *= dy;
+    w1 *= dy;
+    w7 = one + dz;
+    w4 = w0 * w7;
+    w5 = w1 * w7;
+    w6 = w2 * w7;
+    w7 *= w3;
+    dz = one - dz;
+    w0 *= dz;
+    w1 *= dz;
+    w2 *= dz;
+    w3 *= dz;
+    const float wn[8] = {w0, w1, w2, w3, w4, w5, w6, w7};
+    const int step[8] = {0, stride_10, stride_21, stride_10,
+                         stride_43, stride_10, stride_21, stride_10};
+    for (int k = 0; k < 8; k++) {
+      i += step[k];
+      vpic_hydro_t *hh = h + i;
+      t = qsp * wn[k];
+      hh->jx += t * vx;
+      hh->jy += t * vy;
+      hh->jz += t * vz;
+      hh->rho += t;
+      t = mspc * wn[k];
+      dx = t * ux;
+      dy = t * uy;
+      dz = t * uz;
+      hh->px += dx;
+      hh->py += dy;
+      hh->pz += dz;
+      hh->ke += t * ke_mc;
+      hh->txx += dx * vx;
+      hh->tyy += dy * vy;
+      hh->tzz += dz * vz;
+      hh->tyz += dy * vz;
+      hh->tzx += dz * vx;
+      hh->txy += dx * vy;
+    }
+  }
+}
+
+void orc_accumulate_hydro_p(vpic_hydro_t *h, const vpic_particle_t *p, int np,
+                            float q, float m, const vpic_interpolator_t *f,
+                            const vpic_grid_t *g) {
+  /* one pipeline: rank 0 accumulates into h (copy 0), the straggler rank
+   * into a cleared copy 1, then reduce_hydro_array sets h = h + copy 1
+   * (hydro_p_pipeline.cc:214-242; reduce_array_pipeline.cc, O1) */
+  int i0, n0, i1, n1;
+  distribute(np, 16, 0, 1, &i0, &n0);
+  distribute(np, 16, 1, 1, &i1, &n1);
+  vpic_hydro_t *c1 = (vpic_hydro_t *)calloc((size_t)g->nv, sizeof(*h));
+  hydro_range(h, p, i0, i0 + n0, q, m, f, g);
+  hydro_range(c1, p, i1, i1 + n1, q, m, f, g);
+  for (int v = 0; v < g->nv; v++) {
+    float *a = (float *)(h + v);
+    const float *b = (const float *)(c1 + v);
+    for (int k = 0; k < 14; k++) a[k] = a[k] + b[k];
+  }
+  free(c1);
+}
+
+void orc_clear_hydro(vpic_hydro_t *h, const vpic_grid_t *g) {
+  memset(h, 0, sizeof(*h) * (size_t)g->nv);
+}
+
+/* synchronize_hydro_array for one domain (hydro_array.cc:131-311): local
+ * faces doubled (order -x,-y,-z,+x,+y,+z), then the x, y, z periodic face
+ * pairs each take lw*h + rw*remote with lw = rw = 1. */
+void orc_synchronize_hydro(vpic_hydro_t *h, const vpic_grid_t *g) {
+  for (int k = 0; k < 6; k++) {
+    if (g->fbc[k] == VPIC_BC_PERIODIC) continue;
+    const int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
+    int c[3];
+    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+        c[X] = face; c[Y] = yy; c[Z] = zz;
+        float *a = (float *)(h + fvox(g, c));
+        for (int q = 0; q < 14; q++) a[q] *= 2;
+      }
+  }
+  for (int X = 0; X < 3; X++) {
+    if (g->fbc[X] != VPIC_BC_PERIODIC || g->fbc[X + 3] != VPIC_BC_PERIODIC)
+      continue;
+    const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const float d = X == 0 ? g->dx : X == 1 ? g->dy : g->dz;
+    float rw = d, lw = rw + d;
+    rw /= lw;
+    lw = d / lw;
+    lw += lw;
+    rw += rw;
+    int a[3], b[3];
+    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
+      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
+        a[X] = 1; a[Y] = yy; a[Z] = zz;
+        b[X] = nd(g, X) + 1; b[Y] = yy; b[Z] = zz;
+        float *pa = (float *)(h + fvox(g, a)), *pb = (float *)(h + fvox(g, b));
+        for (int q = 0; q < 14; q++) {
+          const float va = pa[q], vb = pb[q];
+          pb[q] = lw * vb + rw * va;
+          pa[q] = lw * va + rw * vb;
+        }
+      }
+  }
+}

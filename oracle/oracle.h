@@ -144,6 +144,15 @@ void orc_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
                         const vpic_field_params_t *mp);
 double orc_synchronize_tang_e_norm_b(vpic_field_t *f, const vpic_grid_t *g);
 
+/* accumulate_hydro_p with one pipeline and its reduction
+ * (hydro_p_pipeline.cc:17-242, reduce_array_pipeline.cc:209-231), clear,
+ * and synchronize_hydro_array for one domain (hydro_array.cc:131-311). */
+void orc_accumulate_hydro_p(vpic_hydro_t *h, const vpic_particle_t *p, int np,
+                            float q, float m, const vpic_interpolator_t *f,
+                            const vpic_grid_t *g);
+void orc_clear_hydro(vpic_hydro_t *h, const vpic_grid_t *g);
+void orc_synchronize_hydro(vpic_hydro_t *h, const vpic_grid_t *g);
+
 /* The reference's CPU parallel scheme for advance_p, used only as the
  * timed CPU baseline: n_thread pthreads each run one DISTRIBUTE(np,16)
  * block into a private accumulator copy, the caller runs the np%16 tail,

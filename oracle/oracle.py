@@ -90,6 +90,13 @@ class Oracle:
             getattr(lib, name).argtypes = [_vp, _pg,
                                            ctypes.POINTER(T.FieldParams)]
             getattr(lib, name).restype = None
+        lib.orc_accumulate_hydro_p.argtypes = [_vp, _vp, _i32, _f32, _f32,
+                                               _vp, _pg]
+        lib.orc_accumulate_hydro_p.restype = None
+        lib.orc_clear_hydro.argtypes = [_vp, _pg]
+        lib.orc_clear_hydro.restype = None
+        lib.orc_synchronize_hydro.argtypes = [_vp, _pg]
+        lib.orc_synchronize_hydro.restype = None
         for name in ("orc_compute_div_b_err", "orc_clean_div_b"):
             getattr(lib, name).argtypes = [_vp, _pg]
             getattr(lib, name).restype = None
@@ -179,6 +186,14 @@ class Oracle:
                               ctypes.byref(mp))
         return en
 
+    # hydro moments
+    def accumulate_hydro_p(self, h, p, q, m, fi, g):
+        self.lib.orc_accumulate_hydro_p(_ptr(h), _ptr(p), len(p), q, m,
+                                        _ptr(fi), ctypes.byref(g))
+
+    def synchronize_hydro(self, h, g):
+        self.lib.orc_synchronize_hydro(_ptr(h), ctypes.byref(g))
+
     # divergence cleaning / initialization kernels of the vacuum table
     def compute_div_e_err(self, f, g, mp):
         self.lib.orc_compute_div_e_err(_ptr(f), ctypes.byref(g),
@@ -243,6 +258,9 @@ class Ref:
         for name in ("ref_div_b_err", "ref_clean_div_b_bulk"):
             getattr(lib, name).argtypes = [_vp, _pg]
             getattr(lib, name).restype = None
+        lib.ref_accumulate_hydro_p.argtypes = [_vp, _vp, ctypes.c_int, _f32,
+                                               _f32, _vp, _pg]
+        lib.ref_accumulate_hydro_p.restype = None
         for name in ("ref_rms_div_b_err", "ref_rms_div_e_err_bulk"):
             getattr(lib, name).argtypes = [_vp, _pg]
             getattr(lib, name).restype = ctypes.c_double
@@ -344,3 +362,7 @@ class Ref:
 
     def rms_div_e_err_bulk(self, f, g):
         return self.lib.ref_rms_div_e_err_bulk(_ptr(f), ctypes.byref(g))
+
+    def accumulate_hydro_p(self, h, p, q, m, fi, g):
+        self.lib.ref_accumulate_hydro_p(_ptr(h), _ptr(p), len(p), q, m,
+                                        _ptr(fi), ctypes.byref(g))

@@ -298,6 +298,46 @@ def test_divergence_cleaning_bitexact(dev, orc, op):
             assert H.same_bits(got[c], fo[c]), (name, op, c)
 
 
+def _hydro_err(got, want):
+    worst = 0.0
+    for c in T.HYDRO_FIELDS:
+        den = float(np.max(np.abs(want[c])))
+        if den:
+            worst = max(worst, float(np.max(np.abs(got[c] - want[c]))) / den)
+        else:
+            assert not np.any(got[c]), c
+    return worst
+
+
+@pytest.mark.parametrize("sorted_", [False, True], ids=["flat", "tiled"])
+def test_accumulate_hydro_p(dev, orc, sorted_):
+    """accumulate_hydro_p + synchronize_hydro_array against the oracle (node
+    sums to 1e-5 of each moment's largest value), flat and tiled, with
+    particles appended after the sort."""
+    for name, g in _field_bc_grids():
+        p, fi = _inputs(orc, g, 9, 0.6, 81, e_amp=0.2, b_amp=0.3,
+                        shuffle=True)
+        part = None
+        if sorted_:
+            part = orc.sort_p(p, g)
+            p = np.concatenate([p, H.thermal_particles(g, 1, 0.6, 82,
+                                                       shuffle=True)[:53]])
+        ho = np.zeros(g.nv, dtype=T.HYDRO)
+        orc.accumulate_hydro_p(ho, p, -1.0, 1.0, fi, g)
+        orc.synchronize_hydro(ho, g)
+        dh = dev.DeviceArray((g.nv,), T.HYDRO)
+        dp = dev.DeviceArray.from_host(p)
+        dfi = dev.DeviceArray.from_host(fi)
+        dpart = dev.DeviceArray.from_host(part) if part is not None else None
+        G = ctypes.byref(g)
+        say("accumulate_hydro_p,", name, "tiled" if sorted_ else "flat")
+        _lib.call("vpic_hip_clear_hydro_array", dh.ptr, G, None)
+        _lib.call("vpic_hip_accumulate_hydro_p", dh.ptr, dp.ptr, len(p), -1.0,
+                  1.0, dfi.ptr, G, dpart.ptr if dpart else None, None)
+        _lib.call("vpic_hip_synchronize_hydro_array", dh.ptr, G, None)
+        assert _hydro_err(dh.download(), ho) <= 1e-5, name
+
+
 def test_synchronize_rho_bitexact(dev, orc):
     for name, g in _field_bc_grids():
         f = H.random_full_fields(g, 5)

@@ -445,3 +445,66 @@ def test_synchronize_tang_e_norm_b_closed_forms(orc):
     assert abs(err - 2 * float(d) ** 2) <= 1e-12
     want = np.float32(0.5 * (float(lo) + float(lo + d)))
     assert a["cbx"][2, 2, 1] == want and a["cbx"][2, 2, g.nx + 1] == want
+
+
+# -- hydro moments -----------------------------------------------------------
+
+@pytest.mark.parametrize("q,m,uth", [(-1.0, 1.0, 0.2), (2.0, 25.0, 1.5)])
+def test_accumulate_hydro_p_bitexact(orc, ref, q, m, uth):
+    g = _box(6, 5, 4)
+    p, fi = _push_inputs(g, 9, uth, 31, e_amp=0.2, b_amp=0.3)
+    h0 = np.zeros(g.nv, dtype=T.HYDRO)
+    h0[:] = np.random.default_rng(2).standard_normal((g.nv, 16)).astype(
+        np.float32).view(T.HYDRO).reshape(-1)
+    ha, hb = h0.copy(), h0.copy()
+    orc.accumulate_hydro_p(ha, p, q, m, fi, g)
+    ref.accumulate_hydro_p(hb, p, q, m, fi, g)
+    for c in T.HYDRO_FIELDS:
+        assert H.same_bits(ha[c], hb[c]), c
+
+
+def test_hydro_closed_forms(orc):
+    """A particle at rest at a cell centre with zero fields puts w r8V q
+    (rho) and w r8V m c (only through t; px = 0) on each of its 8 nodes;
+    sums over the nodes give the particle's totals."""
+    g = _box(4, 4, 4)
+    fi = np.zeros(g.nv, dtype=T.INTERPOLATOR)
+    p = np.zeros(1, dtype=T.PARTICLE)
+    p["i"] = g.voxel(2, 2, 2)
+    p["w"] = 3.0
+    p["ux"] = 0.5
+    h = np.zeros(g.nv, dtype=T.HYDRO)
+    orc.accumulate_hydro_p(h, p, -2.0, 4.0, fi, g)
+    r8V = float(g.r8V)
+    gam = np.sqrt(1 + 0.25)
+    nodes = [g.voxel(2 + a, 2 + b, 2 + c) for c in (0, 1) for b in (0, 1)
+             for a in (0, 1)]
+    assert np.allclose(h["rho"][nodes], -2.0 * 3.0 * r8V, rtol=1e-6)
+    assert np.allclose(h["px"][nodes], 4.0 * 3.0 * r8V * 0.5, rtol=1e-6)
+    assert np.allclose(h["jx"][nodes], -2.0 * 3.0 * r8V * 0.5 / gam, rtol=1e-6)
+    assert np.allclose(h["ke"][nodes], 4.0 * 3.0 * r8V * (gam - 1), rtol=1e-5)
+    assert np.allclose(h["txx"][nodes], 4.0 * 3.0 * r8V * 0.25 / gam, rtol=1e-6)
+    other = np.ones(g.nv, bool)
+    other[nodes] = False
+    assert not np.any(h["rho"][other])
+
+
+def test_synchronize_hydro_closed_forms(orc):
+    """Periodic faces: both node planes take the sum; local faces double."""
+    g = _box(4, 3, 5)
+    h = np.zeros(g.nv, dtype=T.HYDRO)
+    rng = np.random.default_rng(4)
+    for c in T.HYDRO_FIELDS:
+        h[c] = rng.integers(-8, 8, g.nv).astype(np.float32)
+    a = h.reshape(g.nz + 2, g.ny + 2, g.nx + 2)
+    want = a.copy()
+    s = want["rho"][2, 2, 1] + want["rho"][2, 2, g.nx + 1]
+    orc.synchronize_hydro(h, g)
+    assert a["rho"][2, 2, 1] == s and a["rho"][2, 2, g.nx + 1] == s
+    g2 = g.copy()
+    g2.fbc[0] = g2.fbc[3] = T.BC_PEC_FIELDS
+    h2 = want.copy().reshape(-1)
+    orc.synchronize_hydro(h2, g2)
+    b = h2.reshape(a.shape)
+    assert b["jx"][2, 2, 1] == 2 * want["jx"][2, 2, 1]
+    assert b["jx"][2, 2, g.nx + 1] == 2 * want["jx"][2, 2, g.nx + 1]

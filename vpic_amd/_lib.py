@@ -58,6 +58,10 @@ _SIGS = {
     "vpic_hip_compute_rhob": [_vp, _pg, _pmp, _vp],
     "vpic_hip_compute_curl_b": [_vp, _pg, _pmp, _vp],
     "vpic_hip_synchronize_tang_e_norm_b": [_pf64, _vp, _pg, _vp],
+    "vpic_hip_clear_hydro_array": [_vp, _pg, _vp],
+    "vpic_hip_accumulate_hydro_p": [_vp, _vp, _i32, _f32, _f32, _vp, _pg, _vp,
+                                    _vp],
+    "vpic_hip_synchronize_hydro_array": [_vp, _pg, _vp],
     "vpic_hip_grid_periodic": [_pg, _f64, _f64, _f64, _f64, _f64, _f64, _i32,
                                _i32, _i32, _f32, _f32, _f32],
     "vpic_hip_field_params_vacuum": [_pmp, _f32],

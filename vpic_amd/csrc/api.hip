@@ -505,6 +505,40 @@ extern "C" int vpic_hip_synchronize_tang_e_norm_b(double *host_err,
   return launch_synchronize_tang_e_norm_b(host_err, f, g, S(stream));
 }
 
+// Hydro moments.
+extern "C" int vpic_hip_clear_hydro_array(vpic_hydro_t *h, const vpic_grid_t *g,
+                                          void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(h && aligned16(h), "bad argument");
+  return launch_clear_hydro(h, g, S(stream));
+}
+
+extern "C" int vpic_hip_accumulate_hydro_p(vpic_hydro_t *h,
+                                           const vpic_particle_t *p,
+                                           int32_t np, float q, float m,
+                                           const vpic_interpolator_t *f,
+                                           const vpic_grid_t *g,
+                                           const int32_t *partition,
+                                           void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(h && f && aligned16(h) && (np == 0 || p) && np >= 0,
+        "Bad args.");
+  CHECK(m != 0, "Bad args: m == 0");
+  return launch_accumulate_hydro_p(h, p, np, q, m, f, g,
+                                   (const int *)partition, S(stream));
+}
+
+extern "C" int vpic_hip_synchronize_hydro_array(vpic_hydro_t *h,
+                                                const vpic_grid_t *g,
+                                                void *stream) {
+  int rc = check_grid(g);
+  if (rc || (rc = check_fields_supported(g))) return rc;
+  CHECK(h && aligned16(h), "NULL hydro array.");
+  return launch_synchronize_hydro(h, g, S(stream));
+}
+
 // ---------------------------------------------------------------------------
 // set-up helpers
 // ---------------------------------------------------------------------------

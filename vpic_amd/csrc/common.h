@@ -204,6 +204,14 @@ int launch_compute_curl_b(vpic_field_t *f, const vpic_grid_t *g,
                           const vpic_field_params_t *m, hipStream_t s);
 int launch_synchronize_tang_e_norm_b(double *out, vpic_field_t *f,
                                      const vpic_grid_t *g, hipStream_t s);
+int launch_clear_hydro(vpic_hydro_t *h, const vpic_grid_t *g, hipStream_t s);
+int launch_accumulate_hydro_p(vpic_hydro_t *h, const vpic_particle_t *p,
+                              int np, float q, float m,
+                              const vpic_interpolator_t *f,
+                              const vpic_grid_t *g, const int *partition,
+                              hipStream_t s);
+int launch_synchronize_hydro(vpic_hydro_t *h, const vpic_grid_t *g,
+                             hipStream_t s);
 int launch_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
                                int ppc, float uthx, float uthy, float uthz,
                                float udx, float udy, float udz, float w,

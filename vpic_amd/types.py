@@ -70,6 +70,12 @@ class Grid(ctypes.Structure):
         return x + (self.nx + 2) * (y + (self.ny + 2) * z)
 
 
+HYDRO_FIELDS = ("jx", "jy", "jz", "rho", "px", "py", "pz", "ke", "txx",
+                "tyy", "tzz", "tyz", "tzx", "txy")
+HYDRO = np.dtype([(n, np.float32) for n in HYDRO_FIELDS] +
+                 [("pad0", np.float32), ("pad1", np.float32)])
+
+
 class FieldParams(ctypes.Structure):
     """vpic_field_params_t."""
     _fields_ = [(n, ctypes.c_float) for n in
