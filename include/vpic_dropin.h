@@ -138,6 +138,22 @@ typedef struct accumulator_array {
   grid_t *g;
 } accumulator_array_t;
 
+/* src/sf_interface/sf_interface.h:185-200 */
+typedef struct hydro {
+  float jx, jy, jz, rho;
+  float px, py, pz, ke;
+  float txx, tyy, tzz;
+  float tyz, tzx, txy;
+  float _pad[2];
+} hydro_t;
+
+typedef struct hydro_array {
+  hydro_t *h;
+  int n_pipeline;
+  int stride;
+  grid_t *g;
+} hydro_array_t;
+
 /* src/field_advance/field_advance.h:152-229 */
 typedef struct field {
   float ex, ey, ez, div_e_err;
@@ -235,6 +251,13 @@ void accumulate_rho_p(field_array_t *fa, const species_t *sp);
 void center_p(species_t *sp, const interpolator_array_t *ia);
 void uncenter_p(species_t *sp, const interpolator_array_t *ia);
 
+/* species_advance.h:140-143 (hydro_p.cc:12-19, hydro_p_pipeline.cc:
+ * 218-244): adds the species' 14 moments into ha->h (copy 0).  After a
+ * sort_p of this library the sum runs cell-tiled.  Particles whose voxel
+ * is not interior are an error. */
+void accumulate_hydro_p(hydro_array_t *ha, const species_t *sp,
+                        const interpolator_array_t *ia);
+
 /* ------------------------------------------------------------------------ */
 /* Field-particle interface (src/sf_interface/sf_interface.h)               */
 /* ------------------------------------------------------------------------ */
@@ -253,6 +276,17 @@ void reduce_accumulator_array(accumulator_array_t *aa);
 /* sf_interface.h:173-174 (unload_accumulator.cc:11-23). */
 void unload_accumulator_array(field_array_t *fa,
                               const accumulator_array_t *aa);
+
+/* sf_interface.h:218-222 (clear_array.cc:29-38): zeroes every copy. */
+void clear_hydro_array(hydro_array_t *ha);
+
+/* sf_interface.h:224-231 (reduce_array.cc:29-38).  One device copy: a
+ * checked no-op, as for the accumulators. */
+void reduce_hydro_array(hydro_array_t *ha);
+
+/* sf_interface.h:233-240 (hydro_array.cc:131-311) for a domain that is
+ * the whole problem: local faces doubled, periodic node planes summed. */
+void synchronize_hydro_array(hydro_array_t *ha);
 
 /* ------------------------------------------------------------------------ */
 /* Field kernels.  The reference reaches them through fa->kernel            */
@@ -316,11 +350,12 @@ int vpic_hip_dropin_resident(int on);
 /* Copy between a host array and its device copy: to_device = 1 uploads the
  * host data (after host edits), 0 downloads the device data.  Species:
  * p[0:np] and pm[0:nm]; field array: f[0:nv]; interpolator: i[0:nv];
- * accumulator: copy 0. */
+ * accumulator and hydro arrays: copy 0. */
 int vpic_hip_species_sync(species_t *sp, int to_device);
 int vpic_hip_field_array_sync(field_array_t *fa, int to_device);
 int vpic_hip_interpolator_array_sync(interpolator_array_t *ia, int to_device);
 int vpic_hip_accumulator_array_sync(accumulator_array_t *aa, int to_device);
+int vpic_hip_hydro_array_sync(hydro_array_t *ha, int to_device);
 
 /* Free the device copy kept for a host array (call before freeing or
  * reusing the host array).  Unknown pointers are ignored. */

@@ -49,6 +49,10 @@
   S(accumulator_array_t) O(accumulator_array_t, a)                          \
   O(accumulator_array_t, n_pipeline) O(accumulator_array_t, stride)         \
   O(accumulator_array_t, g)                                                 \
+  S(hydro_t) O(hydro_t, jx) O(hydro_t, rho) O(hydro_t, px) O(hydro_t, ke)   \
+  O(hydro_t, txx) O(hydro_t, tyz) O(hydro_t, txy)                           \
+  S(hydro_array_t) O(hydro_array_t, h) O(hydro_array_t, n_pipeline)         \
+  O(hydro_array_t, stride) O(hydro_array_t, g)                              \
   S(field_t) O(field_t, ex) O(field_t, ey) O(field_t, ez)                   \
   O(field_t, div_e_err) O(field_t, cbx) O(field_t, cby) O(field_t, cbz)     \
   O(field_t, div_b_err) O(field_t, tcax) O(field_t, tcay) O(field_t, tcaz)  \

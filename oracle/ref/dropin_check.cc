@@ -46,6 +46,9 @@ void ref_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
                             const vpic_grid_t *g);
 void ref_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
                   const vpic_field_params_t *mp);
+void ref_accumulate_hydro_p(vpic_hydro_t *h, const vpic_particle_t *p, int np,
+                            float q, float m, const vpic_interpolator_t *f,
+                            const vpic_grid_t *g);
 }
 
 static int g_fail = 0;
@@ -428,6 +431,49 @@ static void check_species_kernels() {
     }
     report(num <= 1e-5 * den, "accumulate_rho_p + synchronize_rho",
            "max |d rhof| %.3g of %.3g", num, den);
+  }
+  // hydro moments in the sequence of dump.cc:240-242 (synchronize reduces
+  // first, hydro_array.cc:145-147); sorted (tiled) and unsorted (flat)
+  {
+    Species u;
+    make_species(u, b, 1.0f, 1836.0f, 5, 0.02f, 13);
+    Species *sps[2] = {&e, &u};
+    const char *names[2] = {"hydro moments (sorted species: tiled)",
+                            "hydro moments (unsorted species: flat)"};
+    const int nv = b.G.nv;
+    const int stride = POW2_CEIL(nv, 2);
+    for (int k = 0; k < 2; k++) {
+      std::vector<hydro_t> hh((size_t)2 * stride);
+      for (size_t j = 0; j < hh.size(); j++) hh[j].rho = 99.0f;  // stale
+      hydro_array_t ha;
+      ha.h = hh.data();
+      ha.n_pipeline = 1;
+      ha.stride = stride;
+      ha.g = &b.rg;
+      clear_hydro_array(&ha);
+      accumulate_hydro_p(&ha, &sps[k]->sp, &A.ia);
+      reduce_hydro_array(&ha);
+      synchronize_hydro_array(&ha);
+      std::vector<vpic_hydro_t> r(nv);
+      memset(r.data(), 0, sizeof(vpic_hydro_t) * nv);
+      ref_accumulate_hydro_p(r.data(), (const vpic_particle_t *)sps[k]->p.data(),
+                             sps[k]->sp.np, sps[k]->sp.q, sps[k]->sp.m,
+                             (const vpic_interpolator_t *)A.fi.data(), &b.G);
+      orc_synchronize_hydro(r.data(), &b.G);
+      double num = 0, den = 0;
+      for (int v = 0; v < nv; v++)
+        for (int c = 0; c < 14; c++) {
+          const double x = (&hh[v].jx)[c], y = (&r[v].jx)[c];
+          num = fmax(num, fabs(x - y));
+          den = fmax(den, fabs(y));
+        }
+      bool copy1 = true;  // the host's second copy was cleared
+      for (int v = 0; v < nv; v++) copy1 = copy1 && hh[stride + v].rho == 0;
+      report(num <= 1e-5 * den && copy1, names[k],
+             "max |d| %.3g of %.3g, copy 1 %s", num, den,
+             copy1 ? "clear" : "NOT CLEARED");
+    }
+    vpic_hip_dropin_release(u.sp.p);
   }
   vpic_hip_dropin_release_all();
 }

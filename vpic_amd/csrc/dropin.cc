@@ -26,6 +26,7 @@ static_assert(sizeof(interpolator_t) == sizeof(vpic_interpolator_t),
 static_assert(sizeof(accumulator_t) == sizeof(vpic_accumulator_t),
               "accumulator_t");
 static_assert(sizeof(field_t) == sizeof(vpic_field_t), "field_t");
+static_assert(sizeof(hydro_t) == sizeof(vpic_hydro_t), "hydro_t");
 
 namespace {
 
@@ -269,6 +270,7 @@ size_t pbytes(int n) { return (size_t)(n > 0 ? n : 0) * sizeof(particle_t); }
 size_t abytes(const grid_t *g) { return (size_t)g->nv * sizeof(accumulator_t); }
 size_t ibytes(const grid_t *g) { return (size_t)g->nv * sizeof(interpolator_t); }
 size_t fbytes(const grid_t *g) { return (size_t)g->nv * sizeof(field_t); }
+size_t hbytes(const grid_t *g) { return (size_t)g->nv * sizeof(hydro_t); }
 
 bool species_ok(const char *fn, const species_t *sp) {
   if (!sp || !sp->g || sp->np < 0 || sp->np > sp->max_np ||
@@ -522,6 +524,37 @@ extern "C" void uncenter_p(species_t *sp, const interpolator_array_t *ia) {
   center("uncenter_p", sp, ia, 1);
 }
 
+extern "C" void accumulate_hydro_p(hydro_array_t *ha, const species_t *sp,
+                                   const interpolator_array_t *ia) {
+  const char *fn = "accumulate_hydro_p";
+  // hydro_p_pipeline.cc:224-231
+  if (!ha || !sp || !ia || ha->g != sp->g || ha->g != ia->g || !ha->h ||
+      !ia->i) {
+    fail(fn, "Bad args.");
+    return;
+  }
+  if (!species_ok(fn, sp) || !ensure_init(fn)) return;
+  const grid_t *g = sp->g;
+  vpic_grid_t G;
+  if (!make_grid(fn, g, &G)) return;
+  vpic_hydro_t *h;
+  vpic_particle_t *p;
+  vpic_interpolator_t *f;
+  if (!dev_in(fn, ha->h, hbytes(g), hbytes(g), &h) ||
+      !dev_in(fn, sp->p, pbytes(sp->max_np), pbytes(sp->np), &p) ||
+      !dev_in(fn, ia->i, ibytes(g), ibytes(g), &f))
+    return;
+  const SpeciesState &ss = st().species[sp];
+  const int32_t *part =
+      (ss.sorted && ss.part_nv == g->nv) ? ss.dpart : nullptr;
+  if (vpic_hip_accumulate_hydro_p(h, p, sp->np, sp->q, sp->m, f, &G, part,
+                                  nullptr) != 0) {
+    fail(fn, "%s", vpic_hip_last_error());
+    return;
+  }
+  dev_wrote(fn, ha->h, hbytes(g));
+}
+
 // ---------------------------------------------------------------------------
 // field-particle interface
 // ---------------------------------------------------------------------------
@@ -603,6 +636,56 @@ extern "C" void unload_accumulator_array(field_array_t *fa,
     return;
   }
   dev_wrote(fn, fa->f, fbytes(fa->g));
+}
+
+extern "C" void clear_hydro_array(hydro_array_t *ha) {
+  const char *fn = "clear_hydro_array";
+  if (!ha || !ha->h || !ha->g) {
+    fail(fn, "Bad args");
+    return;
+  }
+  if (!ensure_init(fn)) return;
+  vpic_grid_t G;
+  if (!make_grid(fn, ha->g, &G)) return;
+  vpic_hydro_t *h;
+  if (!dev_out(fn, ha->h, hbytes(ha->g), &h)) return;
+  if (vpic_hip_clear_hydro_array(h, &G, nullptr) != 0) {
+    fail(fn, "%s", vpic_hip_last_error());
+    return;
+  }
+  if (!dev_wrote(fn, ha->h, hbytes(ha->g))) return;
+  // clear_array_pipeline.cc:72-94 clears nv entries of every copy; the
+  // host-only copies 1..n_pipeline of a reference-built array get the same.
+  if (!st().resident)
+    for (int c = 1; c <= ha->n_pipeline; c++)
+      memset(ha->h + (size_t)c * ha->stride, 0, hbytes(ha->g));
+}
+
+extern "C" void reduce_hydro_array(hydro_array_t *ha) {
+  const char *fn = "reduce_hydro_array";
+  if (!ha || !ha->h || !ha->g) {
+    fail(fn, "Bad args.");
+    return;
+  }
+  // One device copy: the moments are already in copy 0.
+}
+
+extern "C" void synchronize_hydro_array(hydro_array_t *ha) {
+  const char *fn = "synchronize_hydro_array";
+  if (!ha || !ha->h || !ha->g) {
+    fail(fn, "NULL hydro array.");
+    return;
+  }
+  if (!ensure_init(fn)) return;
+  vpic_grid_t G;
+  if (!make_grid(fn, ha->g, &G)) return;
+  vpic_hydro_t *h;
+  if (!dev_in(fn, ha->h, hbytes(ha->g), hbytes(ha->g), &h)) return;
+  if (vpic_hip_synchronize_hydro_array(h, &G, nullptr) != 0) {
+    fail(fn, "%s", vpic_hip_last_error());
+    return;
+  }
+  dev_wrote(fn, ha->h, hbytes(ha->g));
 }
 
 // ---------------------------------------------------------------------------
@@ -843,6 +926,15 @@ extern "C" int vpic_hip_accumulator_array_sync(accumulator_array_t *aa,
                       : download("accumulator_array_sync", aa->a,
                                  abytes(aa->g));
   return ok ? 0 : vpic_set_error(-1, "accumulator_array_sync failed");
+}
+
+extern "C" int vpic_hip_hydro_array_sync(hydro_array_t *ha, int to_device) {
+  if (!ha || !ha->h || !ha->g) SYNC_FAIL("hydro_array_sync: Bad args");
+  if (!ensure_init("hydro_array_sync")) SYNC_FAIL("init failed");
+  bool ok = to_device ? upload("hydro_array_sync", ha->h, hbytes(ha->g),
+                               hbytes(ha->g))
+                      : download("hydro_array_sync", ha->h, hbytes(ha->g));
+  return ok ? 0 : vpic_set_error(-1, "hydro_array_sync failed");
 }
 
 extern "C" int vpic_hip_dropin_release(const void *host_array) {
