@@ -136,11 +136,13 @@ struct WindowSink {
       // Scaling by a power of two is exact in float, so when every scaled
       // component is below 2^31 the rounding to an integer is done in FP32
       // (the same integer the FP64 path gives).
+      // (the sum of the magnitudes bounds every component and is NaN when
+      // one is, which then takes the path below)
       const float fs = fscale;
-      bool small = true;
+      float big = fabsf(j[0]);
 #pragma unroll
-      for (int k = 0; k < 12; k++) small = small && fabsf(j[k] * fs) < 2147483520.0f;
-      if (small) {
+      for (int k = 1; k < 12; k++) big += fabsf(j[k]);
+      if (big * fs < 2147483520.0f) {
 #pragma unroll
         for (int k = 0; k < 12; k++)
           __hip_atomic_fetch_add(a + k * ncell,
@@ -527,6 +529,20 @@ __device__ __forceinline__ void push_compute_f(
   }
 }
 
+// sqrtf(x) for x >= 1, +inf or NaN, correctly rounded: the hardware square
+// root (within 1 ulp) followed by the one-ulp correction that sqrtf lowers
+// to on this target, without the rescaling of arguments below 2^-96 that
+// such x never need (the result is bit-identical to sqrtf).  An infinite
+// argument is returned as it is, as sqrtf does.
+__device__ __forceinline__ float sqrt_ge1(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __int_as_float(__float_as_int(s) - 1);
+  const float sp = __int_as_float(__float_as_int(s) + 1);
+  float r = (fmaf(-sm, s, x) <= 0.0f) ? sm : s;
+  r = (fmaf(-sp, s, x) > 0.0f) ? sp : r;
+  return x == __builtin_inff() ? x : r;
+}
+
 // The push of push_compute_f with the particle stored unconditionally to
 // dst (p + n, or a scratch slot for a lane without a particle): a lane that
 // moves nothing writes back what it read.  Keeping the same memory
@@ -553,7 +569,7 @@ __device__ __forceinline__ void push_compute_u(
   ux += hax;
   uy += hay;
   uz += haz;
-  v0 = C.qdt_2mc / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+  v0 = C.qdt_2mc / sqrt_ge1(one + (ux * ux + (uy * uy + uz * uz)));
   v1 = cbx * cbx + (cby * cby + cbz * cbz);
   v2 = (v0 * v0) * v1;
   v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
@@ -569,7 +585,7 @@ __device__ __forceinline__ void push_compute_u(
   uy += hay;
   uz += haz;
   const float4 Bnew = make_float4(ux, uy, uz, q);
-  v0 = one / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
+  v0 = one / sqrt_ge1(one + (ux * ux + (uy * uy + uz * uz)));
   ux *= C.cdt_dx;
   uy *= C.cdt_dy;
   uz *= C.cdt_dz;
@@ -595,12 +611,11 @@ __device__ __forceinline__ void push_compute_u(
   const float4 Anew = make_float4(v3, v4, v5, A.w);
   q *= C.qsp;
   const float v6 = q * ux * uy * uz * one_third;
-  float j[12];
-  streak4(j + 0, q * ux, v1, v2, v6);
-  streak4(j + 4, q * uy, v2, v0, v6);
-  streak4(j + 8, q * uz, v0, v1, v6);
-#pragma unroll
-  for (int k = 0; k < 12; k++) R.j[k] = stay ? j[k] : 0.0f;
+  // R.j is meaningful only for a particle that stays (the caller adds it
+  // under that condition)
+  streak4(R.j + 0, q * ux, v1, v2, v6);
+  streak4(R.j + 4, q * uy, v2, v0, v6);
+  streak4(R.j + 8, q * uz, v0, v1, v6);
   // per-component selects (a select of whole float4 values is lowered
   // through private memory)
   dst[0] = make_float4(stay ? Anew.x : A.x, stay ? Anew.y : A.y,
@@ -906,7 +921,14 @@ __device__ __forceinline__ void drain_wave_queue(
 // shares.
 // (launch bounds: at least 4 waves per SIMD, i.e. one 1024-lane workgroup
 // per CU, which caps the kernel at 128 VGPRs)
-template <int GS>
+//
+// MODE is the binning state the instance is compiled for: 0 no codes, 1
+// codes recorded (slots from the partition rows), 2 codes recorded and slots
+// from the tile lists.  The state lives on the device (control words set by
+// k_bin_decide), so a binning push launches all three instances and the two
+// that do not match return at once; compiling each state separately keeps
+// the unused paths' values out of the scalar registers of the particle loop.
+template <int GS, int MODE>
 __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
@@ -939,10 +961,14 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
             z1 = min(z0 + T.tz - 1, G.nz);
   const int rows_y = y1 - y0 + 1;
   const int nrow = rows_y * (z1 - z0 + 1);
-  const bool keyed =
-      Bn.key != nullptr && __builtin_amdgcn_readfirstlane(Bn.ctl[9]) != 0;
-  const bool listed =
-      keyed && __builtin_amdgcn_readfirstlane(Bn.ctl[5]) != 0;
+  {
+    const bool dkeyed =
+        Bn.key != nullptr && __builtin_amdgcn_readfirstlane(Bn.ctl[9]) != 0;
+    const bool dlisted =
+        dkeyed && __builtin_amdgcn_readfirstlane(Bn.ctl[5]) != 0;
+    if ((dlisted ? 2 : dkeyed ? 1 : 0) != MODE) return;
+  }
+  constexpr bool keyed = MODE >= 1, listed = MODE == 2;
 
   WindowSink W;
   W.lds = (lds_u64 *)lds_win;
@@ -1058,7 +1084,10 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   int nidx = (listed && k + GS < g1) ? L[k + GS] : 0;
   // One slot of this lane: push (A, B) = particle n while (An, Bn) is
   // loaded.  The loop below alternates two register pairs, so a prefetch
-  // is waited for only one full slot after it is issued.
+  // is waited for only one full slot after it is issued.  (Loading two
+  // slots ahead gains nothing: a coefficient load for a particle outside
+  // the tile is younger than the older prefetch, and the in-order counter
+  // of outstanding loads makes the wait for it drain both.)
   float4 A0, B0, A1, B1;
   load_particle(p, n, cur_ok, A0, B0);
   {
@@ -1568,16 +1597,23 @@ int tiled_kernel_id(int gs) {
   return id;  // log2 of the group size (1 .. 64)
 }
 
-TiledKernel tiled_kernel(int gs) {
+template <int MODE>
+TiledKernel tiled_kernel_mode(int gs) {
   switch (tiled_kernel_id(gs)) {
-    case 0: return k_advance_p_tiled<1>;
-    case 1: return k_advance_p_tiled<2>;
-    case 2: return k_advance_p_tiled<4>;
-    case 3: return k_advance_p_tiled<8>;
-    case 4: return k_advance_p_tiled<16>;
-    case 5: return k_advance_p_tiled<32>;
-    default: return k_advance_p_tiled<64>;
+    case 0: return k_advance_p_tiled<1, MODE>;
+    case 1: return k_advance_p_tiled<2, MODE>;
+    case 2: return k_advance_p_tiled<4, MODE>;
+    case 3: return k_advance_p_tiled<8, MODE>;
+    case 4: return k_advance_p_tiled<16, MODE>;
+    case 5: return k_advance_p_tiled<32, MODE>;
+    default: return k_advance_p_tiled<64, MODE>;
   }
+}
+
+TiledKernel tiled_kernel(int gs, int mode) {
+  return mode == 0 ? tiled_kernel_mode<0>(gs)
+                   : mode == 1 ? tiled_kernel_mode<1>(gs)
+                               : tiled_kernel_mode<2>(gs);
 }
 
 // ceil(2^32 / d) for fast_div.
@@ -1687,13 +1723,13 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         return rc;
       if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
         return rc;
-      const TiledKernel kern = tiled_kernel(g_group);
       static bool attr_set[7] = {false};
       const int ak = tiled_kernel_id(g_group);
       if (!attr_set[ak]) {
-        VPIC_HIP(hipFuncSetAttribute(
-            reinterpret_cast<const void *>(kern),
-            hipFuncAttributeMaxDynamicSharedMemorySize, TILE_WIN_BYTES));
+        for (int mode = 0; mode < 3; mode++)
+          VPIC_HIP(hipFuncSetAttribute(
+              reinterpret_cast<const void *>(tiled_kernel(g_group, mode)),
+              hipFuncAttributeMaxDynamicSharedMemorySize, TILE_WIN_BYTES));
         attr_set[ak] = true;
       }
       // Tile lists from the codes of the last tiled push of this array.
@@ -1758,11 +1794,16 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         BSt = &B;
       }
       if (timed) push_timing_record(s, np, true);
-      hipLaunchKernelGGL(kern, dim3(nblk), dim3(TILE_BS), lds, s, p, np,
-                         partition, f, (float *)a, (float *)S.wbuf.ptr,
-                         wstride, (vpic_particle_t *)S.trash.ptr, rlist, xcap,
-                         cnt, C, G, T, BA);
-      VPIC_LAUNCHED("k_advance_p_tiled");
+      // without binning only the mode-0 instance; with it all three, of
+      // which the one matching the device's binning state runs
+      for (int mode = 0; mode < (BSt ? 3 : 1); mode++) {
+        hipLaunchKernelGGL(tiled_kernel(g_group, mode), dim3(nblk),
+                           dim3(TILE_BS), lds, s, p, np, partition, f,
+                           (float *)a, (float *)S.wbuf.ptr, wstride,
+                           (vpic_particle_t *)S.trash.ptr, rlist, xcap, cnt,
+                           C, G, T, BA);
+        VPIC_LAUNCHED("k_advance_p_tiled");
+      }
       if (timed) push_timing_record(s, np, false);
       if (BSt) {
         hipLaunchKernelGGL(k_bin_post, dim3(1), dim3(64), 0, s,
