@@ -72,6 +72,15 @@ typedef struct particle_mover {
   int32_t i;
 } particle_mover_t;
 
+/* src/species_advance/species_advance_aos.h:45-52 */
+typedef struct particle_injector {
+  float dx, dy, dz;
+  int32_t i;
+  float ux, uy, uz, w;
+  float dispx, dispy, dispz;
+  species_id sp_id;
+} particle_injector_t;
+
 /* src/grid/grid.h:73-131 */
 typedef struct grid {
   float dt, cvac, eps0;
@@ -153,6 +162,24 @@ typedef struct hydro_array {
   int stride;
   grid_t *g;
 } hydro_array_t;
+
+/* src/boundary/boundary_private.h:17-42: a custom particle boundary; its
+ * id is the (negative) code its faces carry in the neighbor table, and
+ * boundary_p calls interact for each particle that hits one. */
+struct particle_bc;
+typedef struct particle_bc particle_bc_t;
+typedef int (*particle_bc_func_t)(void *b, struct species *sp, particle_t *p,
+                                  particle_mover_t *pm,
+                                  particle_injector_t *pi, int max_pi,
+                                  int face);
+typedef void (*delete_particle_bc_func_t)(particle_bc_t *pbc);
+struct particle_bc {
+  void *params;
+  particle_bc_func_t interact;
+  delete_particle_bc_func_t delete_pbc;
+  int64_t id;
+  particle_bc_t *next;
+};
 
 /* src/field_advance/field_advance.h:152-229 */
 typedef struct field {
@@ -257,6 +284,21 @@ void uncenter_p(species_t *sp, const interpolator_array_t *ia);
  * is not interior are an error. */
 void accumulate_hydro_p(hydro_array_t *ha, const species_t *sp,
                         const interpolator_array_t *ia);
+
+/* boundary.h:35-39 (boundary_p.cc:42-750) on a domain with no face shared
+ * with another rank: for every species, the movers advance_p left are
+ * taken last first; a particle on an absorbing face puts its charge on
+ * rhob and is removed, one on a custom boundary face (a particle_bc_t of
+ * pbc_list, matched by id) is handed to that boundary's interact function
+ * on the host and removed, anything else is dropped with the reference's
+ * warning; the last particle fills each hole, as in the reference.  The
+ * particles the handlers inject are then appended and moved with move_p,
+ * last first.  Particle storage is not resized: an injection that does not
+ * fit is dropped with the warning of the reference's fixed-size build.
+ * Custom handlers see the host field array (downloaded before and uploaded
+ * after them when any of them runs). */
+void boundary_p(particle_bc_t *pbc_list, species_t *sp_list,
+                field_array_t *fa, accumulator_array_t *aa);
 
 /* ------------------------------------------------------------------------ */
 /* Field-particle interface (src/sf_interface/sf_interface.h)               */

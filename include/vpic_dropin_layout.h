@@ -49,6 +49,11 @@
   S(accumulator_array_t) O(accumulator_array_t, a)                          \
   O(accumulator_array_t, n_pipeline) O(accumulator_array_t, stride)         \
   O(accumulator_array_t, g)                                                 \
+  S(particle_injector_t) O(particle_injector_t, i)                          \
+  O(particle_injector_t, w) O(particle_injector_t, dispx)                   \
+  O(particle_injector_t, sp_id)                                             \
+  S(particle_bc_t) O(particle_bc_t, params) O(particle_bc_t, interact)      \
+  O(particle_bc_t, delete_pbc) O(particle_bc_t, id) O(particle_bc_t, next)  \
   S(hydro_t) O(hydro_t, jx) O(hydro_t, rho) O(hydro_t, px) O(hydro_t, ke)   \
   O(hydro_t, txx) O(hydro_t, tyz) O(hydro_t, txy)                           \
   S(hydro_array_t) O(hydro_array_t, h) O(hydro_array_t, n_pipeline)         \

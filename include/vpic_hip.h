@@ -43,6 +43,29 @@ typedef struct vpic_particle_mover {
   int32_t i;
 } vpic_particle_mover_t;
 
+/* src/species_advance/species_advance_aos.h:45-52 (48 B): a particle to
+ * add to species sp_id, with the displacement it still has to move. */
+typedef struct vpic_particle_injector {
+  float dx, dy, dz;
+  int32_t i;
+  float ux, uy, uz, w;
+  float dispx, dispy, dispz;
+  int32_t sp_id;
+} vpic_particle_injector_t;
+
+/* What boundary_p found for one mover (64 B): the particle as the boundary
+ * handler sees it (voxel index restored, boundary_p.cc:257-266), the mover,
+ * the face of the voxel it hit, and the boundary's code (the neighbor
+ * table entry of that face: VPIC_BC_ABSORB_PARTICLES, a custom particle
+ * boundary <= -3, or VPIC_BOUNDARY_UNKNOWN). */
+typedef struct vpic_boundary_hit {
+  vpic_particle_t p;
+  vpic_particle_mover_t m;
+  int32_t face;
+  int32_t code;
+  int32_t pad_[2];
+} vpic_boundary_hit_t;
+
 /* src/sf_interface/sf_interface.h:62-74, scalar build (PAD_SIZE 2, 80 B). */
 typedef struct vpic_interpolator {
   float ex, dexdy, dexdz, d2exdydz;
@@ -81,7 +104,10 @@ enum {
   VPIC_BC_REFLECT_PARTICLES     = -1,
   VPIC_BC_ABSORB_PARTICLES      = -2,
   /* Face partner is this same domain (one-rank periodic wrap). */
-  VPIC_BC_PERIODIC              = 0
+  VPIC_BC_PERIODIC              = 0,
+  /* vpic_boundary_hit_t.code of a mover whose face leads to a voxel of
+   * another domain (boundary_p.cc:277-309): no such face on one domain. */
+  VPIC_BOUNDARY_UNKNOWN         = 1
 };
 
 /* The local grid, restating the fields of grid_t (src/grid/grid.h:73-131)
@@ -224,6 +250,42 @@ int vpic_hip_move_p(vpic_particle_t *p, vpic_particle_mover_t *pm,
 int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int32_t np,
                     int32_t *partition, const vpic_grid_t *g, int in_place,
                     void *stream);
+
+/* boundary_p for one species on a domain with no face shared with another
+ * rank (boundary_p.cc:42-750; the loop of :257-375).  pm[0..nm) are the
+ * movers advance_p left, in increasing particle order; each one's particle
+ * has the index 8*voxel + face of the face it stopped on.  Taken last
+ * first, as the reference does: the face's neighbor code decides
+ *   VPIC_BC_ABSORB_PARTICLES  its charge q*w goes to f.rhob
+ *                             (accumulate_rhob, rho_p.cc:126-173);
+ *   a code <= -3              a custom particle boundary: the caller's
+ *                             handler runs on hits[m] (host side);
+ *   anything else             dropped (the reference warns, :348-352);
+ * and every mover's particle is removed by moving the last particle into
+ * its slot (:354-371), which this call does in parallel with the same
+ * result.  hits[m] (nm entries, may be NULL) receives what mover m found.
+ * *np is updated; counts[0] = custom hits, counts[1] = dropped (host
+ * values).  Synchronous.  Particles are bit-identical to the reference;
+ * rhob sums differ only in order. */
+int vpic_hip_boundary_p(vpic_particle_t *p, int32_t *np,
+                        const vpic_particle_mover_t *pm, int32_t nm,
+                        vpic_field_t *f, float q, const vpic_grid_t *g,
+                        vpic_boundary_hit_t *hits, int32_t *counts,
+                        void *stream);
+
+/* The injection loop of boundary_p (boundary_p.cc:595-711) for one
+ * species: injectors pi[n-1] down to pi[0] are appended to p (p[np] =
+ * particle, np += 1) and moved by their displacement with move_p, which
+ * deposits into a; a particle that hits a boundary again leaves a mover
+ * (pm[nm], nm += 1).  In order, on one lane.  An injector that finds p or
+ * pm full is dropped and counted in counts[0] / counts[1] (the reference
+ * built without dynamic resizing does the same, :652-698).  np, nm and
+ * counts are host values.  Synchronous. */
+int vpic_hip_inject_p(vpic_particle_t *p, int32_t *np, int32_t max_np,
+                      vpic_particle_mover_t *pm, int32_t *nm, int32_t max_nm,
+                      const vpic_particle_injector_t *pi, int32_t n,
+                      vpic_accumulator_t *a, float q, const vpic_grid_t *g,
+                      int32_t *counts, void *stream);
 
 /* energy_p: kinetic energy of a species in FP64 (species_advance.h:119-120;
  * energy_p_pipeline.cc:18-115).  Result in *host_en.  Synchronous. */

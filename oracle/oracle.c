@@ -1136,6 +1136,133 @@ void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
   }
 }
 
+/* accumulate_rhob (rho_p.cc:126-173): the particle's charge on the 8 nodes
+ * of its voxel, FMA/FNMS written out as z + x*y and z - x*y as the file's
+ * macros define them; a node weight is doubled on each face of the local
+ * domain the voxel touches (rho_p.cc:153-165). */
+void orc_accumulate_rhob(vpic_field_t *f, const vpic_particle_t *p,
+                         float qsp, const vpic_grid_t *g) {
+  float w0 = p->dx, w1 = p->dy, w2, w3, w4, w5, w6, w7, dz = p->dz;
+  const int v = p->i, sy = g->sy, sz = g->sz;
+  int x, y, z;
+  w7 = (qsp * g->r8V) * p->w;
+  w6 = w7 - w0 * w7;
+  w7 = w7 + w0 * w7;
+  w4 = w6 - w1 * w6;
+  w5 = w7 - w1 * w7;
+  w6 = w6 + w1 * w6;
+  w7 = w7 + w1 * w7;
+  w0 = w4 - dz * w4;
+  w1 = w5 - dz * w5;
+  w2 = w6 - dz * w6;
+  w3 = w7 - dz * w7;
+  w4 = w4 + dz * w4;
+  w5 = w5 + dz * w5;
+  w6 = w6 + dz * w6;
+  w7 = w7 + dz * w7;
+  x = v;
+  z = x / sz;
+  if (z == 1) w0 += w0, w1 += w1, w2 += w2, w3 += w3;
+  if (z == g->nz) w4 += w4, w5 += w5, w6 += w6, w7 += w7;
+  x -= sz * z;
+  y = x / sy;
+  if (y == 1) w0 += w0, w1 += w1, w4 += w4, w5 += w5;
+  if (y == g->ny) w2 += w2, w3 += w3, w6 += w6, w7 += w7;
+  x -= sy * y;
+  if (x == 1) w0 += w0, w2 += w2, w4 += w4, w6 += w6;
+  if (x == g->nx) w1 += w1, w3 += w3, w5 += w5, w7 += w7;
+  f[v].rhob += w0;
+  f[v + 1].rhob += w1;
+  f[v + sy].rhob += w2;
+  f[v + sy + 1].rhob += w3;
+  f[v + sz].rhob += w4;
+  f[v + sz + 1].rhob += w5;
+  f[v + sz + sy].rhob += w6;
+  f[v + sz + sy + 1].rhob += w7;
+}
+
+/* boundary_p's mover loop for one species of a domain with no shared face
+ * (boundary_p.cc:254-375): the movers last first; the particle's voxel and
+ * face come from its index 8*voxel + face (:257-266); an absorbing face
+ * puts its charge on rhob (:268-275), a custom boundary (code <= -3,
+ * :332-346) and anything else (:348-352) are recorded in hits[m]; then the
+ * last particle moves into the mover's slot (:354-371).  counts[0] = custom
+ * hits, counts[1] = dropped.  Returns the new particle count. */
+int orc_boundary_p(vpic_particle_t *p0, int np,
+                   const vpic_particle_mover_t *pm, int nm, vpic_field_t *f,
+                   float q, const vpic_grid_t *g, vpic_boundary_hit_t *hits,
+                   int32_t *counts) {
+  counts[0] = counts[1] = 0;
+  for (int m = nm - 1; m >= 0; m--) {
+    const int i = pm[m].i;
+    int voxel = p0[i].i;
+    const int face = voxel & 7;
+    voxel >>= 3;
+    p0[i].i = voxel;
+    const int64_t nn = g->neighbor[6 * (int64_t)voxel + face];
+    int32_t code = VPIC_BOUNDARY_UNKNOWN;
+    if (nn == VPIC_BC_ABSORB_PARTICLES) {
+      orc_accumulate_rhob(f, p0 + i, q, g);
+      code = VPIC_BC_ABSORB_PARTICLES;
+    } else if (nn <= -3) {
+      code = (int32_t)nn;
+      counts[0]++;
+    } else {
+      counts[1]++;
+    }
+    if (hits) {
+      memset(hits + m, 0, sizeof(*hits));
+      hits[m].p = p0[i];
+      hits[m].m = pm[m];
+      hits[m].face = face;
+      hits[m].code = code;
+    }
+    np--;
+    p0[i] = p0[np];
+  }
+  return np;
+}
+
+/* The injection loop of boundary_p (boundary_p.cc:595-711) for one species:
+ * the injectors last first, each appended to p with a mover for its
+ * remaining displacement and moved with move_p; a full array drops it
+ * (the fixed-size branch, :652-698).  counts[0] / counts[1] = particles /
+ * movers dropped. */
+void orc_inject_p(vpic_particle_t *p, int *np, int max_np,
+                  vpic_particle_mover_t *pm, int *nm, int max_nm,
+                  const vpic_particle_injector_t *pi, int n,
+                  vpic_accumulator_t *a, float q, const vpic_grid_t *g,
+                  int32_t *counts) {
+  counts[0] = counts[1] = 0;
+  for (int k = n - 1; k >= 0; k--) {
+    const vpic_particle_injector_t *c = pi + k;
+    if (*np >= max_np) {
+      counts[0]++;
+      continue;
+    }
+    vpic_particle_t *d = p + *np;
+    d->dx = c->dx;
+    d->dy = c->dy;
+    d->dz = c->dz;
+    d->i = c->i;
+    d->ux = c->ux;
+    d->uy = c->uy;
+    d->uz = c->uz;
+    d->w = c->w;
+    const int at = (*np)++;
+    if (*nm >= max_nm) {
+      counts[1]++;
+      continue;
+    }
+    vpic_particle_mover_t *m = pm + *nm;
+    m->dispx = c->dispx;
+    m->dispy = c->dispy;
+    m->dispz = c->dispz;
+    m->i = at;
+    *nm += orc_move_p(p, m, a, g, q);
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Divergence cleaning and initialization kernels of the vacuum field table */
 /* (sfa.cc:203-210)                                                         */

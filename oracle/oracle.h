@@ -117,6 +117,23 @@ void orc_center_p(vpic_particle_t *p, int np, float q, float m,
 void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
                           float q, const vpic_grid_t *g);
 
+/* accumulate_rhob (rho_p.cc:126-173) of one particle. */
+void orc_accumulate_rhob(vpic_field_t *f, const vpic_particle_t *p,
+                         float qsp, const vpic_grid_t *g);
+
+/* boundary_p's mover loop and injection loop for one species of a domain
+ * with no shared face (boundary_p.cc:254-375, 595-711); the grid needs its
+ * neighbor table.  orc_boundary_p returns the new particle count. */
+int orc_boundary_p(vpic_particle_t *p0, int np,
+                   const vpic_particle_mover_t *pm, int nm, vpic_field_t *f,
+                   float q, const vpic_grid_t *g, vpic_boundary_hit_t *hits,
+                   int32_t *counts);
+void orc_inject_p(vpic_particle_t *p, int *np, int max_np,
+                  vpic_particle_mover_t *pm, int *nm, int max_nm,
+                  const vpic_particle_injector_t *pi, int n,
+                  vpic_accumulator_t *a, float q, const vpic_grid_t *g,
+                  int32_t *counts);
+
 /* The divergence-cleaning and initialization kernels of the vacuum field
  * table (sfa.cc:203-210), each over the whole local domain with its
  * ghosts and local adjusts:

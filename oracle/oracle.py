@@ -93,6 +93,14 @@ class Oracle:
         lib.orc_accumulate_hydro_p.argtypes = [_vp, _vp, _i32, _f32, _f32,
                                                _vp, _pg]
         lib.orc_accumulate_hydro_p.restype = None
+        lib.orc_accumulate_rhob.argtypes = [_vp, _vp, _f32, _pg]
+        lib.orc_accumulate_rhob.restype = None
+        lib.orc_boundary_p.argtypes = [_vp, _i32, _vp, _i32, _vp, _f32, _pg,
+                                       _vp, _vp]
+        lib.orc_boundary_p.restype = ctypes.c_int
+        lib.orc_inject_p.argtypes = [_vp, _vp, _i32, _vp, _vp, _i32, _vp,
+                                     _i32, _vp, _f32, _pg, _vp]
+        lib.orc_inject_p.restype = None
         lib.orc_clear_hydro.argtypes = [_vp, _pg]
         lib.orc_clear_hydro.restype = None
         lib.orc_synchronize_hydro.argtypes = [_vp, _pg]
@@ -191,6 +199,33 @@ class Oracle:
         self.lib.orc_accumulate_hydro_p(_ptr(h), _ptr(p), len(p), q, m,
                                         _ptr(fi), ctypes.byref(g))
 
+    # particle boundaries
+    def accumulate_rhob(self, f, p, q, hg):
+        """accumulate_rhob of each particle of p in index order."""
+        for k in range(len(p)):
+            self.lib.orc_accumulate_rhob(_ptr(f), _ptr(p[k:k + 1]), q,
+                                         ctypes.byref(hg.g))
+
+    def boundary_p(self, p, np_, pm, f, q, hg):
+        """boundary_p's mover loop: returns (new np, hits, counts)."""
+        hits = np.zeros(max(len(pm), 1), dtype=T.BOUNDARY_HIT)
+        counts = np.zeros(2, dtype=np.int32)
+        n = self.lib.orc_boundary_p(_ptr(p), np_, _ptr(pm), len(pm), _ptr(f),
+                                    q, ctypes.byref(hg.g), _ptr(hits),
+                                    _ptr(counts))
+        return n, hits[:len(pm)], counts
+
+    def inject_p(self, p, np_, pm, nm, pi, acc, q, hg):
+        """Injection loop: returns (np, nm, counts); p and pm are the
+        capacity."""
+        npc = np.array([np_], dtype=np.int32)
+        nmc = np.array([nm], dtype=np.int32)
+        counts = np.zeros(2, dtype=np.int32)
+        self.lib.orc_inject_p(_ptr(p), _ptr(npc), len(p), _ptr(pm), _ptr(nmc),
+                              len(pm), _ptr(pi), len(pi), _ptr(acc), q,
+                              ctypes.byref(hg.g), _ptr(counts))
+        return int(npc[0]), int(nmc[0]), counts
+
     def synchronize_hydro(self, h, g):
         self.lib.orc_synchronize_hydro(_ptr(h), ctypes.byref(g))
 
@@ -261,6 +296,8 @@ class Ref:
         lib.ref_accumulate_hydro_p.argtypes = [_vp, _vp, ctypes.c_int, _f32,
                                                _f32, _vp, _pg]
         lib.ref_accumulate_hydro_p.restype = None
+        lib.ref_accumulate_rhob.argtypes = [_vp, _vp, ctypes.c_int, _f32, _pg]
+        lib.ref_accumulate_rhob.restype = None
         for name in ("ref_rms_div_b_err", "ref_rms_div_e_err_bulk"):
             getattr(lib, name).argtypes = [_vp, _pg]
             getattr(lib, name).restype = ctypes.c_double
@@ -366,3 +403,8 @@ class Ref:
     def accumulate_hydro_p(self, h, p, q, m, fi, g):
         self.lib.ref_accumulate_hydro_p(_ptr(h), _ptr(p), len(p), q, m,
                                         _ptr(fi), ctypes.byref(g))
+
+    def accumulate_rhob(self, f, p, q, hg):
+        """The reference's accumulate_rhob of each particle in index order."""
+        self.lib.ref_accumulate_rhob(_ptr(f), _ptr(p), len(p), q,
+                                     ctypes.byref(hg.g))

@@ -11,6 +11,8 @@
 #define IN_sfa
 #include "src/field_advance/standard/sfa_private.h"
 #include "src/species_advance/species_advance.h"
+#define IN_boundary
+#include "src/boundary/boundary_private.h"
 
 #define VPIC_DROPIN_NO_TYPES
 #include "../../include/vpic_dropin.h"
@@ -556,6 +558,113 @@ static void check_absorb() {
   vpic_hip_dropin_release_all();
 }
 
+// A custom particle boundary for the check: the particle comes back as an
+// injector with its normal momentum and remaining displacement reversed.
+static int g_custom_hits = 0;
+static int reflux_handler(void *params, species_t *sp, particle_t *p,
+                          particle_mover_t *pm, particle_injector_t *pi,
+                          int max_pi, int face) {
+  g_custom_hits++;
+  if (max_pi < 1) return 0;
+  const int axis = face % 3;
+  pi->dx = p->dx;
+  pi->dy = p->dy;
+  pi->dz = p->dz;
+  pi->i = p->i;
+  pi->ux = p->ux;
+  pi->uy = p->uy;
+  pi->uz = p->uz;
+  pi->w = p->w;
+  pi->dispx = pm->dispx;
+  pi->dispy = pm->dispy;
+  pi->dispz = pm->dispz;
+  (&pi->ux)[axis] = -(&pi->ux)[axis];
+  (&pi->dispx)[axis] = -(&pi->dispx)[axis];
+  pi->sp_id = sp->id;
+  return 1;
+}
+
+// boundary_p through the drop-in after a push against absorbing x walls
+// and custom z walls, against the oracle's boundary_p, the same handler
+// on the oracle's hits, and the oracle's injection.
+static void check_boundary() {
+  Box b;
+  make_box(b, 6, 5, 4, absorb_particles);
+  const int nx = b.G.nx, ny = b.G.ny, nz = b.G.nz;
+  for (int y = 1; y <= ny; y++)
+    for (int x = 1; x <= nx; x++) {
+      b.nb[6 * (x + (nx + 2) * (y + (ny + 2) * 1)) + 2] = -3;
+      b.nb[6 * (x + (nx + 2) * (y + (ny + 2) * nz)) + 5] = -3;
+    }
+  b.G.pbc[2] = b.G.pbc[5] = -3;
+  // a face with a particle boundary has a local field boundary too
+  b.rg.bc[BOUNDARY(0, 0, -1)] = anti_symmetric_fields;
+  b.rg.bc[BOUNDARY(0, 0, 1)] = anti_symmetric_fields;
+  b.G.fbc[2] = b.G.fbc[5] = VPIC_BC_ANTI_SYMMETRIC_FIELDS;
+  Arrays A;
+  make_arrays(A, b, 61);
+  load_interpolator_array(&A.ia, &A.fa);
+  int rc = vpic_hip_install_field_kernels(&A.fa);
+  Species e;
+  make_species(e, b, -1.0f, 1.0f, 7, 0.6f, 71);
+  e.sp.id = 0;
+  clear_accumulator_array(&A.aa);
+  advance_p(&e.sp, &A.aa, &A.ia);
+  const int np0 = e.sp.np, nm0 = e.sp.nm;
+  std::vector<particle_t> p0(e.p);
+  std::vector<particle_mover_t> pm0(e.pm);
+  std::vector<field_t> f0(A.f);
+  std::vector<accumulator_t> a0(A.acc);
+  particle_bc_t pbc;
+  memset(&pbc, 0, sizeof pbc);
+  pbc.interact = (particle_bc_func_t)reflux_handler;
+  pbc.id = -3;
+  g_custom_hits = 0;
+  boundary_p(&pbc, &e.sp, &A.fa, &A.aa);
+  const int dev_hits = g_custom_hits;
+  // the oracle, with the same handler on its hits
+  std::vector<vpic_boundary_hit_t> hits(nm0 > 0 ? nm0 : 1);
+  int32_t counts[2];
+  int np = orc_boundary_p((vpic_particle_t *)p0.data(), np0,
+                          (const vpic_particle_mover_t *)pm0.data(), nm0,
+                          (vpic_field_t *)f0.data(), e.sp.q, &b.G,
+                          hits.data(), counts);
+  std::vector<particle_injector_t> ci;
+  g_custom_hits = 0;
+  for (int m = nm0 - 1; m >= 0; m--) {
+    if (hits[m].code != -3) continue;
+    particle_injector_t c;
+    if (reflux_handler(nullptr, &e.sp, (particle_t *)&hits[m].p,
+                       (particle_mover_t *)&hits[m].m, &c, 1, hits[m].face))
+      ci.push_back(c);
+  }
+  int nm = 0;
+  int32_t ic[2];
+  orc_inject_p((vpic_particle_t *)p0.data(), &np, e.sp.max_np,
+               (vpic_particle_mover_t *)pm0.data(), &nm, e.sp.max_nm,
+               (const vpic_particle_injector_t *)ci.data(), (int)ci.size(),
+               (vpic_accumulator_t *)a0.data(), e.sp.q, &b.G, ic);
+  bool okn = e.sp.np == np && e.sp.nm == nm && dev_hits == g_custom_hits;
+  bool okp = okn && same(e.p.data(), p0.data(), sizeof(particle_t) * np);
+  bool okm = okn && same(e.pm.data(), pm0.data(),
+                         sizeof(particle_mover_t) * nm);
+  double num = 0, den = 0;
+  for (int v = 0; v < b.G.nv; v++) {
+    num = fmax(num, fabs((double)A.f[v].rhob - f0[v].rhob));
+    den = fmax(den, fabs((double)f0[v].rhob));
+  }
+  const double ea = acc_err(A.acc.data(), (const vpic_accumulator_t *)a0.data(),
+                            b.G.nv);
+  report(rc == 0 && nm0 > 0 && g_custom_hits > 0 && okp && okm &&
+             num <= 1e-5 * den && ea <= 1e-5,
+         "boundary_p (absorbing x walls, custom z walls, injection)",
+         "movers %d, np %d vs %d, nm %d vs %d, handler calls %d vs %d, "
+         "particles %s, movers %s, rhob err %.3g of %.3g, acc err %.3g",
+         nm0, e.sp.np, np, e.sp.nm, nm, dev_hits, g_custom_hits,
+         okp ? "same" : "DIFFER", okm ? "same" : "DIFFER", num, den, ea);
+  vpic_hip_dropin_release_all();
+}
+
 // Resident mode: several steps with no transfers in between; the state is
 // read back after each step (a download does not disturb the device copy)
 // and compared with the reference stepping in lockstep.
@@ -635,6 +744,7 @@ int main(int argc, char **argv) {
   check_field_kernels(0, "field kernels via fa->kernel (periodic)");
   check_field_kernels(absorb_particles, "field kernels via fa->kernel (pec x)");
   check_absorb();
+  check_boundary();
   check_resident();
   check_errors();
   vpic_hip_dropin_release_all();

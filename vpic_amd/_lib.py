@@ -62,6 +62,10 @@ _SIGS = {
     "vpic_hip_accumulate_hydro_p": [_vp, _vp, _i32, _f32, _f32, _vp, _pg, _vp,
                                     _vp],
     "vpic_hip_synchronize_hydro_array": [_vp, _pg, _vp],
+    "vpic_hip_boundary_p": [_vp, _pi32, _vp, _i32, _vp, _f32, _pg, _vp,
+                            _pi32, _vp],
+    "vpic_hip_inject_p": [_vp, _pi32, _i32, _vp, _pi32, _i32, _vp, _i32, _vp,
+                          _f32, _pg, _pi32, _vp],
     "vpic_hip_grid_periodic": [_pg, _f64, _f64, _f64, _f64, _f64, _f64, _i32,
                                _i32, _i32, _f32, _f32, _f32],
     "vpic_hip_field_params_vacuum": [_pmp, _f32],

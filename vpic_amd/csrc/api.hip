@@ -274,6 +274,44 @@ extern "C" int vpic_hip_move_p(vpic_particle_t *p, vpic_particle_mover_t *pm,
   return launch_move_p_list(p, pm, nm, a, g, q, dev_ret, S(stream));
 }
 
+extern "C" int vpic_hip_boundary_p(vpic_particle_t *p, int32_t *np,
+                                   const vpic_particle_mover_t *pm, int32_t nm,
+                                   vpic_field_t *f, float q,
+                                   const vpic_grid_t *g,
+                                   vpic_boundary_hit_t *hits, int32_t *counts,
+                                   void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(np && counts, "NULL argument");
+  CHECK(*np >= 0 && nm >= 0, "np < 0 or nm < 0");
+  CHECK(nm <= *np, "more movers than particles");
+  counts[0] = counts[1] = 0;
+  if (nm == 0) return VPIC_OK;
+  CHECK(p && pm && f, "NULL argument");
+  CHECK(aligned16(p), "particle array must be 16-byte aligned");
+  return launch_boundary_p(p, np, pm, nm, f, q, g, hits, counts, S(stream));
+}
+
+extern "C" int vpic_hip_inject_p(vpic_particle_t *p, int32_t *np,
+                                 int32_t max_np, vpic_particle_mover_t *pm,
+                                 int32_t *nm, int32_t max_nm,
+                                 const vpic_particle_injector_t *pi, int32_t n,
+                                 vpic_accumulator_t *a, float q,
+                                 const vpic_grid_t *g, int32_t *counts,
+                                 void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(np && nm && counts, "NULL argument");
+  CHECK(*np >= 0 && *nm >= 0 && n >= 0, "negative count");
+  CHECK(*np <= max_np && *nm <= max_nm, "count above its capacity");
+  counts[0] = counts[1] = 0;
+  if (n == 0) return VPIC_OK;
+  CHECK(p && pm && pi && a, "NULL argument");
+  CHECK(aligned16(p), "particle array must be 16-byte aligned");
+  return launch_inject_p(p, np, max_np, pm, nm, max_nm, pi, n, a, q, g,
+                         counts, S(stream));
+}
+
 extern "C" int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux,
                                int32_t np, int32_t *partition,
                                const vpic_grid_t *g, int in_place,

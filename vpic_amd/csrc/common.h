@@ -161,6 +161,16 @@ int device_exclusive_scan(const int *in, int *out, int64_t n, int *sums,
 int launch_move_p_list(vpic_particle_t *p, vpic_particle_mover_t *pm, int nm,
                        vpic_accumulator_t *a, const vpic_grid_t *g, float q,
                        int *dev_ret, hipStream_t s);
+int launch_boundary_p(vpic_particle_t *p, int *np_host,
+                      const vpic_particle_mover_t *pm, int nm,
+                      vpic_field_t *f, float q, const vpic_grid_t *g,
+                      vpic_boundary_hit_t *hits, int *counts_host,
+                      hipStream_t s);
+int launch_inject_p(vpic_particle_t *p, int *np, int max_np,
+                    vpic_particle_mover_t *pm, int *nm, int max_nm,
+                    const vpic_particle_injector_t *pi, int n,
+                    vpic_accumulator_t *a, float q, const vpic_grid_t *g,
+                    int *counts, hipStream_t s);
 int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
                   int *partition, const vpic_grid_t *g, int in_place,
                   hipStream_t s);

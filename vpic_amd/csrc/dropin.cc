@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <vector>
 
 #include "../../include/vpic_dropin.h"
 #include "../../include/vpic_hip.h"
@@ -27,6 +28,8 @@ static_assert(sizeof(accumulator_t) == sizeof(vpic_accumulator_t),
               "accumulator_t");
 static_assert(sizeof(field_t) == sizeof(vpic_field_t), "field_t");
 static_assert(sizeof(hydro_t) == sizeof(vpic_hydro_t), "hydro_t");
+static_assert(sizeof(particle_injector_t) == sizeof(vpic_particle_injector_t),
+              "particle_injector_t");
 
 namespace {
 
@@ -55,6 +58,8 @@ struct State {
   void *aux = nullptr;  // sort_p scratch
   size_t aux_cap = 0;
   void *one = nullptr;  // move_p: {particle, mover, ret} staging
+  void *stage = nullptr;  // boundary_p: hits / injectors
+  size_t stage_cap = 0;
 };
 
 State &st() {
@@ -94,6 +99,16 @@ void warn(const char *fn, const char *fmt, ...) {
     hipError_t e_ = (call);                                              \
     if (e_ != hipSuccess)                                                \
       return fail(fn, "%s: %s", #call, hipGetErrorString(e_));          \
+  } while (0)
+
+// HIPOK for functions returning void.
+#define HIPOK_V(fn, call)                                                \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      fail(fn, "%s: %s", #call, hipGetErrorString(e_));                  \
+      return;                                                            \
+    }                                                                    \
   } while (0)
 
 #define ABI(fn, call)                                                    \
@@ -553,6 +568,170 @@ extern "C" void accumulate_hydro_p(hydro_array_t *ha, const species_t *sp,
     return;
   }
   dev_wrote(fn, ha->h, hbytes(g));
+}
+
+// Device staging buffer of at least `bytes`.
+static bool stage(const char *fn, size_t bytes, void **out) {
+  State &S = st();
+  if (S.stage_cap < bytes) {
+    if (S.stage) HIPOK(fn, hipFree(S.stage));
+    S.stage = nullptr;
+    S.stage_cap = 0;
+    HIPOK(fn, hipMalloc(&S.stage, bytes));
+    S.stage_cap = bytes;
+  }
+  *out = S.stage;
+  return true;
+}
+
+extern "C" void boundary_p(particle_bc_t *pbc_list, species_t *sp_list,
+                           field_array_t *fa, accumulator_array_t *aa) {
+  const char *fn = "boundary_p";
+  if (!sp_list) return;  // boundary_p.cc:98-101: nothing to do
+  if (!fa || !aa || !fa->f || !aa->a || sp_list->g != aa->g ||
+      fa->g != aa->g) {
+    fail(fn, "Bad args.");
+    return;
+  }
+  // boundary_p.cc:111-127: the custom boundaries by code (id = -3 - k)
+  enum { MAX_PBC = 32 };
+  particle_bc_t *pbc_of[MAX_PBC] = {nullptr};
+  int nb = 0;
+  for (particle_bc_t *b = pbc_list; b; b = b->next) nb++;
+  if (nb > MAX_PBC) {
+    fail(fn, "Update this to support more particle boundary conditions.");
+    return;
+  }
+  for (particle_bc_t *b = pbc_list; b; b = b->next) {
+    const int64_t k = -b->id - 3;
+    if (k < 0 || k >= MAX_PBC) {
+      fail(fn, "particle boundary id %lld is not a custom boundary id",
+           (long long)b->id);
+      return;
+    }
+    pbc_of[k] = b;
+  }
+  if (!ensure_init(fn)) return;
+  const grid_t *g = fa->g;
+  vpic_grid_t G;
+  if (!make_grid(fn, g, &G)) return;
+  vpic_field_t *f;
+  if (!dev_in(fn, fa->f, fbytes(g), fbytes(g), &f)) return;
+  std::vector<particle_injector_t> ci;
+  std::vector<vpic_boundary_hit_t> hits;
+  bool handled = false, f_on_host = false;
+  for (species_t *sp = sp_list; sp; sp = sp->next) {
+    if (!species_ok(fn, sp) || sp->g != g) {
+      fail(fn, "Bad args.");
+      return;
+    }
+    const int nm = sp->nm;
+    if (nm <= 0) {
+      sp->nm = 0;
+      continue;
+    }
+    vpic_particle_t *p;
+    vpic_particle_mover_t *pm;
+    void *dh;
+    if (!dev_in(fn, sp->p, pbytes(sp->max_np), pbytes(sp->np), &p) ||
+        !dev_in(fn, sp->pm, (size_t)sp->max_nm * sizeof(particle_mover_t),
+                (size_t)nm * sizeof(particle_mover_t), &pm) ||
+        !stage(fn, (size_t)nm * sizeof(vpic_boundary_hit_t), &dh))
+      return;
+    int32_t np = sp->np, counts[2] = {0, 0};
+    if (vpic_hip_boundary_p(p, &np, pm, nm, f, sp->q, &G,
+                            (vpic_boundary_hit_t *)dh, counts, nullptr) != 0) {
+      fail(fn, "%s", vpic_hip_last_error());
+      return;
+    }
+    sp->np = np;
+    sp->nm = 0;
+    if (!dev_wrote(fn, sp->p, pbytes(np))) return;
+    // boundary_p.cc:348-352, once per species with the count
+    if (counts[1])
+      warn(fn, "Unknown boundary interaction ... dropping %d particles "
+               "(species=%s)", counts[1], sp->name ? sp->name : "");
+    if (!counts[0]) continue;
+    // custom boundaries: the handlers run on the host, last mover first,
+    // with the host field array current (they may deposit into rhob)
+    hits.resize(nm);
+    HIPOK_V(fn, hipMemcpy(hits.data(), dh, (size_t)nm * sizeof(hits[0]),
+                          hipMemcpyDeviceToHost));
+    if (!f_on_host) {
+      if (!download(fn, fa->f, fbytes(g))) return;
+      f_on_host = true;
+    }
+    for (int m = nm - 1; m >= 0; m--) {
+      vpic_boundary_hit_t &h = hits[m];
+      if (h.code > -3) continue;
+      const int64_t k = -(int64_t)h.code - 3;
+      particle_bc_t *b = k < MAX_PBC ? pbc_of[k] : nullptr;
+      if (!b || !b->interact) {
+        warn(fn, "Unknown boundary interaction ... dropping particle "
+                 "(species=%s)", sp->name ? sp->name : "");
+        continue;
+      }
+      ci.resize(ci.size() + 1);
+      const int n = b->interact(b->params, sp, (particle_t *)&h.p,
+                                (particle_mover_t *)&h.m,
+                                &ci[ci.size() - 1], 1, h.face);
+      ci.resize(ci.size() - 1 + (n > 0 ? 1 : 0));
+      handled = true;
+    }
+  }
+  if (f_on_host) {
+    // the handlers' rhob deposits (and the absorbed charges) go back
+    if (!upload(fn, fa->f, fbytes(g), fbytes(g))) return;
+  } else if (!dev_wrote(fn, fa->f, fbytes(g))) {
+    return;
+  }
+  (void)handled;
+  if (ci.empty()) return;
+  // boundary_p.cc:595-711: the injectors last first, per species in the
+  // order the handlers produced them; move_p deposits into aa's copy 0
+  vpic_accumulator_t *a;
+  if (!dev_in(fn, aa->a, abytes(g), abytes(g), &a)) return;
+  for (species_t *sp = sp_list; sp; sp = sp->next) {
+    std::vector<particle_injector_t> mine;
+    for (const particle_injector_t &c : ci)
+      if (c.sp_id == sp->id) mine.push_back(c);
+    if (mine.empty()) continue;
+    vpic_particle_t *p;
+    vpic_particle_mover_t *pm;
+    void *dpi;
+    const size_t pib = mine.size() * sizeof(particle_injector_t);
+    if (!dev_in(fn, sp->p, pbytes(sp->max_np), pbytes(sp->np), &p) ||
+        !dev_in(fn, sp->pm, (size_t)sp->max_nm * sizeof(particle_mover_t),
+                (size_t)sp->nm * sizeof(particle_mover_t), &pm) ||
+        !stage(fn, pib, &dpi))
+      return;
+    HIPOK_V(fn, hipMemcpy(dpi, mine.data(), pib, hipMemcpyHostToDevice));
+    int32_t np = sp->np, nm = sp->nm, counts[2] = {0, 0};
+    if (vpic_hip_inject_p(p, &np, sp->max_np, pm, &nm, sp->max_nm,
+                          (const vpic_particle_injector_t *)dpi,
+                          (int32_t)mine.size(), a, sp->q, &G, counts,
+                          nullptr) != 0) {
+      fail(fn, "%s", vpic_hip_last_error());
+      return;
+    }
+    // boundary_p.cc:716-738 (the fixed-size build's warnings)
+    if (counts[0])
+      warn(fn, "Dropped %i particles from species \"%s\".  Use a larger "
+               "local particle allocation in your simulation setup for this "
+               "species on this node.", counts[0], sp->name ? sp->name : "");
+    if (counts[1])
+      warn(fn, "%i particles were not completed moved to their final "
+               "location this timestep for species \"%s\".  Use a larger "
+               "local particle mover buffer in your simulation setup for "
+               "this species on this node.", counts[1],
+           sp->name ? sp->name : "");
+    sp->np = np;
+    sp->nm = nm;
+    if (!dev_wrote(fn, sp->p, pbytes(np)) ||
+        !dev_wrote(fn, sp->pm, (size_t)nm * sizeof(particle_mover_t)))
+      return;
+  }
+  dev_wrote(fn, aa->a, abytes(g));
 }
 
 // ---------------------------------------------------------------------------

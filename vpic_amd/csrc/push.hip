@@ -1537,6 +1537,50 @@ __global__ void k_move_p_serial(vpic_particle_t *p, vpic_particle_mover_t *pm,
   }
 }
 
+// The injection loop of boundary_p (boundary_p.cc:595-711) for one species,
+// on one lane: the injectors last first, each appended to p with a mover
+// for its remaining displacement and moved by move_p (depositing into acc);
+// the mover stays in use when move_p returns 1.  st = {np, nm, particles
+// dropped, movers dropped, injectors with a voxel off the grid}.
+__global__ void k_inject_p(vpic_particle_t *p, int max_np,
+                           vpic_particle_mover_t *pm, int max_nm,
+                           const vpic_particle_injector_t *pi, int n,
+                           float *acc, GridDev G, float q, int *st) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int np = st[0], nm = st[1];
+  const GlobalSink sink{acc};
+  for (int k = n - 1; k >= 0; k--) {
+    const vpic_particle_injector_t c = pi[k];
+    if (np >= max_np) {
+      st[2]++;
+      continue;
+    }
+    // move_p deposits into the injector's voxel and looks up its faces
+    if (c.i < 0 || c.i + G.sz + G.sy + 1 >= G.nv) {
+      st[4]++;
+      continue;
+    }
+    float4 *d = reinterpret_cast<float4 *>(p + np);
+    d[0] = make_float4(c.dx, c.dy, c.dz, __int_as_float(c.i));
+    d[1] = make_float4(c.ux, c.uy, c.uz, c.w);
+    const int at = np++;
+    if (nm >= max_nm) {
+      st[3]++;
+      continue;
+    }
+    vpic_particle_mover_t mv;
+    mv.dispx = c.dispx;
+    mv.dispy = c.dispy;
+    mv.dispz = c.dispz;
+    mv.i = at;
+    const int r = dev_move_p(p, mv, sink, G, q);
+    pm[nm] = mv;
+    nm += r;
+  }
+  st[0] = np;
+  st[1] = nm;
+}
+
 // Scratch of the push: counters, crosser list, returned-mover list and the
 // ordering buffer, one set per device.
 // Tile lists of one particle array (keyed by its address).
@@ -1549,7 +1593,7 @@ struct BinState {
 };
 
 struct PushScratch {
-  DeviceScratch counters, xlist, rlist, sortbuf, wbuf, trash;
+  DeviceScratch counters, xlist, rlist, sortbuf, wbuf, trash, inject;
   BinState bins[8];
   int64_t bin_clock = 0;
 };
@@ -1836,6 +1880,36 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
   hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
                      xcap, (int2 *)S.sortbuf.ptr, pm, max_nm, cnt, dev_nm);
   VPIC_LAUNCHED("k_order_movers");
+  return VPIC_OK;
+}
+
+int launch_inject_p(vpic_particle_t *p, int *np, int max_np,
+                    vpic_particle_mover_t *pm, int *nm, int max_nm,
+                    const vpic_particle_injector_t *pi, int n,
+                    vpic_accumulator_t *a, float q, const vpic_grid_t *g,
+                    int *counts, hipStream_t s) {
+  counts[0] = counts[1] = 0;
+  if (n <= 0) return VPIC_OK;
+  PushScratch &S = push_scratch();
+  int rc;
+  if ((rc = scratch_reserve(S.inject, 64))) return rc;
+  int st[8] = {*np, *nm, 0, 0, 0, 0, 0, 0};
+  int *dst = (int *)S.inject.ptr;
+  VPIC_HIP(hipMemcpyAsync(dst, st, sizeof st, hipMemcpyHostToDevice, s));
+  const GridDev G = make_grid_dev(g);
+  hipLaunchKernelGGL(k_inject_p, dim3(1), dim3(64), 0, s, p, max_np, pm,
+                     max_nm, pi, n, (float *)a, G, q, dst);
+  VPIC_LAUNCHED("k_inject_p");
+  VPIC_HIP(hipMemcpyAsync(st, dst, sizeof st, hipMemcpyDeviceToHost, s));
+  VPIC_HIP(hipStreamSynchronize(s));
+  *np = st[0];
+  *nm = st[1];
+  counts[0] = st[2];
+  counts[1] = st[3];
+  if (st[4])
+    return vpic_set_error(VPIC_EINVAL,
+                          "inject_p: %d injectors with a voxel index off the "
+                          "grid were skipped", st[4]);
   return VPIC_OK;
 }
 

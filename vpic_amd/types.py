@@ -70,6 +70,14 @@ class Grid(ctypes.Structure):
         return x + (self.nx + 2) * (y + (self.ny + 2) * z)
 
 
+INJECTOR = np.dtype([("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"),
+                     ("i", "<i4"), ("ux", "<f4"), ("uy", "<f4"), ("uz", "<f4"),
+                     ("w", "<f4"), ("dispx", "<f4"), ("dispy", "<f4"),
+                     ("dispz", "<f4"), ("sp_id", "<i4")])
+BOUNDARY_HIT = np.dtype([("p", PARTICLE), ("m", MOVER), ("face", "<i4"),
+                         ("code", "<i4"), ("pad", "<i4", (2,))])
+BOUNDARY_UNKNOWN = 1
+
 HYDRO_FIELDS = ("jx", "jy", "jz", "rho", "px", "py", "pz", "ke", "txx",
                 "tyy", "tzz", "tyz", "tzx", "txy")
 HYDRO = np.dtype([(n, np.float32) for n in HYDRO_FIELDS] +
