@@ -5,8 +5,9 @@
  *
  * A VPIC build uses them by linking libvpichip.so in place of the
  * translation units that define these functions (advance_p.cc, sort_p.cc,
- * move_p.cc, energy_p.cc, rho_p.cc, interpolator_array.cc's
- * load_interpolator_array, clear_array.cc, reduce_array.cc,
+ * move_p.cc, energy_p.cc, rho_p.cc, hydro_p.cc, boundary_p.cc,
+ * interpolator_array.cc's load_interpolator_array, hydro_array.cc's
+ * synchronize_hydro_array, clear_array.cc, reduce_array.cc,
  * unload_accumulator.cc), and by calling vpic_hip_install_field_kernels()
  * on each field array after new_standard_field_array().  Such a build keeps
  * including its own headers; this header restates the same types for
