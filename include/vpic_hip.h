@@ -457,6 +457,56 @@ int vpic_hip_synchronize_hydro_array(vpic_hydro_t *h, const vpic_grid_t *g,
                                      void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Output formats (src/vpic/dump.cc, src/vpic/dumpmacros.h), written from  */
+/* host arrays: the caller downloads what it dumps                          */
+/* ------------------------------------------------------------------------ */
+
+/* What the version-0 header takes besides the grid (dumpmacros.h:7-43):
+ * the step, the local domain's low corner (grid_t.x0, y0, z0) and the
+ * rank / rank count. */
+typedef struct vpic_dump_info {
+  int64_t step;
+  float x0, y0, z0;
+  int32_t rank, nproc;
+} vpic_dump_info_t;
+
+/* dump_energies (dump.cc:38-77): one line "step ex ey ez bx by bz KE..."
+ * (%li, then %e), the field energies en_f as energy_f returns them and one
+ * energy_p per species; a new file (append = 0) first gets the layout
+ * comment with the species names and the time step. */
+int vpic_hip_write_energies(const char *fname, int append,
+                            const vpic_dump_info_t *info, float dt,
+                            const double en_f[6], const double *en_p,
+                            const char *const *sp_names, int32_t nsp);
+
+/* dump_fields (dump.cc:181-219): header (dump type 1), then the
+ * (nx+2)(ny+2)(nz+2) field_t array f. */
+int vpic_hip_write_fields(const char *fname, const vpic_field_t *f,
+                          const vpic_grid_t *g, const vpic_dump_info_t *info);
+
+/* dump_hydro (dump.cc:221-270): header (type 2, species id, q/m), then the
+ * hydro array h (as synchronize_hydro_array leaves it). */
+int vpic_hip_write_hydro(const char *fname, const vpic_hydro_t *h,
+                         const vpic_grid_t *g, const vpic_dump_info_t *info,
+                         int32_t sp_id, float q_m);
+
+/* dump_particles (dump.cc:273-335): header (type 3, species id, q/m) and an
+ * array header of np_total particles, then the n particles of p; the rest
+ * (np_total - n, already time-centred with center_p as dump_particles
+ * does) is appended to the file by the caller. */
+int vpic_hip_write_particles(const char *fname, const vpic_particle_t *p,
+                             int32_t n, int32_t np_total,
+                             const vpic_grid_t *g,
+                             const vpic_dump_info_t *info, int32_t sp_id,
+                             float q_m);
+
+/* dump_grid (dump.cc:138-179): header (type 0), then grid_t.bc[27],
+ * range[nproc+1] and the 6*nv neighbor table. */
+int vpic_hip_write_grid(const char *fname, const vpic_grid_t *g,
+                        const vpic_dump_info_t *info, const int32_t bc[27],
+                        const int64_t *range, const int64_t *neighbor);
+
+/* ------------------------------------------------------------------------ */
 /* Set-up helpers (no reference counterpart on the device; they fill the   */
 /* same arrays the reference's set-up code fills on the host)               */
 /* ------------------------------------------------------------------------ */

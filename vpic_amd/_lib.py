@@ -21,6 +21,7 @@ _pmp = ctypes.POINTER(T.FieldParams)
 _pi32 = ctypes.POINTER(ctypes.c_int32)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _pf64 = ctypes.POINTER(ctypes.c_double)
+_pdi = ctypes.POINTER(T.DumpInfo)
 
 # name -> argtypes (all return int status)
 _SIGS = {
@@ -66,6 +67,13 @@ _SIGS = {
                             _pi32, _vp],
     "vpic_hip_inject_p": [_vp, _pi32, _i32, _vp, _pi32, _i32, _vp, _i32, _vp,
                           _f32, _pg, _pi32, _vp],
+    "vpic_hip_write_energies": [ctypes.c_char_p, ctypes.c_int, _pdi, _f32,
+                                _vp, _vp, _vp, _i32],
+    "vpic_hip_write_fields": [ctypes.c_char_p, _vp, _pg, _pdi],
+    "vpic_hip_write_hydro": [ctypes.c_char_p, _vp, _pg, _pdi, _i32, _f32],
+    "vpic_hip_write_particles": [ctypes.c_char_p, _vp, _i32, _i32, _pg, _pdi,
+                                 _i32, _f32],
+    "vpic_hip_write_grid": [ctypes.c_char_p, _pg, _pdi, _vp, _vp, _vp],
     "vpic_hip_grid_periodic": [_pg, _f64, _f64, _f64, _f64, _f64, _f64, _i32,
                                _i32, _i32, _f32, _f32, _f32],
     "vpic_hip_field_params_vacuum": [_pmp, _f32],

@@ -3,8 +3,9 @@ vpic_simulation::advance() (src/vpic/advance.cc:15-207) restricted to the
 hot path: sort_p, clear_accumulator_array, advance_p per species,
 reduce_accumulator_array, clear_jf / unload_accumulator_array /
 synchronize_jf, advance_b(0.5), advance_e(1), advance_b(0.5),
-load_interpolator_array.  Collisions, emitters, boundary_p's inter-domain
-exchange, divergence cleaning and dumps are not part of the path.
+load_interpolator_array.  Collisions, emitters and boundary_p's
+inter-domain exchange are not part of the path.  The dump_* methods write
+the reference's output formats (src/vpic/dump.cc) from the device arrays.
 
 All arrays live in HBM (vpic_amd.device); every call is stream-ordered on
 one HIP stream, and the only per-step host synchronisation is the sort's.
@@ -43,8 +44,10 @@ class Species:
 class Simulation:
     """The per-step hot path on one GPU."""
 
-    def __init__(self, grid, field_params=None):
+    def __init__(self, grid, field_params=None, origin=(0.0, 0.0, 0.0)):
         self.g = grid
+        self.origin = tuple(float(v) for v in origin)  # grid_t.x0, y0, z0
+        self.hydro = None  # allocated by the first dump_hydro
         self.mp = field_params or T.vacuum_params()
         self.fields = device.fields(grid)
         self.fields.zero()
@@ -129,6 +132,89 @@ class Simulation:
         v = self.dev_nm.download()
         return int(v[0]), int(v[1]), int(v[2])
 
+    # -- output (src/vpic/dump.cc) -----------------------------------------
+    def _info(self):
+        return T.DumpInfo(self.step, self.origin[0], self.origin[1],
+                          self.origin[2], 0, 1)
+
+    def _fname(self, fbase, ftag):
+        # dump.cc:192-193: fbase.step.rank with a tag, else fbase.rank
+        return ("%s.%d.%d" % (fbase, self.step, 0) if ftag
+                else "%s.%d" % (fbase, 0))
+
+    def dump_energies(self, fname, append=1):
+        """dump_energies (dump.cc:38-77): energy_f and energy_p of every
+        species, one text line per call."""
+        en_f = (ctypes.c_double * 6)(*self.energy_f())
+        nsp = len(self.species)
+        en_p = (ctypes.c_double * max(nsp, 1))(
+            *[self.energy_p(sp) for sp in self.species])
+        names = (ctypes.c_char_p * max(nsp, 1))(
+            *[sp.name.encode() for sp in self.species])
+        _lib.call("vpic_hip_write_energies", fname.encode(), int(append),
+                  ctypes.byref(self._info()), self.g.dt, en_f, en_p, names,
+                  nsp)
+
+    def dump_fields(self, fbase, ftag=1):
+        """dump_fields (dump.cc:181-219)."""
+        f = self.fields.download()
+        _lib.call("vpic_hip_write_fields", self._fname(fbase, ftag).encode(),
+                  f.ctypes.data, self._G(), ctypes.byref(self._info()))
+
+    def dump_hydro(self, sp, fbase, ftag=1):
+        """dump_hydro (dump.cc:221-270): clear_hydro_array,
+        accumulate_hydro_p, synchronize_hydro_array, then the file."""
+        if self.hydro is None:
+            self.hydro = device.DeviceArray((self.g.nv,), T.HYDRO)
+        part = sp.partition.ptr if sp.last_sorted >= 0 else None
+        _lib.call("vpic_hip_clear_hydro_array", self.hydro.ptr, self._G(),
+                  self.stream)
+        _lib.call("vpic_hip_accumulate_hydro_p", self.hydro.ptr, sp.p.ptr,
+                  sp.np, sp.q, sp.m, self.interp.ptr, self._G(), part,
+                  self.stream)
+        _lib.call("vpic_hip_synchronize_hydro_array", self.hydro.ptr,
+                  self._G(), self.stream)
+        h = self.hydro.download()
+        _lib.call("vpic_hip_write_hydro", self._fname(fbase, ftag).encode(),
+                  h.ctypes.data, self._G(), ctypes.byref(self._info()),
+                  sp_id(self, sp), np.float32(sp.q / sp.m))
+
+    def dump_particles(self, sp, fbase, ftag=1, chunk=1 << 24):
+        """dump_particles (dump.cc:273-335): the particles time-centred
+        with center_p (on a device copy, chunk by chunk; the species is
+        unchanged)."""
+        fname = self._fname(fbase, ftag)
+        _lib.call("vpic_hip_write_particles", fname.encode(), None, 0, sp.np,
+                  self._G(), ctypes.byref(self._info()), sp_id(self, sp),
+                  np.float32(sp.q / sp.m))
+        buf = device.particles(min(max(sp.np, 1), chunk))
+        with open(fname, "ab") as fh:
+            for s0 in range(0, sp.np, chunk):
+                n = min(chunk, sp.np - s0)
+                nb = n * T.PARTICLE.itemsize
+                buf.t[:nb].copy_(sp.p.t[s0 * T.PARTICLE.itemsize:
+                                        s0 * T.PARTICLE.itemsize + nb])
+                _lib.call("vpic_hip_center_p", buf.ptr, n, sp.q, sp.m,
+                          self.interp.ptr, self._G(), 0, self.stream)
+                torch.cuda.synchronize()
+                fh.write(buf.t[:nb].cpu().numpy().tobytes())
+
+    def dump_grid(self, fbase):
+        """dump_grid (dump.cc:138-179) of this one-rank domain: bc[27] (this
+        rank, or the face's field boundary code), range {0, nv}, and the
+        neighbor table."""
+        bc = np.zeros(27, dtype=np.int32)
+        for k, (x, y, z) in enumerate(((-1, 0, 0), (0, -1, 0), (0, 0, -1),
+                                       (1, 0, 0), (0, 1, 0), (0, 0, 1))):
+            if self.g.fbc[k] != T.BC_PERIODIC:
+                bc[13 + x + 3 * y + 9 * z] = self.g.fbc[k]
+        rng = np.array([0, self.g.nv], dtype=np.int64)
+        nb = np.zeros(6 * self.g.nv, dtype=np.int64)
+        _lib.call("vpic_hip_grid_neighbor_table", self._G(), nb.ctypes.data)
+        _lib.call("vpic_hip_write_grid", ("%s.%d" % (fbase, 0)).encode(),
+                  self._G(), ctypes.byref(self._info()), bc.ctypes.data,
+                  rng.ctypes.data, nb.ctypes.data)
+
     # -- one time step (advance.cc:15-207) ----------------------------------
     def advance(self, timer=None):
         t = timer or (lambda name: _NullCtx())
@@ -149,6 +235,11 @@ class Simulation:
         with t("interp"):
             self.load_interpolator_array()
         self.step += 1
+
+
+def sp_id(sim, sp):
+    """species_t.id: the species' position in the simulation's list."""
+    return sim.species.index(sp)
 
 
 class _NullCtx:

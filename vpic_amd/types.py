@@ -84,6 +84,13 @@ HYDRO = np.dtype([(n, np.float32) for n in HYDRO_FIELDS] +
                  [("pad0", np.float32), ("pad1", np.float32)])
 
 
+class DumpInfo(ctypes.Structure):
+    """vpic_dump_info_t: step, the domain's low corner, rank / nproc."""
+    _fields_ = [("step", ctypes.c_int64), ("x0", ctypes.c_float),
+                ("y0", ctypes.c_float), ("z0", ctypes.c_float),
+                ("rank", ctypes.c_int32), ("nproc", ctypes.c_int32)]
+
+
 class FieldParams(ctypes.Structure):
     """vpic_field_params_t."""
     _fields_ = [(n, ctypes.c_float) for n in
