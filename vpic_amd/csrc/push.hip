@@ -101,6 +101,16 @@ __device__ __forceinline__ long long to_fixed(double y) {
 // on the order of the additions.  A value too large for the fixed-point
 // range, and any cell outside the window, goes to the global accumulator
 // with float atomics.
+// Out-of-window deposits of the tiled push, by destination tile: record
+// k of tile t is rec[4 * (t * cap + k)] .. +3 = {voxel (bits), j0..j2},
+// {j3..j6}, {j7..j10}, {j11, 0, 0, 0}; k_merge_windows adds them.  A full
+// bucket falls back to float atomics into the accumulator.
+struct OowBuckets {
+  int *cnt;     // records per tile (may exceed cap)
+  float4 *rec;  // cap records per tile
+  int cap;
+};
+
 struct WindowSink {
   lds_u64 *lds;  // 12 * ncell
   int ncell;
@@ -111,6 +121,8 @@ struct WindowSink {
   int sx1, sy1;     // nx+2, ny+2
   unsigned mx, my;  // fast_div multipliers for sx1, sy1 (0: divide)
   int *oow;         // LDS count of deposits outside the window
+  OowBuckets ob;    // where those deposits go
+  int tx, ty, tz, ntx, nty;  // tile extent and tiles per axis (x, y)
   __device__ __forceinline__ int slot(int v) const {
     unsigned t, z;
     if (mx) {
@@ -162,8 +174,47 @@ struct WindowSink {
       }
     } else {
       atomicAdd(oow, 1);
+      // to the bucket of the tile holding the voxel (one integer atomic and
+      // a 64-B store instead of 12 float atomics, which this part performs
+      // beyond the L2)
+      unsigned t1, zz;
+      if (mx) {
+        t1 = fast_div((unsigned)voxel, mx);
+        zz = fast_div(t1, my);
+      } else {
+        t1 = (unsigned)voxel / (unsigned)sx1;
+        zz = t1 / (unsigned)sy1;
+      }
+      const int x = voxel - (int)t1 * sx1, y = (int)t1 - (int)zz * sy1;
+      const int td =
+          (x - 1) / tx + ntx * ((y - 1) / ty + nty * (((int)zz - 1) / tz));
+      // one counter atomic per destination tile and wavefront: the lanes
+      // depositing to the same tile take consecutive slots
+      int k = ob.cap;
+      if (ob.cap > 0) {
+        const int lane = lane_id();
+        uint64_t todo = __ballot(1);
+        while (todo) {
+          const int leader = __ffsll((unsigned long long)todo) - 1;
+          const int tl = __shfl(td, leader);
+          const uint64_t same = __ballot(td == tl) & todo;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(ob.cnt + tl, __popcll(same));
+          base = __shfl(base, leader);
+          if (td == tl) k = base + __popcll(same & ((1ull << lane) - 1ull));
+          todo &= ~same;
+        }
+      }
+      if (k < ob.cap) {
+        float4 *r = ob.rec + 4 * ((int64_t)td * ob.cap + k);
+        r[0] = make_float4(__int_as_float(voxel), j[0], j[1], j[2]);
+        r[1] = make_float4(j[3], j[4], j[5], j[6]);
+        r[2] = make_float4(j[7], j[8], j[9], j[10]);
+        r[3] = make_float4(j[11], 0.0f, 0.0f, 0.0f);
+      } else {
 #pragma unroll
-      for (int k = 0; k < 12; k++) atomic_add_f32(g + k, j[k]);
+        for (int c = 0; c < 12; c++) atomic_add_f32(g + c, j[c]);
+      }
     }
   }
 };
@@ -934,7 +985,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
     float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
     vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
-    PushCoef C, GridDev G, TileGeom T, BinArgs Bn) {
+    PushCoef C, GridDev G, TileGeom T, BinArgs Bn, OowBuckets OB) {
   extern __shared__ unsigned long long lds_win[];  // 12 * ncell
   __shared__ vpic_particle_mover_t wq[TILE_WAVES][WAVE_Q];
   __shared__ int row_start[TILE_MAXROWS];
@@ -984,6 +1035,12 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   W.mx = T.mx;
   W.my = T.my;
   W.oow = &oow_cnt;
+  W.ob = OB;
+  W.tx = T.tx;
+  W.ty = T.ty;
+  W.tz = T.tz;
+  W.ntx = T.ntx;
+  W.nty = T.nty;
   const int ncell = W.wnx * W.wny * W.wnz;
   W.ncell = ncell;
   VoxDiv VD;
@@ -1234,10 +1291,18 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
 // One workgroup per tile, one thread per voxel of the tile, so a workgroup
 // reads its own window and the faces of its neighbours' windows; every
 // cell of acc is written by exactly one thread.
+//
+// The tile's bucket of out-of-window deposits (OowBuckets) is summed first,
+// in LDS as 64-bit fixed point with a scale set from its largest component
+// and record count (sums stay below 2^50, so the rounding of each record
+// is exact to 2^-50 of that bound), and added with the windows.
 __global__ __launch_bounds__(256) void k_merge_windows(
     float *__restrict__ acc, const float *__restrict__ wbuf, int wstride,
-    GridDev G, TileGeom T) {
+    GridDev G, TileGeom T, OowBuckets OB) {
+  __shared__ unsigned long long bsum[12 * TILE_MAXCELLS];
+  __shared__ float bmax[256];
   int b = blockIdx.x;
+  const int tile = b;
   const int bx = b % T.ntx;
   b /= T.ntx;
   const int by = b % T.nty;
@@ -1245,7 +1310,55 @@ __global__ __launch_bounds__(256) void k_merge_windows(
   const int x0 = 1 + bx * T.tx, y0 = 1 + by * T.ty, z0 = 1 + bz * T.tz;
   const int cx = min(T.tx, G.nx - bx * T.tx), cy = min(T.ty, G.ny - by * T.ty),
             cz = min(T.tz, G.nz - bz * T.tz);
-  for (int c = threadIdx.x; c < cx * cy * cz; c += blockDim.x) {
+  const int ncell = cx * cy * cz;
+  const int t = threadIdx.x;
+  const int nrec = OB.cap > 0 ? min(OB.cnt[tile], OB.cap) : 0;
+  double bscale = 0.0;
+  if (nrec > 0) {
+    const float4 *rec = OB.rec + 4 * (int64_t)tile * OB.cap;
+    float m = 0.0f;
+    for (int r = t; r < nrec; r += blockDim.x) {
+      const float4 a = rec[4 * r], c = rec[4 * r + 1], d = rec[4 * r + 2],
+                   e = rec[4 * r + 3];
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(a.y), fabsf(a.z)), fabsf(a.w)));
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)),
+                         fmaxf(fabsf(c.z), fabsf(c.w))));
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)),
+                         fmaxf(fabsf(d.z), fabsf(d.w))));
+      m = fmaxf(m, fabsf(e.x));
+    }
+    bmax[t] = m;
+    for (int k = t; k < 12 * ncell; k += blockDim.x) bsum[k] = 0ull;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w /= 2) {
+      if (t < w) bmax[t] = fmaxf(bmax[t], bmax[t + w]);
+      __syncthreads();
+    }
+    int ex = 0;
+    const double bound = (double)bmax[0] * (double)nrec;
+    if (bound > 0.0) frexp(bound, &ex);  // every sum is below 2^ex
+    bscale = ldexp(1.0, 50 - ex);
+    for (int r = t; r < nrec; r += blockDim.x) {
+      const float4 a = rec[4 * r], c = rec[4 * r + 1], d = rec[4 * r + 2],
+                   e = rec[4 * r + 3];
+      const int v = __float_as_int(a.x);
+      const int z = v / G.sz, rem = v - z * G.sz, y = rem / G.sy,
+                x = rem - y * G.sy;
+      const int lx = x - x0, ly = y - y0, lz = z - z0;
+      if ((unsigned)lx >= (unsigned)cx || (unsigned)ly >= (unsigned)cy ||
+          (unsigned)lz >= (unsigned)cz)
+        continue;  // not this tile's voxel (cannot happen)
+      const int cell = lx + cx * (ly + cy * lz);
+      const float jv[12] = {a.y, a.z, a.w, c.x, c.y, c.z,
+                            c.w, d.x, d.y, d.z, d.w, e.x};
+#pragma unroll
+      for (int k = 0; k < 12; k++)
+        atomicAdd(&bsum[k * ncell + cell],
+                  (unsigned long long)to_fixed((double)jv[k] * bscale));
+    }
+    __syncthreads();
+  }
+  for (int c = t; c < ncell; c += blockDim.x) {
     const int x = x0 + c % cx, y = y0 + (c / cx) % cy, z = z0 + c / (cx * cy);
     // tiles whose window holds v, per axis: own, and the neighbours across
     // the tile faces that v lies on (both, for a tile one cell wide)
@@ -1275,6 +1388,12 @@ __global__ __launch_bounds__(256) void k_merge_windows(
 #pragma unroll
           for (int k = 0; k < 12; k++) s[k] += w[k * nc];
         }
+    if (nrec > 0) {
+      const double inv = 1.0 / bscale;
+#pragma unroll
+      for (int k = 0; k < 12; k++)
+        s[k] += (float)((double)(long long)bsum[k * ncell + c] * inv);
+    }
     float4 *a4 = reinterpret_cast<float4 *>(
         acc + 12 * (int64_t)vox(x, y, z, G.nx, G.ny));
     float4 v0 = a4[0], v1 = a4[1], v2 = a4[2];
@@ -1293,6 +1412,12 @@ __global__ __launch_bounds__(256) void k_merge_windows(
 
 __global__ void k_fill_int32(int *a, int n, int v) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = v;
+}
+
+__global__ void k_fill_i32_grid(int *a, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    a[i] = v;
 }
 
 __device__ __forceinline__ void covered_range(const int *part, int nv, int np,
@@ -1594,6 +1719,7 @@ struct BinState {
 
 struct PushScratch {
   DeviceScratch counters, xlist, rlist, sortbuf, wbuf, trash, inject;
+  DeviceScratch ocnt, orec;  // out-of-window buckets (OowBuckets)
   BinState bins[8];
   int64_t bin_clock = 0;
 };
@@ -1633,7 +1759,8 @@ BinState &bin_state(PushScratch &S, const void *p) {
 typedef void (*TiledKernel)(vpic_particle_t *, int, const int *,
                             const vpic_interpolator_t *, float *, float *,
                             int, vpic_particle_t *, vpic_particle_mover_t *,
-                            int, int *, PushCoef, GridDev, TileGeom, BinArgs);
+                            int, int *, PushCoef, GridDev, TileGeom, BinArgs,
+                            OowBuckets);
 
 int tiled_kernel_id(int gs) {
   int id = 0;
@@ -1767,6 +1894,24 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         return rc;
       if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
         return rc;
+      // Buckets for out-of-window deposits: a quarter of a tile's average
+      // particle count per tile (late in a sort interval about a tenth of
+      // the electrons' deposits leave their window), 256 to 16384 records.
+      OowBuckets OB;
+      {
+        int64_t cap = (int64_t)np / nblk / 4;
+        cap = cap < 256 ? 256 : cap > 16384 ? 16384 : cap;
+        if ((rc = scratch_reserve(S.ocnt, sizeof(int) * (size_t)nblk)) ||
+            (rc = scratch_reserve(S.orec, 64 * (size_t)cap * (size_t)nblk)))
+          return rc;
+        OB.cnt = (int *)S.ocnt.ptr;
+        OB.rec = (float4 *)S.orec.ptr;
+        OB.cap = (int)cap;
+        int fb = (nblk + 255) / 256;
+        hipLaunchKernelGGL(k_fill_i32_grid, dim3(fb < 1024 ? fb : 1024),
+                           dim3(256), 0, s, OB.cnt, nblk, 0);
+        VPIC_LAUNCHED("k_fill_i32_grid");
+      }
       static bool attr_set[7] = {false};
       const int ak = tiled_kernel_id(g_group);
       if (!attr_set[ak]) {
@@ -1845,7 +1990,7 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                            dim3(TILE_BS), lds, s, p, np, partition, f,
                            (float *)a, (float *)S.wbuf.ptr, wstride,
                            (vpic_particle_t *)S.trash.ptr, rlist, xcap, cnt,
-                           C, G, T, BA);
+                           C, G, T, BA, OB);
         VPIC_LAUNCHED("k_advance_p_tiled");
       }
       if (timed) push_timing_record(s, np, false);
@@ -1856,7 +2001,8 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         BSt->cur ^= 1;
       }
       hipLaunchKernelGGL(k_merge_windows, dim3(nblk), dim3(256), 0, s,
-                         (float *)a, (const float *)S.wbuf.ptr, wstride, G, T);
+                         (float *)a, (const float *)S.wbuf.ptr, wstride, G, T,
+                         OB);
       VPIC_LAUNCHED("k_merge_windows");
       // particles past the partition (added since the sort) go flat
       flat_start = partition + g->nv;
