@@ -131,7 +131,7 @@ def main():
     ap.add_argument("--direct-runs", type=int, default=None,
                     help="push deposit threshold (runs per wavefront)")
     ap.add_argument("--sweep", type=str, default=None,
-                    help="tune: 'tx,ty,tz:runs[:group];...' each "
+                    help="tune: 'tx,ty,tz:runs[:group[:bin_mode]];...' each "
                          "timed in turn on one plasma (one JSON line per "
                          "candidate)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -170,10 +170,12 @@ def main():
         for cand in args.sweep.split(";"):
             f = cand.split(":")
             tile, runs = f[0], int(f[1])
-            gs = int(f[2]) if len(f) > 2 else 4
+            gs = int(f[2]) if len(f) > 2 else 16
+            bmode = int(f[3]) if len(f) > 3 else 0
             _lib.call("vpic_hip_push_tile", *[int(v) for v in tile.split(",")])
             _lib.call("vpic_hip_push_direct_runs", runs)
             _lib.call("vpic_hip_push_group", gs)
+            _lib.call("vpic_hip_push_bin_mode", bmode)
             s.advance()  # settle the new setting
             _lib.call("vpic_hip_push_timing", None, None, None)
             _lib.call("vpic_hip_push_timing_enable", 1)
@@ -188,6 +190,7 @@ def main():
             _lib.call("vpic_hip_push_timing", ctypes.byref(launches),
                       ctypes.byref(push_ms), None)
             print(json.dumps({"tile": tile, "direct_runs": runs, "group": gs,
+                              "bin_mode": bmode,
                               "ms_per_step": el * 1e3 / args.steps,
                               "push_ms": push_ms.value / launches.value,
                               "value": npart * args.steps / el}), flush=True)

@@ -222,16 +222,17 @@ int vpic_hip_push_direct_runs(int32_t runs);
  * reference counterpart. */
 int vpic_hip_push_group(int32_t gs);
 
-/* Tile lists for the partition-driven push (default 1).  Each tiled push
- * records which tile holds every particle it pushed; with lists on, the next
- * push of the same particle array gives each tile the particles now in its
- * cells, so particles that drifted since the sort still deposit into their
- * tile's LDS window.  0: off (always the sort-time partition); 1: on once
- * the last push made out-of-window deposits for at least 1/20 of its
- * particles, until the next sort_p of that array (below that the records
- * and list reads cost more than the global atomics they remove); 2: on
- * whenever the last push's records match.  Results do not depend on the
- * mode.  A tuning knob; no reference counterpart. */
+/* Tile lists for the partition-driven push (default 0).  Each tiled push
+ * can record which tile holds every particle it pushed; with lists on, the
+ * next push of the same particle array gives each tile the particles now
+ * in its cells, so particles that drifted since the sort still deposit
+ * into their tile's LDS window.  0: off (always the sort-time partition;
+ * out-of-window deposits go through the per-tile buckets, which on the
+ * benchmark plasma costs less than recording and reading the lists); 1: on
+ * once the last push made out-of-window deposits for at least 1/20 of its
+ * particles, until the next sort_p of that array; 2: on whenever the last
+ * push's records match.  Results do not depend on the mode.  A tuning
+ * knob; no reference counterpart. */
 int vpic_hip_push_bin_mode(int32_t mode);
 
 /* move_p for one mover (species_advance.h:153-157; move_p.cc:215-378),

@@ -589,7 +589,7 @@ def test_tiled_push_tile_lists(dev, orc, mode, tile_reset):
             assert H.same_bits(dp.download()[:np_cur], po[:np_cur]), step
             assert H.acc_rel_err(da.download(), ao) <= ACC_TOL, step
     finally:
-        _lib.call("vpic_hip_push_bin_mode", 1)
+        _lib.call("vpic_hip_push_bin_mode", 0)
 
 
 def test_tiled_push_np_changed(dev, orc):

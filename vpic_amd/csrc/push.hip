@@ -1739,7 +1739,7 @@ int pow2_at_least(int64_t n) {
 
 TileGeom g_tile = {8, 8, 8, 0, 0, 0, 0, 0};
 int g_group = 16;  // lanes sharing one run of particles in the tiled push
-int g_bin_mode = 1;  // tile lists: 0 off, 1 when drift warrants, 2 always
+int g_bin_mode = 0;  // tile lists: 0 off, 1 when drift warrants, 2 always
 
 BinState &bin_state(PushScratch &S, const void *p) {
   BinState *hit = nullptr, *lru = &S.bins[0];
