@@ -101,15 +101,31 @@ __device__ __forceinline__ long long to_fixed(double y) {
 // on the order of the additions.  A value too large for the fixed-point
 // range, and any cell outside the window, goes to the global accumulator
 // with float atomics.
-// Out-of-window deposits of the tiled push, by destination tile: record
-// k of tile t is rec[4 * (t * cap + k)] .. +3 = {voxel (bits), j0..j2},
-// {j3..j6}, {j7..j10}, {j11, 0, 0, 0}; k_merge_windows adds them.  A full
-// bucket falls back to float atomics into the accumulator.
+// Out-of-window deposits of the tiled push, kept by the pushing (source)
+// tile in 27 sub-buckets, one per neighbouring destination tile (offset
+// code (dx+1) + 3(dy+1) + 9(dz+1), tile indices taken periodically).
+// Record k of source tile s, code d, is rec[4 * ((27 s + d) cap + k)] .. +3
+// = {voxel (bits), j0..j2}, {j3..j6}, {j7..j10}, {j11, 0, 0, 0}; cnt[27 s +
+// d] is written at the end of the push.  k_merge_windows of tile t reads,
+// for every code d, the sub-bucket d of the tile t - offset(d): each
+// sub-bucket has exactly one reader.  Slots are taken with LDS atomics; a
+// deposit further than one tile away or into a full sub-bucket goes to the
+// accumulator with float atomics.
 struct OowBuckets {
-  int *cnt;     // records per tile (may exceed cap)
-  float4 *rec;  // cap records per tile
+  int *cnt;     // 27 counts per source tile (may exceed cap)
+  float4 *rec;  // 27 * cap records per source tile
   int cap;
 };
+
+// Offset code of destination tile (x, y, z) seen from tile (bx, by, bz),
+// periodic in the tile indices; -1 when further than one tile on an axis.
+__device__ __forceinline__ int tile_code_1(int d, int n) {
+  d = d < 0 ? d + n : d;  // 0 .. n-1
+  if (d == 0) return 1;
+  if (d == 1) return 2;
+  if (d == n - 1) return 0;
+  return -1;
+}
 
 struct WindowSink {
   lds_u64 *lds;  // 12 * ncell
@@ -122,7 +138,10 @@ struct WindowSink {
   unsigned mx, my;  // fast_div multipliers for sx1, sy1 (0: divide)
   int *oow;         // LDS count of deposits outside the window
   OowBuckets ob;    // where those deposits go
-  int tx, ty, tz, ntx, nty;  // tile extent and tiles per axis (x, y)
+  int *dcnt;        // LDS: this tile's 27 sub-bucket counts
+  int tx, ty, tz, ntx, nty, ntz;  // tile extent and tiles per axis
+  int btx, bty, btz;              // this tile
+  int self;                       // its index
   __device__ __forceinline__ int slot(int v) const {
     unsigned t, z;
     if (mx) {
@@ -174,9 +193,9 @@ struct WindowSink {
       }
     } else {
       atomicAdd(oow, 1);
-      // to the bucket of the tile holding the voxel (one integer atomic and
-      // a 64-B store instead of 12 float atomics, which this part performs
-      // beyond the L2)
+      // to this tile's sub-bucket for the destination tile (an LDS atomic
+      // and a 64-B store instead of 12 float atomics, which this part
+      // performs beyond the L2)
       unsigned t1, zz;
       if (mx) {
         t1 = fast_div((unsigned)voxel, mx);
@@ -186,27 +205,29 @@ struct WindowSink {
         zz = t1 / (unsigned)sy1;
       }
       const int x = voxel - (int)t1 * sx1, y = (int)t1 - (int)zz * sy1;
-      const int td =
-          (x - 1) / tx + ntx * ((y - 1) / ty + nty * (((int)zz - 1) / tz));
-      // one counter atomic per destination tile and wavefront: the lanes
-      // depositing to the same tile take consecutive slots
+      const int cx = tile_code_1((x - 1) / tx - btx, ntx);
+      const int cy = tile_code_1((y - 1) / ty - bty, nty);
+      const int cz = tile_code_1(((int)zz - 1) / tz - btz, ntz);
+      const int d = (cx < 0 || cy < 0 || cz < 0) ? -1 : cx + 3 * cy + 9 * cz;
+      // one LDS atomic per code and wavefront: the lanes depositing to the
+      // same sub-bucket take consecutive slots
       int k = ob.cap;
       if (ob.cap > 0) {
         const int lane = lane_id();
-        uint64_t todo = __ballot(1);
+        uint64_t todo = __ballot(d >= 0);
         while (todo) {
           const int leader = __ffsll((unsigned long long)todo) - 1;
-          const int tl = __shfl(td, leader);
-          const uint64_t same = __ballot(td == tl) & todo;
+          const int dl = __shfl(d, leader);
+          const uint64_t same = __ballot(d == dl) & todo;
           int base = 0;
-          if (lane == leader) base = atomicAdd(ob.cnt + tl, __popcll(same));
+          if (lane == leader) base = atomicAdd(dcnt + dl, __popcll(same));
           base = __shfl(base, leader);
-          if (td == tl) k = base + __popcll(same & ((1ull << lane) - 1ull));
+          if (d == dl) k = base + __popcll(same & ((1ull << lane) - 1ull));
           todo &= ~same;
         }
       }
-      if (k < ob.cap) {
-        float4 *r = ob.rec + 4 * ((int64_t)td * ob.cap + k);
+      if (d >= 0 && k < ob.cap) {
+        float4 *r = ob.rec + 4 * ((int64_t)(27 * self + d) * ob.cap + k);
         r[0] = make_float4(__int_as_float(voxel), j[0], j[1], j[2]);
         r[1] = make_float4(j[3], j[4], j[5], j[6]);
         r[2] = make_float4(j[7], j[8], j[9], j[10]);
@@ -993,6 +1014,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   __shared__ unsigned qw_bits;  // largest |q w| of the first round (bits)
   __shared__ int bin_cnt[27];
   __shared__ int oow_cnt;
+  __shared__ int oow_dcnt[27];
   // interpolation coefficients of the tile's cells (gathers from LDS cost
   // far fewer cache requests than per-lane global gathers once the
   // particles have drifted out of voxel order)
@@ -1036,11 +1058,17 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   W.my = T.my;
   W.oow = &oow_cnt;
   W.ob = OB;
+  W.dcnt = oow_dcnt;
   W.tx = T.tx;
   W.ty = T.ty;
   W.tz = T.tz;
   W.ntx = T.ntx;
   W.nty = T.nty;
+  W.ntz = T.ntz;
+  W.btx = btx;
+  W.bty = bty;
+  W.btz = btz;
+  W.self = blockIdx.x;
   const int ncell = W.wnx * W.wny * W.wnz;
   W.ncell = ncell;
   VoxDiv VD;
@@ -1088,6 +1116,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   if (t == 0) qw_bits = 0u;
   if (t < 27) bin_cnt[t] = 0;
   if (t == 0) oow_cnt = 0;
+  if (t < 27) oow_dcnt[t] = 0;
   __syncthreads();
   if (t == 0) {
     row_pref[0] = 0;
@@ -1278,6 +1307,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   __syncthreads();
   if (keyed && t < 27) Bn.counts[blockIdx.x * 27 + t] = bin_cnt[t];
   if (Bn.oow && t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
+  if (OB.cap > 0 && t < 27) OB.cnt[27 * blockIdx.x + t] = oow_dcnt[t];
   // The window, as floats, to this tile's slot (component-major, ncell).
   float *wout = wbuf + (int64_t)blockIdx.x * wstride;
   const double inv = 1.0 / W.scale;
@@ -1312,20 +1342,49 @@ __global__ __launch_bounds__(256) void k_merge_windows(
             cz = min(T.tz, G.nz - bz * T.tz);
   const int ncell = cx * cy * cz;
   const int t = threadIdx.x;
-  const int nrec = OB.cap > 0 ? min(OB.cnt[tile], OB.cap) : 0;
+  // the sub-buckets addressed to this tile: code d of source tile
+  // (this tile - offset(d)), periodic in the tile indices
+  __shared__ int sb_n[27];
+  __shared__ int64_t sb_at[27];
+  __shared__ int sb_total;
+  if (t < 27) {
+    int n = 0;
+    int64_t at = 0;
+    if (OB.cap > 0) {
+      const int ox = t % 3 - 1, oy = (t / 3) % 3 - 1, oz = t / 9 - 1;
+      const int sx = (bx - ox + T.ntx) % T.ntx, sy = (by - oy + T.nty) % T.nty,
+                sz = (bz - oz + T.ntz) % T.ntz;
+      const int src = sx + T.ntx * (sy + T.nty * sz);
+      at = (int64_t)(27 * src + t) * OB.cap;
+      n = min(OB.cnt[27 * src + t], OB.cap);
+    }
+    sb_n[t] = n;
+    sb_at[t] = at;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int tot = 0;
+    for (int d = 0; d < 27; d++) tot += sb_n[d];
+    sb_total = tot;
+  }
+  __syncthreads();
+  const int nrec = sb_total;
+  (void)tile;
   double bscale = 0.0;
   if (nrec > 0) {
-    const float4 *rec = OB.rec + 4 * (int64_t)tile * OB.cap;
     float m = 0.0f;
-    for (int r = t; r < nrec; r += blockDim.x) {
-      const float4 a = rec[4 * r], c = rec[4 * r + 1], d = rec[4 * r + 2],
-                   e = rec[4 * r + 3];
-      m = fmaxf(m, fmaxf(fmaxf(fabsf(a.y), fabsf(a.z)), fabsf(a.w)));
-      m = fmaxf(m, fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)),
-                         fmaxf(fabsf(c.z), fabsf(c.w))));
-      m = fmaxf(m, fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)),
-                         fmaxf(fabsf(d.z), fabsf(d.w))));
-      m = fmaxf(m, fabsf(e.x));
+    for (int d = 0; d < 27; d++) {
+      const float4 *rec = OB.rec + 4 * sb_at[d];
+      for (int r = t; r < sb_n[d]; r += blockDim.x) {
+        const float4 a = rec[4 * r], c = rec[4 * r + 1], e = rec[4 * r + 2],
+                     f = rec[4 * r + 3];
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(a.y), fabsf(a.z)), fabsf(a.w)));
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)),
+                           fmaxf(fabsf(c.z), fabsf(c.w))));
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(e.x), fabsf(e.y)),
+                           fmaxf(fabsf(e.z), fabsf(e.w))));
+        m = fmaxf(m, fabsf(f.x));
+      }
     }
     bmax[t] = m;
     for (int k = t; k < 12 * ncell; k += blockDim.x) bsum[k] = 0ull;
@@ -1338,23 +1397,26 @@ __global__ __launch_bounds__(256) void k_merge_windows(
     const double bound = (double)bmax[0] * (double)nrec;
     if (bound > 0.0) frexp(bound, &ex);  // every sum is below 2^ex
     bscale = ldexp(1.0, 50 - ex);
-    for (int r = t; r < nrec; r += blockDim.x) {
-      const float4 a = rec[4 * r], c = rec[4 * r + 1], d = rec[4 * r + 2],
-                   e = rec[4 * r + 3];
-      const int v = __float_as_int(a.x);
-      const int z = v / G.sz, rem = v - z * G.sz, y = rem / G.sy,
-                x = rem - y * G.sy;
-      const int lx = x - x0, ly = y - y0, lz = z - z0;
-      if ((unsigned)lx >= (unsigned)cx || (unsigned)ly >= (unsigned)cy ||
-          (unsigned)lz >= (unsigned)cz)
-        continue;  // not this tile's voxel (cannot happen)
-      const int cell = lx + cx * (ly + cy * lz);
-      const float jv[12] = {a.y, a.z, a.w, c.x, c.y, c.z,
-                            c.w, d.x, d.y, d.z, d.w, e.x};
+    for (int d = 0; d < 27; d++) {
+      const float4 *rec = OB.rec + 4 * sb_at[d];
+      for (int r = t; r < sb_n[d]; r += blockDim.x) {
+        const float4 a = rec[4 * r], c = rec[4 * r + 1], e = rec[4 * r + 2],
+                     f = rec[4 * r + 3];
+        const int v = __float_as_int(a.x);
+        const int z = v / G.sz, rem = v - z * G.sz, y = rem / G.sy,
+                  x = rem - y * G.sy;
+        const int lx = x - x0, ly = y - y0, lz = z - z0;
+        if ((unsigned)lx >= (unsigned)cx || (unsigned)ly >= (unsigned)cy ||
+            (unsigned)lz >= (unsigned)cz)
+          continue;  // not this tile's voxel (cannot happen)
+        const int cell = lx + cx * (ly + cy * lz);
+        const float jv[12] = {a.y, a.z, a.w, c.x, c.y, c.z,
+                              c.w, e.x, e.y, e.z, e.w, f.x};
 #pragma unroll
-      for (int k = 0; k < 12; k++)
-        atomicAdd(&bsum[k * ncell + cell],
-                  (unsigned long long)to_fixed((double)jv[k] * bscale));
+        for (int k = 0; k < 12; k++)
+          atomicAdd(&bsum[k * ncell + cell],
+                    (unsigned long long)to_fixed((double)jv[k] * bscale));
+      }
     }
     __syncthreads();
   }
@@ -1894,23 +1956,22 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         return rc;
       if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
         return rc;
-      // Buckets for out-of-window deposits: a quarter of a tile's average
-      // particle count per tile (late in a sort interval about a tenth of
-      // the electrons' deposits leave their window), 256 to 16384 records.
+      // Sub-buckets for out-of-window deposits: 27 per tile of 1/32 of a
+      // tile's average particle count each (late in a sort interval about
+      // a tenth of the electrons' deposits leave their window, mostly
+      // across the six faces), 64 to 4096 records.  Every tiled push writes
+      // all 27 counts of every tile, so they need no clearing.
       OowBuckets OB;
       {
-        int64_t cap = (int64_t)np / nblk / 4;
-        cap = cap < 256 ? 256 : cap > 16384 ? 16384 : cap;
-        if ((rc = scratch_reserve(S.ocnt, sizeof(int) * (size_t)nblk)) ||
-            (rc = scratch_reserve(S.orec, 64 * (size_t)cap * (size_t)nblk)))
+        int64_t cap = (int64_t)np / nblk / 32;
+        cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
+        if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 27 * (size_t)nblk)) ||
+            (rc = scratch_reserve(S.orec,
+                                  64 * 27 * (size_t)cap * (size_t)nblk)))
           return rc;
         OB.cnt = (int *)S.ocnt.ptr;
         OB.rec = (float4 *)S.orec.ptr;
         OB.cap = (int)cap;
-        int fb = (nblk + 255) / 256;
-        hipLaunchKernelGGL(k_fill_i32_grid, dim3(fb < 1024 ? fb : 1024),
-                           dim3(256), 0, s, OB.cnt, nblk, 0);
-        VPIC_LAUNCHED("k_fill_i32_grid");
       }
       static bool attr_set[7] = {false};
       const int ak = tiled_kernel_id(g_group);
