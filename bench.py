@@ -134,6 +134,10 @@ def main():
                     help="tune: 'tx,ty,tz:runs[:group[:bin_mode]];...' each "
                          "timed in turn on one plasma (one JSON line per "
                          "candidate)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only time an in-place read+write of 13.4 GB (one "
+                         "species' particle array) with a plain elementwise "
+                         "kernel: the stream rate the push is compared to")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 4 GiB device-to-device copy three times "
                          "(calibrates the HBM byte counters)")
@@ -159,6 +163,21 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 4 * n,
                           "copy_probe_bytes_written": 4 * n}))
+        return
+    if args.stream_probe:
+        n = 419430400 * 8  # floats: 32 B x 419,430,400 particles
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        for _ in range(2):
+            a.mul_(1.0)
+        e0, e1 = torch.cuda.Event(True), torch.cuda.Event(True)
+        e0.record()
+        for _ in range(5):
+            a.mul_(1.0)  # reads and writes every byte in place
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"stream_probe_bytes": 8 * n, "ms": ms,
+                          "tb_per_s": 8 * n / (ms * 1e-3) / 1e12}))
         return
     if args.sweep:
         # one plasma, several push settings in turn (each timed over the
