@@ -208,7 +208,15 @@ struct WindowSink {
       const int cx = tile_code_1((x - 1) / tx - btx, ntx);
       const int cy = tile_code_1((y - 1) / ty - bty, nty);
       const int cz = tile_code_1(((int)zz - 1) / tz - btz, ntz);
-      const int d = (cx < 0 || cy < 0 || cz < 0) ? -1 : cx + 3 * cy + 9 * cz;
+      // a non-finite deposit keeps the float path (its NaN or infinity
+      // reaches the accumulator as it would have; the merge's fixed point
+      // needs finite records)
+      float mag = fabsf(j[0]);
+#pragma unroll
+      for (int c = 1; c < 12; c++) mag += fabsf(j[c]);
+      const int d = (cx < 0 || cy < 0 || cz < 0 || !(mag < 3.0e38f))
+                        ? -1
+                        : cx + 3 * cy + 9 * cz;
       // one LDS atomic per code and wavefront: the lanes depositing to the
       // same sub-bucket take consecutive slots
       int k = ob.cap;
