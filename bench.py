@@ -227,7 +227,7 @@ def main():
     for _ in range(args.warmup):
         s.advance()
     torch.cuda.synchronize()
-    nm, ign, bad = s.check_movers()
+    nm, ign, bad = [sum(v) for v in zip(*[s.check_movers(sp) for sp in s.species])]
     assert bad == 0 and nm == 0, (nm, ign, bad)
     log("warmup done (%d steps)" % args.warmup)
 
@@ -248,7 +248,7 @@ def main():
     _lib.call("vpic_hip_push_timing", ctypes.byref(launches),
               ctypes.byref(push_ms), ctypes.byref(push_parts))
     phases = timer.totals_ms()
-    nm, ign, bad = s.check_movers()
+    nm, ign, bad = [sum(v) for v in zip(*[s.check_movers(sp) for sp in s.species])]
     assert bad == 0
 
     value = npart * args.steps / el

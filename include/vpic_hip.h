@@ -206,7 +206,7 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
                              void *stream);
 
 /* Tile shape (cells) of the partition-driven push; its LDS window holds
- * (tx+2)(ty+2)(tz+2) accumulators.  Default 8x8x4.  No reference
+ * (tx+2)(ty+2)(tz+2) accumulators.  Default 8x8x8.  No reference
  * counterpart (the reference's pipelines split particles, not cells). */
 int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
 

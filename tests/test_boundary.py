@@ -366,3 +366,42 @@ def test_inject_p_gpu(dev, orc):
         assert H.same_bits(dp.download()[:no], po[:no])
         assert H.same_bits(dpm.download()[:nmo], pmo[:nmo])
         assert H.same_bits(da.download(), acco)
+
+
+@pytest.mark.gpu
+def test_simulation_absorbing_walls_gpu(orc):
+    """The step driver with absorbing z walls: each step's push and
+    boundary_p leave the same particles, in the same slots, as the oracle's
+    advance_p + boundary_p; rhob gets the absorbed charge."""
+    import torch
+    from vpic_amd import _lib
+    from vpic_amd import sim as S
+    assert torch.cuda.is_available()
+    _lib.call("vpic_hip_init", 0)
+    g = _box(6, 5, 4)
+    g.pbc[2] = g.pbc[5] = T.BC_ABSORB_PARTICLES
+    s = S.Simulation(g)
+    f = H.random_fields(g, 21, 0.05, 0.05)
+    s.fields.upload(f)
+    s.load_interpolator_array()
+    p = H.thermal_particles(g, 6, 0.4, 22, shuffle=True)
+    sp = S.Species("electron", -1.0, 1.0, len(p), g)
+    sp.p.upload(p)
+    s.add_species(sp)
+    hg = host_grid(g)
+    po = p.copy()
+    absorbed = 0
+    for step in range(4):
+        fi = s.interp.download().copy()
+        say("absorbing walls, step", step, "np", sp.np)
+        s.advance()
+        nmo, ign, pm = orc.advance_p(po, -1.0, 1.0, H.acc_array(g), fi, hg)
+        assert ign == 0
+        fo = np.zeros(g.nv, dtype=T.FIELD)
+        n, hits, counts = orc.boundary_p(po, len(po), pm, fo, -1.0, hg)
+        po = po[:n].copy()
+        absorbed += nmo
+        assert sp.np == n and sp.nm == 0
+        assert H.same_bits(sp.p.download()[:n], po)
+    assert absorbed > 0
+    assert np.any(s.fields.download()["rhob"] != 0)

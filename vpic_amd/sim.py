@@ -39,6 +39,7 @@ class Species:
         self.sort_interval = int(sort_interval)
         self.last_sorted = -1
         self.partition = device.DeviceArray((grid.nv + 1,), np.int32)
+        self.dev_nm = device.DeviceArray((4,), np.int32)  # last push's movers
 
 
 class Simulation:
@@ -59,7 +60,6 @@ class Simulation:
         self.aux = None
         self.step = 0
         self.stream = None  # the default stream
-        self.dev_nm = device.DeviceArray((4,), np.int32)
         self.nm_log = []
 
     def add_species(self, sp):
@@ -84,7 +84,7 @@ class Simulation:
         # after a sort the partition drives the tiled (LDS) push
         part = sp.partition.ptr if sp.last_sorted >= 0 else None
         _lib.call("vpic_hip_advance_p_async", sp.p.ptr, sp.np, sp.q, sp.m,
-                  sp.pm.ptr, sp.max_nm, self.dev_nm.ptr, self.acc.ptr,
+                  sp.pm.ptr, sp.max_nm, sp.dev_nm.ptr, self.acc.ptr,
                   self.interp.ptr, self._G(), part, self.stream)
 
     def load_interpolator_array(self):
@@ -126,11 +126,36 @@ class Simulation:
                   self.interp.ptr, self._G(), ctypes.byref(en), self.stream)
         return en.value
 
-    def check_movers(self):
-        """nm of the last advance_p (host read; synchronises)."""
+    def check_movers(self, sp=None):
+        """(nm, movers lost to max_nm, bad voxels) of the last advance_p of
+        sp (default: the last species; host read, synchronises)."""
+        sp = sp if sp is not None else self.species[-1]
         torch.cuda.synchronize()
-        v = self.dev_nm.download()
+        v = sp.dev_nm.download()
         return int(v[0]), int(v[1]), int(v[2])
+
+    def boundary_p(self):
+        """boundary_p (boundary_p.cc:42-750) for this one domain: every
+        species' movers (absorbing faces: charge to rhob, particle removed).
+        Custom particle boundaries need host handlers, which this driver
+        does not have: a hit on one is an error."""
+        for sp in self.species:
+            nm = self.check_movers(sp)[0]
+            sp.nm = nm
+            if nm == 0:
+                continue
+            npc = ctypes.c_int32(sp.np)
+            counts = (ctypes.c_int32 * 2)()
+            _lib.call("vpic_hip_boundary_p", sp.p.ptr, ctypes.byref(npc),
+                      sp.pm.ptr, nm, self.fields.ptr, sp.q, self._G(), None,
+                      counts, self.stream)
+            if counts[0] or counts[1]:
+                raise _lib.VpicError(
+                    "boundary_p: %d particles hit custom particle boundaries "
+                    "and %d unknown ones (species %s); this driver handles "
+                    "absorbing walls only" % (counts[0], counts[1], sp.name))
+            sp.np = npc.value
+            sp.nm = 0
 
     # -- output (src/vpic/dump.cc) -----------------------------------------
     def _info(self):
@@ -226,6 +251,9 @@ class Simulation:
             self.clear_accumulator_array()
             for sp in self.species:
                 self.advance_p(sp)
+            # advance.cc:70-77: particles that hit a particle boundary
+            if any(self.g.pbc[k] != T.BC_PERIODIC for k in range(6)):
+                self.boundary_p()
             self.reduce_accumulator_array()
         with t("field"):
             self.unload_accumulator_array()
