@@ -370,36 +370,32 @@ __global__ __launch_bounds__(64) void k_rs_scatter(
   }
 }
 
-// out[j] = p[perm[j]] for the 4 x 256 slots of a block (each load and
-// store instruction covers consecutive slots; four in flight per thread),
-// and partition[v] = first slot whose key is >= v (partition[nv] = np),
-// from the sorted keys.
+// out[j] = p[perm[j]] for the 2 x 256 slots of a block, each particle
+// moved by a pair of lanes (16 B each, so every load and store instruction
+// covers consecutive 16-B pieces; four in flight per lane), and
+// partition[v] = first slot whose key is >= v (partition[nv] = np), from
+// the sorted keys (written by the even lane of the pair).
 __global__ __launch_bounds__(256) void k_gather_part(
     const vpic_particle_t *__restrict__ src, const int *__restrict__ perm,
     const int *__restrict__ keys, int np, int nv,
     vpic_particle_t *__restrict__ dst, int *__restrict__ part) {
-  const int64_t j0 = 1024 * (int64_t)blockIdx.x + threadIdx.x;
-  float4 a[4], b[4];
+  const int half = threadIdx.x & 1;
+  const int64_t j0 = 512 * (int64_t)blockIdx.x + (threadIdx.x >> 1);
+  float4 a[4];
   int kc[4], kp[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const int64_t j = j0 + 256 * u;
+    const int64_t j = j0 + 128 * u;
     const int q = j < np ? perm[j] : 0;
     kc[u] = j < np ? keys[j] : nv;
     kp[u] = j == 0 ? -1 : (j <= np ? keys[j - 1] : nv);
-    const float4 *sp = reinterpret_cast<const float4 *>(src + q);
-    a[u] = sp[0];
-    b[u] = sp[1];
+    a[u] = reinterpret_cast<const float4 *>(src + q)[half];
   }
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const int64_t j = j0 + 256 * u;
-    if (j < np) {
-      float4 *d = reinterpret_cast<float4 *>(dst + j);
-      d[0] = a[u];
-      d[1] = b[u];
-    }
-    if (j <= np)
+    const int64_t j = j0 + 128 * u;
+    if (j < np) reinterpret_cast<float4 *>(dst + j)[half] = a[u];
+    if (!half && j <= np)
       for (int v = kp[u] + 1; v <= kc[u] && v <= nv; v++) part[v] = (int)j;
   }
 }
@@ -508,7 +504,7 @@ int sort_p_axes(vpic_particle_t *p, vpic_particle_t *aux, int np,
                        (const int *)hist, nblocks, kout, iout);
     VPIC_LAUNCHED("k_rs_scatter");
   }
-  hipLaunchKernelGGL(k_gather_part, dim3(blocks_for((int64_t)np + 1, 1024)),
+  hipLaunchKernelGGL(k_gather_part, dim3(blocks_for((int64_t)np + 1, 512)),
                      dim3(256), 0, s, p, (const int *)ia, (const int *)ka, np,
                      nv, aux, partition);
   VPIC_LAUNCHED("k_gather_part");
