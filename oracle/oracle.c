@@ -8,8 +8,12 @@
  * Parity of each function is pinned in tests/ by (a) the reference's own
  * scalar kernels compiled from source into oracle/_ref (bit-exact, where
  * they build), (b) the reference's known-answer decks (accel, cyclo,
- * interpe, inbndj/outbndj charge conservation) and (c) closed forms and
- * brute force; the pin of each function is listed in tests/test_oracle_*.
+ * interpe, inbndj/outbndj charge conservation), (c) closed forms and
+ * brute force and (d) for the field walls (local.cc's ghost and adjust
+ * branches), the method of images: a walled box against the periodic box
+ * of twice its length with mirrored fields; the pin of each function is
+ * listed in tests/test_oracle_*.  absorb_fields walls are unpinned (the
+ * device path does not support them).
  */
 #include "oracle.h"
 
