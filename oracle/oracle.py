@@ -2,7 +2,7 @@
 
 `Oracle` wraps liboracle.so, the CPU restatement in oracle.c.  `Ref` wraps
 _ref/libvpicref.so, the reference's own scalar kernels compiled from the
-lanl/vpic sources (present only where /root/reference was available when it
+lanl/vpic sources (present only where those sources were available when it
 was built).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
 baseline may import this module, and only as the checker.
 
