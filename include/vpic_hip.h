@@ -507,6 +507,62 @@ int vpic_hip_write_grid(const char *fname, const vpic_grid_t *g,
                         const vpic_dump_info_t *info, const int32_t bc[27],
                         const int64_t *range, const int64_t *neighbor);
 
+/* The strided dumps and their global header (dump.cc:337-880, the "new
+ * dump logic"), with the reference's DumpParameters (vpic.h:92-117):
+ * output_vars is its BitField (bit i = variable i: field_t words 0..23,
+ * hydro_t words 0..13; 0xffffffff = all), format band (one block per
+ * variable) or band_interleave (whole records). */
+enum { VPIC_DUMP_BAND = 0, VPIC_DUMP_BAND_INTERLEAVE = 1 };
+typedef struct vpic_dump_params {
+  const char *base_dir;        /* DumpParameters.baseDir */
+  const char *base_file_name;  /* DumpParameters.baseFileName */
+  int32_t stride[3];           /* stride_x, stride_y, stride_z */
+  int32_t format;              /* VPIC_DUMP_BAND or _BAND_INTERLEAVE */
+  uint32_t output_vars;
+} vpic_dump_params_t;
+
+/* What global_header takes besides the grid: the global domain's extents
+ * (grid_t x0..z1) and the domain topology px, py, pz; only rank 0 writes. */
+typedef struct vpic_global_info {
+  float x0, x1, y0, y1, z0, z1;
+  int32_t px, py, pz, rank;
+} vpic_global_info_t;
+
+/* global_header (dump.cc:405-532): writes "<base>.vpc", the text header of
+ * the strided dumps: version, data-header size, time step, cvac, eps0,
+ * extents, spacing, topology, then the field variables of params[0] and
+ * the hydro variables of each species' params[1..nparams-1]
+ * (dump.cc:337-371 tables).  Needs nparams >= 2, as the reference asserts. */
+int vpic_hip_global_header(const char *base, const vpic_grid_t *g,
+                           const vpic_global_info_t *gi,
+                           const vpic_dump_params_t *params, int32_t nparams);
+
+/* field_dump (dump.cc:533-706): mkdir "<base_dir>/T.<step>" and write
+ * "<base_dir>/T.<step>/<base_file_name>.<step>.<rank>": the version-0
+ * header with the strided extents and spacing (nx/stride_x ...,
+ * dx*stride_x ...), an array header of (nx/sx+2)(ny/sy+2)(nz/sz+2)
+ * elements, then the data: band = for each selected word v (ascending),
+ * word v of every output voxel; band_interleave = whole field_t records.
+ * Output voxels along each axis are the ghost 0, then k*stride - 1 for
+ * k = 1..n/stride, then the ghost n+1.  step < 0 means info->step.  As in
+ * the reference, words 20..23 (the last four material groups, counted as
+ * one word each) lie in the next voxel's record; for the array's last
+ * voxel, which has none, 0 is written.  A stride that does not divide the
+ * grid, or an output_vars whose word count the reference would take past
+ * its variable list, is an error (-1). */
+int vpic_hip_field_dump(const vpic_field_t *f, const vpic_grid_t *g,
+                        const vpic_dump_info_t *info,
+                        const vpic_dump_params_t *p, int64_t step);
+
+/* hydro_dump (dump.cc:708-880), the same for a hydro array (as
+ * synchronize_hydro_array leaves it), header with the species id and q/m.
+ * band as for the fields; band_interleave writes nx/sx * ny/sy * nz/sz
+ * records without the ghost layers' extra count, taken as the reference
+ * takes them (the first records of the array for unit strides). */
+int vpic_hip_hydro_dump(const vpic_hydro_t *h, const vpic_grid_t *g,
+                        const vpic_dump_info_t *info, int32_t sp_id,
+                        float q_m, const vpic_dump_params_t *p, int64_t step);
+
 /* ------------------------------------------------------------------------ */
 /* Set-up helpers (no reference counterpart on the device; they fill the   */
 /* same arrays the reference's set-up code fills on the host)               */

@@ -42,6 +42,58 @@ def arrays(g):
     return f, h, p
 
 
+# The strided dumps (field_dump / hydro_dump): name -> (array, stride,
+# format, output_vars).  Bit 23 of a field mask is the fourth word of the
+# next voxel's record, as the reference reads it.
+STRIDED = {
+    "field_band": ("field", (1, 1, 1), T.DUMP_BAND, 0xffffffff),
+    "field_band_s": ("field", (3, 1, 2), T.DUMP_BAND, 0x00808007),
+    "field_il": ("field", (1, 1, 1), T.DUMP_BAND_INTERLEAVE, 0xffffffff),
+    "field_il_s": ("field", (1, 2, 2), T.DUMP_BAND_INTERLEAVE, 0xffffffff),
+    "hydro_band_s": ("hydro", (3, 2, 1), T.DUMP_BAND, 0x00003fff),
+    "hydro_il": ("hydro", (1, 1, 1), T.DUMP_BAND_INTERLEAVE, 0xffffffff),
+    "hydro_il_s": ("hydro", (3, 1, 2), T.DUMP_BAND_INTERLEAVE, 0xffffffff),
+}
+# global_header's dump parameters: the fields, then two species
+HEADER_MASKS = (0xffffffff, 0x00003fff, 0x00000009)
+
+
+def strided_name(outdir, name):
+    return os.path.join(outdir, name, "T.%d" % STEP, "%s.%d.0" % (name, STEP))
+
+
+def write_strided(outdir):
+    """Writes the STRIDED dumps and the global header; returns their file
+    names."""
+    g = grid()
+    f, h, _ = arrays(g)
+    inf = ctypes.byref(info())
+    G = ctypes.byref(g)
+    names = {}
+    for name, (kind, stride, fmt, mask) in STRIDED.items():
+        base = os.path.join(outdir, name)
+        os.makedirs(base, exist_ok=True)
+        prm = T.dump_params(base, name, stride, fmt, mask)
+        if kind == "field":
+            _lib.call("vpic_hip_field_dump", f.ctypes.data, G, inf,
+                      ctypes.byref(prm), -1)
+        else:
+            _lib.call("vpic_hip_hydro_dump", h.ctypes.data, G, inf, SP_ID,
+                      Q_M, ctypes.byref(prm), -1)
+        names[name] = strided_name(outdir, name)
+    prms = (T.DumpParams * 3)(*[
+        T.dump_params("fields_dir" if k == 0 else "hydro_dir",
+                      ("fields", "ehydro", "Hhydro")[k], (1, 1, 1),
+                      T.DUMP_BAND, m) for k, m in enumerate(HEADER_MASKS)])
+    gi = T.GlobalInfo(ORIGIN[0], ORIGIN[0] + 3.0, ORIGIN[1], ORIGIN[1] + 2.0,
+                      ORIGIN[2], ORIGIN[2] + 2.0, 1, 1, 1, 0)
+    base = os.path.join(outdir, "global")
+    _lib.call("vpic_hip_global_header", base.encode(), G, ctypes.byref(gi),
+              prms, 3)
+    names["global"] = base + ".vpc"
+    return names
+
+
 def write_all(outdir):
     """Writes grid, fields, hydro, particles and energies dumps; returns
     their file names."""

@@ -61,6 +61,24 @@ def main():
             out[kind] = rec
         with open(names["energies"], "rb") as fh:
             out["energies"] = {"sha256": hashlib.sha256(fh.read()).hexdigest()}
+        snames = dump_cases.write_strided(d)
+        for name in dump_cases.STRIDED:
+            fn = snames[name]
+            with open(fn, "rb") as fh:
+                hdr = R.read_header(fh)
+                arr = R.read_array_header(fh, hdr)
+                data = R.read_data(fh, arr["dim"], hdr["floatfmt"],
+                                   int(arr["no_fields"]))
+                rec = {"header": {k: plain(v) for k, v in hdr.items()},
+                       "array": {k: plain(v) for k, v in arr.items()},
+                       "field_sums": [float(np.sum(x, dtype=np.float64))
+                                      for x in data]}
+            with open(fn, "rb") as fh:
+                rec["sha256"] = hashlib.sha256(fh.read()).hexdigest()
+            out["strided_" + name] = rec
+        with open(snames["global"], "rb") as fh:
+            out["global_header"] = {
+                "sha256": hashlib.sha256(fh.read()).hexdigest()}
     with open(os.path.join(HERE, "dump_vectors.json"), "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)
     print("wrote", os.path.join(HERE, "dump_vectors.json"))

@@ -153,3 +153,155 @@ def test_simulation_dumps_gpu(tmp_path, orc):
     assert raw[-g.nv * 80:] == f.tobytes()
     lines = open(str(tmp_path / "energies")).read().splitlines()
     assert len(lines) == 4 and lines[3].startswith("0 ")
+    # the strided dumps of the same state and their global header
+    print("field_dump / hydro_dump", flush=True)
+    os.makedirs(str(tmp_path / "fd"))
+    os.makedirs(str(tmp_path / "hd"))
+    fp = T.dump_params(str(tmp_path / "fd"), "fields", (1, 1, 1),
+                       T.DUMP_BAND_INTERLEAVE)
+    hp = T.dump_params(str(tmp_path / "hd"), "ehydro", (1, 1, 1),
+                       T.DUMP_BAND, 0x3fff)
+    s.field_dump(fp)
+    s.hydro_dump(sp, hp)
+    s.global_header(str(tmp_path / "global"), fp, [hp])
+    raw = open(str(tmp_path / "fd" / "T.0" / "fields.0.0"), "rb").read()
+    assert raw[123:] == f.tobytes()
+    raw = open(str(tmp_path / "hd" / "T.0" / "ehydro.0.0"), "rb").read()
+    hb = np.frombuffer(raw[123:], dtype=np.float32).reshape(14, g.nv)
+    for k, c in enumerate(T.HYDRO_FIELDS[:14]):
+        den = float(np.max(np.abs(hw[c]))) or 1.0
+        assert float(np.max(np.abs(hb[k] - hw[c]))) <= 1e-5 * den, c
+    hdr = open(str(tmp_path / "global.vpc")).read()
+    assert "NUM_OUTPUT_SPECIES 1\n" in hdr and "HYDRO_DATA_VARIABLES 5\n" in hdr
+
+
+def _out_index(nout, n, s, unit, ghosts=True):
+    """dump.cc:643-645: the voxels field_dump / hydro_dump write along an
+    axis (unit strides: all of them as they are)."""
+    if unit:
+        return np.arange(nout + 2 if ghosts else nout)
+    idx = [0] + [k * s - 1 for k in range(1, nout + 1)] + [n + 1]
+    return np.array(idx if ghosts else idx[:nout])
+
+
+def _strided_expect(arr, g, stride, fmt, mask):
+    """The data block of a strided dump, restated with numpy indexing."""
+    rec = arr.dtype.itemsize
+    words = np.frombuffer(arr.tobytes() + b"\0" * 16, dtype=np.uint32)
+    n = (g.nx, g.ny, g.nz)
+    nout = [n[a] // stride[a] for a in range(3)]
+    unit = tuple(stride) == (1, 1, 1)
+    field = rec == T.FIELD.itemsize
+    nwords = 24 if field else 14
+    if fmt == T.DUMP_BAND_INTERLEAVE and unit:
+        nrec = g.nv if field else nout[0] * nout[1] * nout[2]
+        return arr.tobytes()[:nrec * rec]
+    ghosts = field or fmt == T.DUMP_BAND
+    ix, iy, iz = (_out_index(nout[a], n[a], stride[a], unit, ghosts)
+                  for a in range(3))
+    vox = (iz[:, None, None] * (g.nx + 2) * (g.ny + 2) +
+           iy[None, :, None] * (g.nx + 2) + ix[None, None, :])
+    if fmt == T.DUMP_BAND_INTERLEAVE:
+        return arr[vox.reshape(-1)].tobytes()
+    out = []
+    for v in range(nwords):
+        if mask >> v & 1:
+            at = vox * (rec // 4) + v
+            w = words[at]
+            w[at >= len(arr) * rec // 4] = 0  # past the array's last record
+            out.append(w.astype(np.uint32).tobytes())
+    return b"".join(out)
+
+
+def test_strided_dumps_as_the_reference_reader_parses_them(tmp_path):
+    """field_dump / hydro_dump (dump.cc:533-880): the files are rewritten
+    byte-identically to the ones the reference's reader parsed
+    (tests/golden/dump_vectors.json); the header the reader saw carries the
+    strided extents and spacing, the array header the record size and the
+    output extents, and the data block is the band / band_interleave
+    restatement of the arrays above."""
+    vec = json.load(open(os.path.join(GOLD, "dump_vectors.json")))
+    names = dump_cases.write_strided(str(tmp_path))
+    g = dump_cases.grid()
+    f, h, _ = dump_cases.arrays(g)
+    for name, (kind, stride, fmt, mask) in dump_cases.STRIDED.items():
+        raw = open(names[name], "rb").read()
+        ref = vec["strided_" + name]
+        assert hashlib.sha256(raw).hexdigest() == ref["sha256"], name
+        hd, ah = ref["header"], ref["array"]
+        nout = (g.nx // stride[0], g.ny // stride[1], g.nz // stride[2])
+        assert (hd["nxout"], hd["nyout"], hd["nzout"]) == nout, name
+        for a, d in enumerate(("dxout", "dyout", "dzout")):
+            sp = float(np.float32((g.dx, g.dy, g.dz)[a]) *
+                       np.float32(stride[a]))
+            assert hd[d] == sp, (name, d)
+        assert hd["dump_type"] == (1 if kind == "field" else 2), name
+        assert hd["sp_id"] == (-1 if kind == "field" else dump_cases.SP_ID)
+        assert hd["step()"] == dump_cases.STEP
+        arr = f if kind == "field" else h
+        assert ah["sizeof(p[0])"] == arr.dtype.itemsize, name
+        ghosts = kind == "field" or fmt == T.DUMP_BAND
+        assert ah["dim"] == [x + 2 if ghosts else x for x in nout], name
+        want = _strided_expect(arr, g, stride, fmt, mask)
+        assert raw[123:] == want, name
+        nwords = (bin(mask & ((1 << (24 if kind == "field" else 14)) - 1))
+                  .count("1") if fmt == T.DUMP_BAND
+                  else arr.dtype.itemsize // 4)
+        assert ah["no_fields"] == nwords, name
+
+
+def test_global_header(tmp_path):
+    """global_header (dump.cc:405-532): the text the reference writes for
+    this grid, the field variables of a full mask and each species' hydro
+    variables of its mask (groups selected by their first word, names from
+    the reference's tables)."""
+    vec = json.load(open(os.path.join(GOLD, "dump_vectors.json")))
+    names = dump_cases.write_strided(str(tmp_path))
+    raw = open(names["global"], "rb").read()
+    assert hashlib.sha256(raw).hexdigest() == vec["global_header"]["sha256"]
+    lines = raw.decode().splitlines()
+    bar = "#" * 80
+    assert lines[0] == bar and lines[1] == "# Header version information"
+    kv = {}
+    for ln in lines:
+        if ln and not ln.startswith("#") and not ln.startswith('"'):
+            k, _, v = ln.partition(" ")
+            kv.setdefault(k, []).append(v)
+    g = dump_cases.grid()
+    o = dump_cases.ORIGIN
+    assert kv["VPIC_HEADER_VERSION"] == ["1.0.0"]
+    assert kv["DATA_HEADER_SIZE"] == ["123"]
+    assert kv["GRID_DELTA_T"] == ["%f" % g.dt]
+    assert kv["GRID_EXTENTS_X"] == ["%f %f" % (o[0], o[0] + 3.0)]
+    assert kv["GRID_DELTA_Z"] == ["%f" % g.dz]
+    assert kv["GRID_TOPOLOGY_Y"] == ["1"]
+    assert kv["NUM_OUTPUT_SPECIES"] == ["2"]
+    assert kv["FIELD_DATA_VARIABLES"] == ["12"]
+    assert kv["HYDRO_DATA_VARIABLES"] == ["5", "2"]
+    assert kv["SPECIES_DATA_BASE_FILENAME"] == ["ehydro", "Hhydro"]
+    quoted = [ln for ln in lines if ln.startswith('"')]
+    assert quoted[0] == '"Electric Field" VECTOR 3 FLOATING_POINT 4'
+    assert quoted[11] == '"Cell Material" SCALAR 1 INTEGER 2'
+    assert quoted[-2:] == ['"Current Density" VECTOR 3 FLOATING_POINT 4',
+                           '"Charge Density" SCALAR 1 FLOATING_POINT 4']
+    assert not raw.endswith(b"\n\n")
+
+
+def test_strided_dump_errors(tmp_path):
+    g = dump_cases.grid()
+    f, _, _ = dump_cases.arrays(g)
+    inf = ctypes.byref(dump_cases.info())
+    bad = [T.dump_params(str(tmp_path), "f", (2, 1, 1)),  # 2 does not divide 3
+           T.dump_params(str(tmp_path), "f", (1, 1, 0)),
+           # bit 30 without bits 1..23: the reference's list runs short
+           T.dump_params(str(tmp_path), "f", (1, 1, 1), T.DUMP_BAND,
+                         (1 << 30) | 1)]
+    for prm in bad:
+        with pytest.raises(_lib.VpicError):
+            _lib.call("vpic_hip_field_dump", f.ctypes.data, ctypes.byref(g),
+                      inf, ctypes.byref(prm), -1)
+    one = (T.DumpParams * 1)(T.dump_params("a", "b"))
+    gi = T.GlobalInfo(0, 1, 0, 1, 0, 1, 1, 1, 1, 0)
+    with pytest.raises(_lib.VpicError):
+        _lib.call("vpic_hip_global_header", str(tmp_path / "x").encode(),
+                  ctypes.byref(g), ctypes.byref(gi), one, 1)

@@ -74,6 +74,13 @@ _SIGS = {
     "vpic_hip_write_particles": [ctypes.c_char_p, _vp, _i32, _i32, _pg, _pdi,
                                  _i32, _f32],
     "vpic_hip_write_grid": [ctypes.c_char_p, _pg, _pdi, _vp, _vp, _vp],
+    "vpic_hip_global_header": [ctypes.c_char_p, _pg,
+                               ctypes.POINTER(T.GlobalInfo),
+                               ctypes.POINTER(T.DumpParams), _i32],
+    "vpic_hip_field_dump": [_vp, _pg, _pdi, ctypes.POINTER(T.DumpParams),
+                            ctypes.c_int64],
+    "vpic_hip_hydro_dump": [_vp, _pg, _pdi, _i32, _f32,
+                            ctypes.POINTER(T.DumpParams), ctypes.c_int64],
     "vpic_hip_grid_periodic": [_pg, _f64, _f64, _f64, _f64, _f64, _f64, _i32,
                                _i32, _i32, _f32, _f32, _f32],
     "vpic_hip_field_params_vacuum": [_pmp, _f32],

@@ -189,6 +189,14 @@ class Simulation:
     def dump_hydro(self, sp, fbase, ftag=1):
         """dump_hydro (dump.cc:221-270): clear_hydro_array,
         accumulate_hydro_p, synchronize_hydro_array, then the file."""
+        h = self.hydro_moments(sp)
+        _lib.call("vpic_hip_write_hydro", self._fname(fbase, ftag).encode(),
+                  h.ctypes.data, self._G(), ctypes.byref(self._info()),
+                  sp_id(self, sp), np.float32(sp.q / sp.m))
+
+    def hydro_moments(self, sp):
+        """clear_hydro_array, accumulate_hydro_p, synchronize_hydro_array
+        of species sp; returns the host copy of the hydro array."""
         if self.hydro is None:
             self.hydro = device.DeviceArray((self.g.nv,), T.HYDRO)
         part = sp.partition.ptr if sp.last_sorted >= 0 else None
@@ -199,10 +207,34 @@ class Simulation:
                   self.stream)
         _lib.call("vpic_hip_synchronize_hydro_array", self.hydro.ptr,
                   self._G(), self.stream)
-        h = self.hydro.download()
-        _lib.call("vpic_hip_write_hydro", self._fname(fbase, ftag).encode(),
-                  h.ctypes.data, self._G(), ctypes.byref(self._info()),
-                  sp_id(self, sp), np.float32(sp.q / sp.m))
+        return self.hydro.download()
+
+    def global_header(self, base, field_params, species_params):
+        """global_header (dump.cc:405-532): "<base>.vpc" for the strided
+        dumps, field_params and one DumpParams per dumped species."""
+        n = 1 + len(species_params)
+        arr = (T.DumpParams * n)(field_params, *species_params)
+        lo, hi = self.origin, [self.origin[0] + self.g.nx * self.g.dx,
+                               self.origin[1] + self.g.ny * self.g.dy,
+                               self.origin[2] + self.g.nz * self.g.dz]
+        gi = T.GlobalInfo(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 1, 1, 1,
+                          0)
+        _lib.call("vpic_hip_global_header", base.encode(), self._G(),
+                  ctypes.byref(gi), arr, n)
+
+    def field_dump(self, params, step=-1):
+        """field_dump (dump.cc:533-706) of the field array."""
+        f = self.fields.download()
+        _lib.call("vpic_hip_field_dump", f.ctypes.data, self._G(),
+                  ctypes.byref(self._info()), ctypes.byref(params), step)
+
+    def hydro_dump(self, sp, params, step=-1):
+        """hydro_dump (dump.cc:708-880): the species' moments, then the
+        file."""
+        h = self.hydro_moments(sp)
+        _lib.call("vpic_hip_hydro_dump", h.ctypes.data, self._G(),
+                  ctypes.byref(self._info()), sp_id(self, sp),
+                  np.float32(sp.q / sp.m), ctypes.byref(params), step)
 
     def dump_particles(self, sp, fbase, ftag=1, chunk=1 << 24):
         """dump_particles (dump.cc:273-335): the particles time-centred

@@ -91,6 +91,36 @@ class DumpInfo(ctypes.Structure):
                 ("rank", ctypes.c_int32), ("nproc", ctypes.c_int32)]
 
 
+DUMP_BAND = 0
+DUMP_BAND_INTERLEAVE = 1
+
+
+class DumpParams(ctypes.Structure):
+    """vpic_dump_params_t: DumpParameters (src/vpic/vpic.h:92-117) of one
+    strided dump: directory, file base name, strides, format, and the
+    output_vars bit field (bit i = word i of the record)."""
+    _fields_ = [("base_dir", ctypes.c_char_p),
+                ("base_file_name", ctypes.c_char_p),
+                ("stride", ctypes.c_int32 * 3), ("format", ctypes.c_int32),
+                ("output_vars", ctypes.c_uint32)]
+
+
+def dump_params(base_dir, base_file_name, stride=(1, 1, 1), fmt=DUMP_BAND,
+                output_vars=0xffffffff):
+    """A DumpParams with the reference's defaults (all variables, unit
+    strides, band format)."""
+    return DumpParams(base_dir.encode(), base_file_name.encode(),
+                      (ctypes.c_int32 * 3)(*stride), fmt,
+                      output_vars & 0xffffffff)
+
+
+class GlobalInfo(ctypes.Structure):
+    """vpic_global_info_t: the global domain's extents and topology."""
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("x0", "x1", "y0", "y1", "z0", "z1")] + \
+               [(n, ctypes.c_int32) for n in ("px", "py", "pz", "rank")]
+
+
 class FieldParams(ctypes.Structure):
     """vpic_field_params_t."""
     _fields_ = [(n, ctypes.c_float) for n in
