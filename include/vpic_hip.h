@@ -252,6 +252,27 @@ int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int32_t np,
                     int32_t *partition, const vpic_grid_t *g, int in_place,
                     void *stream);
 
+/* The Takizuka-Abe binary collision operator: replaces
+ * apply_takizuka_abe_pipeline (src/collision/pipeline/
+ * takizuka_abe_pipeline.cc:81-222) of takizuka_abe(name, spi, spj, rp,
+ * cvar0, interval) (src/collision/takizuka_abe.cc:50-75).  Species i and j
+ * must be sorted (their partitions from vpic_hip_sort_p), as
+ * apply_takizuka_abe makes sure (takizuka_abe.cc:6-12); the caller applies
+ * the step % interval test there.  pi == pj (with part_i == part_j) is the
+ * intra-species operator.  Every voxel's particles of species i are
+ * shuffled and paired with species j's as the reference does, with the
+ * reference's collision arithmetic; the random numbers come from a
+ * counter-based stream per voxel (Philox4x32-10 with key `seed` and
+ * counter {draw/4, voxel, call}: pass a new `call` each time, e.g. the
+ * step) instead of the reference's per-pipeline SFMT generator, so the
+ * result is deterministic whatever the launch shape.  Synchronous. */
+int vpic_hip_takizuka_abe(vpic_particle_t *pi, const int32_t *part_i,
+                          int32_t np_i, float qi, float mi,
+                          vpic_particle_t *pj, const int32_t *part_j,
+                          int32_t np_j, float qj, float mj, double cvar0,
+                          int32_t interval, uint64_t seed, uint64_t call,
+                          const vpic_grid_t *g, void *stream);
+
 /* boundary_p for one species on a domain with no face shared with another
  * rank (boundary_p.cc:42-750; the loop of :257-375).  pm[0..nm) are the
  * movers advance_p left, in increasing particle order; each one's particle

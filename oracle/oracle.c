@@ -17,6 +17,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -1747,5 +1748,197 @@ void orc_synchronize_hydro(vpic_hydro_t *h, const vpic_grid_t *g) {
           pa[q] = lw * va + rw * vb;
         }
       }
+  }
+}
+
+/* Takizuka-Abe binary collisions -------------------------------------- */
+
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2],
+                    uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+typedef struct {
+  uint32_t key[2], ctr[4], buf[4];
+  uint32_t n; /* draws taken in this voxel */
+} ta_rng_t;
+
+static void ta_rng_voxel(ta_rng_t *r, int v) {
+  r->ctr[1] = (uint32_t)v;
+  r->n = 0;
+}
+
+static uint32_t ta_u32(ta_rng_t *r) {
+  if ((r->n & 3) == 0) {
+    r->ctr[0] = r->n >> 2;
+    orc_philox4x32(r->ctr, r->key, r->buf);
+  }
+  return r->buf[r->n++ & 3];
+}
+
+static float ta_frand_c0(ta_rng_t *r) {
+  return (float)(ta_u32(r) >> 8) * (1.0f / 16777216.0f);
+}
+
+static float ta_frandn(ta_rng_t *r) {
+  const double u1 = ((double)(ta_u32(r) >> 8) + 1.0) * (1.0 / 16777216.0);
+  const double u2 = (double)(ta_u32(r) >> 8) * (1.0 / 16777216.0);
+  return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+/* takizuka_abe_collision (takizuka_abe_pipeline.cc:14-79), with the same
+ * operations in the same order (float, except the double sin/cos of the
+ * scattering phase as there). */
+static void ta_collide(vpic_particle_t *pi, vpic_particle_t *pj, float mu_i,
+                       float mu_j, float std, ta_rng_t *rng) {
+  float dd, ur, urx, ury, urz, tx, ty, tz, t0, t1, t2, wi, wj, stack[3];
+  int d0, d1, d2;
+  urx = pi->ux - pj->ux;
+  ury = pi->uy - pj->uy;
+  urz = pi->uz - pj->uz;
+  wi = pi->w;
+  wj = pj->w;
+  /* the axis of the smallest |ur| component (:30-33) */
+  t0 = urx * urx; d0 = 0; d1 = 1; d2 = 2; t1 = t0; ur = t0;
+  t0 = ury * ury;
+  if (t0 < t1) { d0 = 1; d1 = 2; d2 = 0; t1 = t0; }
+  ur += t0;
+  t0 = urz * urz;
+  if (t0 < t1) { d0 = 2; d1 = 0; d2 = 1; }
+  ur += t0;
+  ur = sqrtf(ur);
+  /* a unit vector perpendicular to ur (:35-46) */
+  stack[0] = urx; stack[1] = ury; stack[2] = urz;
+  t1 = stack[d1];
+  t2 = stack[d2];
+  t0 = 1 / sqrtf(t1 * t1 + t2 * t2 + FLT_MIN);
+  stack[d0] = 0;
+  stack[d1] = t0 * t2;
+  stack[d2] = -t0 * t1;
+  tx = stack[0]; ty = stack[1]; tz = stack[2];
+  /* tan(theta/2) = dd ~ N(0, std^2/ur^3), at most 1 standard unit (:48-52) */
+  t0 = 1;
+  t2 = 1 / ur;
+  t1 = std * sqrtf(t2) * t2;
+  if (t0 < t1) t1 = t0;
+  dd = t1 * ta_frandn(rng);
+  t0 = 2 * dd / (1 + dd * dd);
+  t1 = (float)(2 * M_PI * ta_frand_c0(rng));
+  t2 = (float)(t0 * sin((double)t1));
+  t1 = (float)(t0 * ur * cos((double)t1));
+  t0 *= -dd;
+  /* (1 - cos theta) u + |u| sin theta T (:60-63) */
+  stack[0] = (t0 * urx + t1 * tx) + t2 * (ury * tz - urz * ty);
+  stack[1] = (t0 * ury + t1 * ty) + t2 * (urz * tx - urx * tz);
+  stack[2] = (t0 * urz + t1 * tz) + t2 * (urx * ty - ury * tx);
+  /* unequal weights (:65-70) */
+  t0 = ta_frand_c0(rng);
+  t1 = mu_i;
+  t2 = mu_j;
+  if (wj < wi && wi * t0 > wj) t1 = 0;
+  if (wi < wj && wj * t0 > wi) t2 = 0;
+  pi->ux += t1 * stack[0];
+  pi->uy += t1 * stack[1];
+  pi->uz += t1 * stack[2];
+  pj->ux -= t2 * stack[0];
+  pj->uy -= t2 * stack[1];
+  pj->uz -= t2 * stack[2];
+}
+
+void orc_takizuka_abe(vpic_particle_t *spi_p, const int *spi_partition,
+                      float qi, float mi, vpic_particle_t *spj_p,
+                      const int *spj_partition, float qj, float mj,
+                      double cvar0, int interval, uint64_t seed,
+                      uint64_t call, const vpic_grid_t *g) {
+  const int same = spi_p == spj_p;
+  const float dtinterval_dV = (g->dt * (float)interval) / g->dV;
+  const float mu = (mi * mj) / (mi + mj);
+  const float mu_i = mj / (mi + mj);
+  const float mu_j = mi / (mi + mj);
+  const double cvar = cvar0 * (qi * qi * qj * qj) / (mu * mu);
+  ta_rng_t rng;
+  rng.key[0] = (uint32_t)seed;
+  rng.key[1] = (uint32_t)(seed >> 32);
+  rng.ctr[2] = (uint32_t)call;
+  rng.ctr[3] = (uint32_t)(call >> 32);
+  const int v1 = g->nx + g->sy * g->ny + g->sz * g->nz + 1; /* VOXEL(nx,ny,nz)+1 */
+  for (int v = 0; v < v1; v++) {
+    ta_rng_voxel(&rng, v);
+    int k0 = spi_partition[v];
+    int nk = spi_partition[v + 1] - k0;
+    if (!nk) continue;
+    /* Fisher-Yates shuffle of species i in this voxel, summing its
+     * weights (:113-127) */
+    float density_k = 0, density_l, std;
+    const int k1 = k0 + nk;
+    int i, j, l0, nl;
+    for (i = k0; i < k1 - 1; ++i) {
+      const uint32_t rn = UINT32_MAX / (uint32_t)(k1 - i);
+      do { j = i + (int)(ta_u32(&rng) / rn); } while (j >= k1);
+      const vpic_particle_t t = spi_p[j];
+      spi_p[j] = spi_p[i];
+      spi_p[i] = t;
+      density_k += spi_p[i].w;
+    }
+    density_k += spi_p[i].w;
+    if (same) {
+      /* an odd count: the first three collide pairwise (:131-144) */
+      if (nk % 2 && nk >= 3) {
+        std = sqrtf((float)(0.5 * density_k * cvar * dtinterval_dV));
+        ta_collide(spi_p + k0, spi_p + k0 + 1, mu_i, mu_j, std, &rng);
+        ta_collide(spi_p + k0, spi_p + k0 + 2, mu_i, mu_j, std, &rng);
+        ta_collide(spi_p + k0 + 1, spi_p + k0 + 2, mu_i, mu_j, std, &rng);
+        nk -= 3;
+        k0 += 3;
+      }
+      nl = nk = nk / 2;
+      l0 = k0 + nk;
+      if (!nk) continue;
+      std = sqrtf((float)(cvar * density_k * dtinterval_dV));
+    } else {
+      l0 = spj_partition[v];
+      nl = spj_partition[v + 1] - l0;
+      if (!nl) continue;
+      density_l = 0;
+      for (i = 0; i < nl; ++i) density_l += spj_p[l0 + i].w;
+      std = sqrtf((float)(cvar * (density_l > density_k ? density_k
+                                                        : density_l) *
+                          dtinterval_dV));
+    }
+    /* pair the larger set's particles round the smaller's (:167-212) */
+    int ii, rn;
+    if (nk > nl) {
+      ii = nk / nl;
+      rn = nk - ii * nl;
+      for (i = 0; i < rn; ++i, ++l0)
+        for (j = 0; j <= ii; ++j, ++k0)
+          ta_collide(spi_p + k0, spj_p + l0, mu_i, mu_j, std, &rng);
+      for (; i < nl; ++i, ++l0)
+        for (j = 0; j < ii; ++j, ++k0)
+          ta_collide(spi_p + k0, spj_p + l0, mu_i, mu_j, std, &rng);
+    } else {
+      ii = nl / nk;
+      rn = nl - ii * nk;
+      for (i = 0; i < rn; ++i, ++k0)
+        for (j = 0; j <= ii; ++j, ++l0)
+          ta_collide(spi_p + k0, spj_p + l0, mu_i, mu_j, std, &rng);
+      for (; i < nk; ++i, ++k0)
+        for (j = 0; j < ii; ++j, ++l0)
+          ta_collide(spi_p + k0, spj_p + l0, mu_i, mu_j, std, &rng);
+    }
   }
 }

@@ -182,6 +182,27 @@ int orc_advance_p_threaded(vpic_particle_t *p, int np, float q, float m,
                            const vpic_interpolator_t *f, const vpic_grid_t *g,
                            int n_thread);
 
+/* Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:
+ * as easy as 1, 2, 3", SC'11): out = the 10-round bijection of ctr under
+ * key.  The collision operator's random stream (the device draws the same
+ * numbers). */
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2],
+                    uint32_t out[4]);
+
+/* takizuka_abe_pipeline_scalar (src/collision/pipeline/
+ * takizuka_abe_pipeline.cc:81-215) on one pipeline: for every voxel
+ * 0..VOXEL(nx,ny,nz) the shuffle of species i's particles, then the
+ * binary collisions of takizuka_abe_collision (:14-79).  pi == pj (with
+ * part_i == part_j) is the intra-species operator.  The random numbers
+ * are the collision stream of the device (Philox4x32-10, key = seed,
+ * counter = {draw/4, voxel, lo(call), hi(call)}; uirand = a word,
+ * frand_c0 = (word >> 8) 2^-24, frandn = Box-Muller in double from two
+ * words) in place of the reference's per-pipeline SFMT generator. */
+void orc_takizuka_abe(vpic_particle_t *pi, const int *part_i, float qi,
+                      float mi, vpic_particle_t *pj, const int *part_j,
+                      float qj, float mj, double cvar0, int interval,
+                      uint64_t seed, uint64_t call, const vpic_grid_t *g);
+
 #ifdef __cplusplus
 }
 #endif

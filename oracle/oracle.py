@@ -101,6 +101,13 @@ class Oracle:
         lib.orc_inject_p.argtypes = [_vp, _vp, _i32, _vp, _vp, _i32, _vp,
                                      _i32, _vp, _f32, _pg, _vp]
         lib.orc_inject_p.restype = None
+        lib.orc_philox4x32.argtypes = [_vp, _vp, _vp]
+        lib.orc_philox4x32.restype = None
+        lib.orc_takizuka_abe.argtypes = [_vp, _vp, _f32, _f32, _vp, _vp,
+                                         _f32, _f32, ctypes.c_double, _i32,
+                                         ctypes.c_uint64, ctypes.c_uint64,
+                                         _pg]
+        lib.orc_takizuka_abe.restype = None
         lib.orc_clear_hydro.argtypes = [_vp, _pg]
         lib.orc_clear_hydro.restype = None
         lib.orc_synchronize_hydro.argtypes = [_vp, _pg]
@@ -214,6 +221,24 @@ class Oracle:
                                     q, ctypes.byref(hg.g), _ptr(hits),
                                     _ptr(counts))
         return n, hits[:len(pm)], counts
+
+    def philox4x32(self, ctr, key):
+        c = np.asarray(ctr, dtype=np.uint32)
+        k = np.asarray(key, dtype=np.uint32)
+        out = np.zeros(4, dtype=np.uint32)
+        self.lib.orc_philox4x32(_ptr(c), _ptr(k), _ptr(out))
+        return out
+
+    def takizuka_abe(self, pi, part_i, qi, mi, pj, part_j, qj, mj, cvar0,
+                     interval, seed, call, g):
+        """Collisions of species i with j (pj is pi for one species), in
+        place; part_* are the sort partitions (int32, nv+1)."""
+        same = pj is pi
+        self.lib.orc_takizuka_abe(_ptr(pi), _ptr(part_i), qi, mi,
+                                  _ptr(pi if same else pj),
+                                  _ptr(part_i if same else part_j), qj, mj,
+                                  cvar0, interval, seed, call,
+                                  ctypes.byref(g))
 
     def inject_p(self, p, np_, pm, nm, pi, acc, q, hg):
         """Injection loop: returns (np, nm, counts); p and pm are the

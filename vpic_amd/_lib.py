@@ -36,6 +36,9 @@ _SIGS = {
     "vpic_hip_push_bin_mode": [_i32],
     "vpic_hip_move_p": [_vp, _vp, _i32, _vp, _pg, _f32, _vp, _vp],
     "vpic_hip_sort_p": [_vp, _vp, _i32, _vp, _pg, ctypes.c_int, _vp],
+    "vpic_hip_takizuka_abe": [_vp, _vp, _i32, _f32, _f32, _vp, _vp, _i32,
+                              _f32, _f32, ctypes.c_double, _i32,
+                              ctypes.c_uint64, ctypes.c_uint64, _pg, _vp],
     "vpic_hip_energy_p": [_vp, _i32, _f32, _f32, _vp, _pg, _pf64, _vp],
     "vpic_hip_accumulate_rho_p": [_vp, _vp, _i32, _f32, _pg, _vp],
     "vpic_hip_center_p": [_vp, _i32, _f32, _f32, _vp, _pg, _i32, _vp],

@@ -312,6 +312,25 @@ extern "C" int vpic_hip_inject_p(vpic_particle_t *p, int32_t *np,
                          counts, S(stream));
 }
 
+extern "C" int vpic_hip_takizuka_abe(
+    vpic_particle_t *pi, const int32_t *part_i, int32_t np_i, float qi,
+    float mi, vpic_particle_t *pj, const int32_t *part_j, int32_t np_j,
+    float qj, float mj, double cvar0, int32_t interval, uint64_t seed,
+    uint64_t call, const vpic_grid_t *g, void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(pi && pj && part_i && part_j, "NULL argument");
+  CHECK(np_i >= 0 && np_j >= 0, "negative particle count");
+  CHECK((pi == pj) == (part_i == part_j) && (pi != pj || np_i == np_j),
+        "one species: the same particles, partition and count");
+  CHECK(mi > 0 && mj > 0, "masses must be positive");
+  CHECK(interval >= 1, "interval must be at least 1");
+  CHECK(aligned16(pi) && aligned16(pj), "particle arrays must be 16-byte "
+        "aligned");
+  return launch_takizuka_abe(pi, part_i, np_i, qi, mi, pj, part_j, np_j, qj,
+                             mj, cvar0, interval, seed, call, g, S(stream));
+}
+
 extern "C" int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux,
                                int32_t np, int32_t *partition,
                                const vpic_grid_t *g, int in_place,

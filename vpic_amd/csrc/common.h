@@ -166,6 +166,11 @@ int launch_boundary_p(vpic_particle_t *p, int *np_host,
                       vpic_field_t *f, float q, const vpic_grid_t *g,
                       vpic_boundary_hit_t *hits, int *counts_host,
                       hipStream_t s);
+int launch_takizuka_abe(vpic_particle_t *pi, const int *part_i, int np_i,
+                        float qi, float mi, vpic_particle_t *pj,
+                        const int *part_j, int np_j, float qj, float mj,
+                        double cvar0, int interval, uint64_t seed,
+                        uint64_t call, const vpic_grid_t *g, hipStream_t s);
 int launch_inject_p(vpic_particle_t *p, int *np, int max_np,
                     vpic_particle_mover_t *pm, int *nm, int max_nm,
                     const vpic_particle_injector_t *pi, int n,

@@ -49,6 +49,7 @@ class Simulation:
         self.g = grid
         self.origin = tuple(float(v) for v in origin)  # grid_t.x0, y0, z0
         self.hydro = None  # allocated by the first dump_hydro
+        self.collisions = []  # define_collision_op, applied each step
         self.mp = field_params or T.vacuum_params()
         self.fields = device.fields(grid)
         self.fields.zero()
@@ -79,6 +80,28 @@ class Simulation:
                   sp.partition.ptr, self._G(), 0, self.stream)
         sp.p, self.aux = self.aux, sp.p
         sp.last_sorted = self.step
+
+    def takizuka_abe(self, spi, spj, cvar0, interval, seed=0):
+        """takizuka_abe(name, spi, spj, rp, cvar0, interval)
+        (src/collision/takizuka_abe.cc:50-75) added to the step's collision
+        operators (define_collision_op): every interval steps, both species
+        sorted at this step first (apply_takizuka_abe, takizuka_abe.cc:6-12),
+        then vpic_hip_takizuka_abe with the step as the stream's counter."""
+        if interval < 1:
+            raise _lib.VpicError("takizuka_abe: interval must be >= 1")
+
+        def apply(sim):
+            if sim.step % interval:
+                return
+            for sp in (spi, spj):
+                if sp.last_sorted != sim.step:
+                    sim.sort_p(sp)
+            _lib.call("vpic_hip_takizuka_abe", spi.p.ptr, spi.partition.ptr,
+                      spi.np, spi.q, spi.m, spj.p.ptr, spj.partition.ptr,
+                      spj.np, spj.q, spj.m, float(cvar0), int(interval),
+                      int(seed), int(sim.step), sim._G(), sim.stream)
+        self.collisions.append(apply)
+        return apply
 
     def advance_p(self, sp):
         # after a sort the partition drives the tiled (LDS) push
@@ -279,6 +302,9 @@ class Simulation:
             for sp in self.species:
                 if sp.sort_interval > 0 and self.step % sp.sort_interval == 0:
                     self.sort_p(sp)
+        # advance.cc:39-46: the collision operators, after the sorts
+        for op in self.collisions:
+            op(self)
         with t("push"):
             self.clear_accumulator_array()
             for sp in self.species:
