@@ -13,7 +13,10 @@
  * branches), the method of images: a walled box against the periodic box
  * of twice its length with mirrored fields; the pin of each function is
  * listed in tests/test_oracle_*.  absorb_fields walls are unpinned (the
- * device path does not support them).
+ * device path does not support them).  orc_takizuka_abe draws from the
+ * device's Philox stream, not the reference's SFMT generator: its
+ * arithmetic is pinned by conservation and scattering statistics, its
+ * parity with the reference's outputs is unpinned.
  */
 #include "oracle.h"
 
