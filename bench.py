@@ -138,6 +138,10 @@ def main():
                     help="only time an in-place read+write of 13.4 GB (one "
                          "species' particle array) with a plain elementwise "
                          "kernel: the stream rate the push is compared to")
+    ap.add_argument("--collision-probe", action="store_true",
+                    help="only time the Takizuka-Abe operator on the sorted "
+                         "plasma (electron-ion and electron-electron), one "
+                         "JSON line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 4 GiB device-to-device copy three times "
                          "(calibrates the HBM byte counters)")
@@ -163,6 +167,35 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 4 * n,
                           "copy_probe_bytes_written": 4 * n}))
+        return
+    if args.collision_probe:
+        # the benchmark plasma, both species sorted; each operator timed
+        # over 5 calls (a new stream counter per call) after one warm-up
+        s = build_sim(args.nx, args.ny, args.nz, args.ppc)
+        el, io = s.species
+        for sp in (el, io):
+            s.sort_p(sp)
+        torch.cuda.synchronize()
+        out = {"collision_probe": "takizuka_abe", "cells": args.nx * args.ny *
+               args.nz, "ppc": args.ppc}
+        for name, a, b in (("electron_ion", el, io),
+                           ("electron_electron", el, el)):
+            def call(k):
+                _lib.call("vpic_hip_takizuka_abe", a.p.ptr, a.partition.ptr,
+                          a.np, a.q, a.m, b.p.ptr, b.partition.ptr, b.np,
+                          b.q, b.m, 1e-3, 1, 7, k, s._G(), s.stream)
+            call(0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(1, 6):
+                call(k)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / 5 * 1e3
+            npairs = a.np if a is not b else a.np // 2
+            out[name] = {"ms_per_call": ms,
+                         "collisions_per_s": npairs / (ms * 1e-3)}
+            log("%s: %.2f ms per call" % (name, ms))
+        print(json.dumps(out))
         return
     if args.stream_probe:
         n = 419430400 * 8  # floats: 32 B x 419,430,400 particles
