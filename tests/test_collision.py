@@ -225,3 +225,54 @@ def test_takizuka_abe_gpu_rejects_bad_partition(dev):
         _lib.call("vpic_hip_takizuka_abe", dp.ptr, db.ptr, len(p), -1.0, 1.0,
                   dp.ptr, db.ptr, len(p), -1.0, 1.0, 0.5, 1, 1, 0,
                   ctypes.byref(g), None)
+
+
+@pytest.mark.gpu
+def test_simulation_collision_operator_gpu(dev):
+    """Simulation.takizuka_abe (define_collision_op): at a collision step
+    both species are sorted and the operator runs; the plasma's total
+    momentum and kinetic energy (equal weights) are conserved, the
+    partition still indexes every particle, and a step that is not a
+    multiple of the interval leaves the particles alone."""
+    import torch
+    from vpic_amd import sim as S
+    g = T.grid_periodic(0, 0, 0, 8, 6, 4, 8, 6, 4, 0.2)
+    s = S.Simulation(g)
+    S.uniform_plasma(s, 20, -1.0, 1.0, 0.1, 5, "electron", w=0.05,
+                     sort_interval=0)
+    S.uniform_plasma(s, 20, 1.0, 25.0, 0.02, 6, "ion", w=0.05,
+                     sort_interval=0)
+    el, io = s.species
+    op = s.takizuka_abe(el, io, 0.5, 2, seed=3)
+
+    def totals():
+        mom, ke = np.zeros(3), 0.0
+        for sp in (el, io):
+            p = sp.p.download()[:sp.np]
+            u = np.stack([p["ux"], p["uy"], p["uz"]], 1).astype(np.float64)
+            mom += sp.m * u.sum(0)
+            ke += 0.5 * sp.m * float((u ** 2).sum())
+        return mom, ke
+
+    s.step = 1  # not a collision step
+    m0, k0 = totals()
+    before = el.p.download()[:el.np].copy()
+    op(s)
+    torch.cuda.synchronize()
+    assert H.same_bits(el.p.download()[:el.np], before)
+    s.step = 2
+    print("collision step", flush=True)
+    op(s)
+    torch.cuda.synchronize()
+    assert el.last_sorted == 2 and io.last_sorted == 2
+    m1, k1 = totals()
+    scale = sum(sp.m * sp.np * 0.1 for sp in (el, io))
+    assert np.abs(m1 - m0).max() <= 1e-5 * scale
+    assert abs(k1 - k0) <= 1e-5 * k0
+    assert not H.same_bits(el.p.download()[:el.np], before)
+    for sp in (el, io):
+        part = sp.partition.download()
+        p = sp.p.download()[:sp.np]
+        assert part[g.nv] == sp.np
+        v = np.repeat(np.arange(g.nv), np.diff(part.astype(np.int64)))
+        assert np.array_equal(p["i"], v)
