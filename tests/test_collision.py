@@ -209,6 +209,37 @@ def test_takizuka_abe_gpu_bitexact(dev, orc):
 
 
 @pytest.mark.gpu
+def test_takizuka_abe_gpu_crowded_voxels(dev, orc):
+    """Voxels with 256, 257 and 300 particles of species i and a species j
+    much larger or smaller than i, bit-exact against the oracle."""
+    g = _grid(4, 3, 3)
+    rng = np.random.default_rng(5)
+    ce = _counts(g, rng, 0, 9)
+    ci = _counts(g, rng, 0, 40)
+    inner = np.nonzero(ce)[0]
+    ce[inner[0]], ce[inner[1]], ce[inner[2]] = 256, 257, 300
+    ci[inner[2]] = 3
+    pe, parte = _species(g, ce, rng, 0.1)
+    pi, parti = _species(g, ci, rng, 0.02)
+    de, dpe = dev.DeviceArray.from_host(pe), dev.DeviceArray.from_host(parte)
+    di, dpi = dev.DeviceArray.from_host(pi), dev.DeviceArray.from_host(parti)
+    G = ctypes.byref(g)
+    print("takizuka_abe crowded, intra", flush=True)
+    orc.takizuka_abe(pe, parte, -1.0, 1.0, pe, parte, -1.0, 1.0, 0.5, 1, 99,
+                     3, g)
+    _lib.call("vpic_hip_takizuka_abe", de.ptr, dpe.ptr, len(pe), -1.0, 1.0,
+              de.ptr, dpe.ptr, len(pe), -1.0, 1.0, 0.5, 1, 99, 3, G, None)
+    assert H.same_bits(de.download(), pe)
+    print("takizuka_abe crowded, inter", flush=True)
+    orc.takizuka_abe(pe, parte, -1.0, 1.0, pi, parti, 1.0, 25.0, 0.5, 1, 99,
+                     4, g)
+    _lib.call("vpic_hip_takizuka_abe", de.ptr, dpe.ptr, len(pe), -1.0, 1.0,
+              di.ptr, dpi.ptr, len(pi), 1.0, 25.0, 0.5, 1, 99, 4, G, None)
+    assert H.same_bits(de.download(), pe)
+    assert H.same_bits(di.download(), pi)
+
+
+@pytest.mark.gpu
 def test_takizuka_abe_gpu_rejects_bad_partition(dev):
     """A partition that indexes past the species' particles is reported,
     not followed."""
