@@ -19,7 +19,8 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Build only when nothing is built yet (the driver builds explicitly)."""
+    """Build only when nothing is built yet (after a source change, run
+    __graft_entry__.build() first)."""
     from vpic_amd import _lib
     oracle_so = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(oracle_so) or not os.path.exists(_lib.LIB_PATH):
