@@ -261,11 +261,14 @@ int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int32_t np,
  * the step % interval test there.  pi == pj (with part_i == part_j) is the
  * intra-species operator.  Every voxel's particles of species i are
  * shuffled and paired with species j's as the reference does, with the
- * reference's collision arithmetic; the random numbers come from a
- * counter-based stream per voxel (Philox4x32-10 with key `seed` and
- * counter {draw/4, voxel, call}: pass a new `call` each time, e.g. the
- * step) instead of the reference's per-pipeline SFMT generator, so the
- * result is deterministic whatever the launch shape.  Synchronous. */
+ * reference's collision arithmetic (its float sin and cos of the
+ * scattering phase taken as the correctly rounded float values); the
+ * random numbers come from a counter-based stream per voxel
+ * (Philox4x32-10 with key `seed` and counter {draw/4, voxel, call}: pass a
+ * new `call` each time, e.g. the step, and a different `seed` to each
+ * operator applied at the same step, or their draws coincide) instead of
+ * the reference's per-pipeline SFMT generator, so the result is
+ * deterministic whatever the launch shape.  Synchronous. */
 int vpic_hip_takizuka_abe(vpic_particle_t *pi, const int32_t *part_i,
                           int32_t np_i, float qi, float mi,
                           vpic_particle_t *pj, const int32_t *part_j,

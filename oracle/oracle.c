@@ -126,291 +126,48 @@ void orc_neighbor_table(const vpic_grid_t *g, int64_t *nb) {
       }
 }
 
-/* ------------------------------------------------------------------------ */
-/* move_p (move_p.cc:215-378)                                               */
-/* ------------------------------------------------------------------------ */
+/* Face k in -x,-y,-z,+x,+y,+z order -> axis and side. */
+#define FACE_AXIS(k) ((k) % 3)
+#define FACE_HIGH(k) ((k) >= 3)
 
-/* Deposit of one streak; the twelve updates of move_p.cc:281-303 and
- * advance_p_pipeline.cc:186-208 (identical arithmetic). */
-#define ORC_ACC(a4, qd, mY, mZ, v5)     \
-  do {                                  \
-    float v0_, v1_, v2_, v3_, v4_;      \
-    v4_ = (qd);                         \
-    v1_ = v4_ * (mY);                   \
-    v0_ = v4_ - v1_;                    \
-    v1_ += v4_;                         \
-    v4_ = 1 + (mZ);                     \
-    v2_ = v0_ * v4_;                    \
-    v3_ = v1_ * v4_;                    \
-    v4_ = 1 - (mZ);                     \
-    v0_ *= v4_;                         \
-    v1_ *= v4_;                         \
-    v0_ += (v5);                        \
-    v1_ -= (v5);                        \
-    v2_ -= (v5);                        \
-    v3_ += (v5);                        \
-    (a4)[0] += v0_;                     \
-    (a4)[1] += v1_;                     \
-    (a4)[2] += v2_;                     \
-    (a4)[3] += v3_;                     \
-  } while (0)
+/* The step functions: FP32 (orc_*), then the same code in FP64 (orc64_*). */
+#define ORC_FP64 0
+#include "oracle_step.inc"
+#undef ORC_FP64
+#define ORC_FP64 1
+#include "oracle_step.inc"
+#undef ORC_FP64
 
-int orc_move_p(vpic_particle_t *p0, vpic_particle_mover_t *pm,
-               vpic_accumulator_t *a0, const vpic_grid_t *g, float qsp) {
-  float s_midx, s_midy, s_midz, s_dispx, s_dispy, s_dispz, s_dir[3];
-  float v0, v1, v2, v3, v5, q;
-  int axis, face;
-  int64_t neighbor;
-  float *a;
-  vpic_particle_t *p = p0 + pm->i;
-
-  q = qsp * p->w;
-  for (;;) {
-    s_midx = p->dx;
-    s_midy = p->dy;
-    s_midz = p->dz;
-    s_dispx = pm->dispx;
-    s_dispy = pm->dispy;
-    s_dispz = pm->dispz;
-    s_dir[0] = (s_dispx > 0.0f) ? 1.0f : -1.0f;
-    s_dir[1] = (s_dispy > 0.0f) ? 1.0f : -1.0f;
-    s_dir[2] = (s_dispz > 0.0f) ? 1.0f : -1.0f;
-    v0 = (s_dispx == 0.0f) ? 3.4e38f : (s_dir[0] - s_midx) / s_dispx;
-    v1 = (s_dispy == 0.0f) ? 3.4e38f : (s_dir[1] - s_midy) / s_dispy;
-    v2 = (s_dispz == 0.0f) ? 3.4e38f : (s_dir[2] - s_midz) / s_dispz;
-    v3 = 2.0f, axis = 3;
-    if (v0 < v3) v3 = v0, axis = 0;
-    if (v1 < v3) v3 = v1, axis = 1;
-    if (v2 < v3) v3 = v2, axis = 2;
-    v3 *= 0.5f;
-    s_dispx *= v3;
-    s_dispy *= v3;
-    s_dispz *= v3;
-    s_midx += s_dispx;
-    s_midy += s_dispy;
-    s_midz += s_dispz;
-    /* move_p.cc:277: the product is promoted to double by 1.0/3.0. */
-    v5 = (float)((double)(q * s_dispx * s_dispy * s_dispz) * (1.0 / 3.0));
-    a = (float *)(a0 + p->i);
-    ORC_ACC(a + 0, q * s_dispx, s_midy, s_midz, v5);
-    ORC_ACC(a + 4, q * s_dispy, s_midz, s_midx, v5);
-    ORC_ACC(a + 8, q * s_dispz, s_midx, s_midy, v5);
-    pm->dispx -= s_dispx;
-    pm->dispy -= s_dispy;
-    pm->dispz -= s_dispz;
-    p->dx += s_dispx + s_dispx;
-    p->dy += s_dispy + s_dispy;
-    p->dz += s_dispz + s_dispz;
-    if (axis == 3) break;
-    v0 = s_dir[axis];
-    (&(p->dx))[axis] = v0;
-    face = axis;
-    if (v0 > 0.0f) face += 3;
-    neighbor = g->neighbor[6 * (int64_t)p->i + face];
-    if (neighbor == VPIC_BC_REFLECT_PARTICLES) {
-      (&(p->ux))[axis] = -(&(p->ux))[axis];
-      (&(pm->dispx))[axis] = -(&(pm->dispx))[axis];
-      continue;
-    }
-    if (neighbor < g->rangel || neighbor > g->rangeh) {
-      p->i = 8 * p->i + face;
-      return 1;
-    }
-    p->i = (int32_t)(neighbor - g->rangel);
-    (&(p->dx))[axis] = -v0;
+void orc64_grid_from(orc64_grid_t *g64, const vpic_grid_t *g) {
+  memset(g64, 0, sizeof(*g64));
+  g64->dt = g->dt;
+  g64->cvac = g->cvac;
+  g64->eps0 = g->eps0;
+  g64->dx = g->dx;
+  g64->dy = g->dy;
+  g64->dz = g->dz;
+  g64->dV = g->dV;
+  g64->rdx = g->rdx;
+  g64->rdy = g->rdy;
+  g64->rdz = g->rdz;
+  g64->r8V = g->r8V;
+  g64->nx = g->nx;
+  g64->ny = g->ny;
+  g64->nz = g->nz;
+  g64->sx = g->sx;
+  g64->sy = g->sy;
+  g64->sz = g->sz;
+  g64->nv = g->nv;
+  g64->rangel = g->rangel;
+  g64->rangeh = g->rangeh;
+  g64->neighbor = g->neighbor;
+  for (int k = 0; k < 6; k++) {
+    g64->pbc[k] = g->pbc[k];
+    g64->fbc[k] = g->fbc[k];
   }
-  return 0;
 }
 
-/* ------------------------------------------------------------------------ */
-/* advance_p (advance_p_pipeline.cc:21-245,253-340)                         */
-/* ------------------------------------------------------------------------ */
-
-void orc_push_coefficients(float q, float m, const vpic_grid_t *g,
-                           float out[4]) {
-  out[0] = (q * g->dt) / (2 * m * g->cvac);
-  out[1] = g->cvac * g->dt * g->rdx;
-  out[2] = g->cvac * g->dt * g->rdy;
-  out[3] = g->cvac * g->dt * g->rdz;
-}
-
-typedef struct {
-  vpic_particle_t *p0;
-  vpic_particle_mover_t *pm;
-  vpic_accumulator_t *a0;
-  const vpic_interpolator_t *f0;
-  const vpic_grid_t *g;
-  float qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
-  int np, max_nm;
-  int64_t stride;
-  /* per-pipeline results (index = pipeline rank) */
-  vpic_particle_mover_t *seg_pm[257];
-  int seg_nm[257], seg_ignored[257];
-} orc_push_args_t;
-
-static void orc_push_pipeline(orc_push_args_t *args, int rank, int P) {
-  const float qdt_2mc = args->qdt_2mc, cdt_dx = args->cdt_dx,
-              cdt_dy = args->cdt_dy, cdt_dz = args->cdt_dz, qsp = args->qsp;
-  const float one = 1.0, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
-  vpic_particle_t *p0 = args->p0, *p;
-  vpic_accumulator_t *a0 = args->a0;
-  vpic_particle_mover_t *pm, local_pm;
-  float dx, dy, dz, ux, uy, uz, q, hax, hay, haz, cbx, cby, cbz;
-  float v0, v1, v2, v3, v4, v5;
-  int ii, itmp, n, nm, max_nm;
-
-  distribute(args->np, 16, rank, P, &itmp, &n);
-  p = p0 + itmp;
-  max_nm = args->max_nm - (args->np & 15);
-  if (max_nm < 0) max_nm = 0;
-  distribute(max_nm, 8, rank, P, &itmp, &max_nm);
-  if (rank == P) max_nm = args->max_nm - itmp;
-  pm = args->pm + itmp;
-  nm = 0;
-  itmp = 0;
-  if (rank != P) a0 += (int64_t)(1 + rank) * args->stride;
-
-  for (; n; n--, p++) {
-    const vpic_interpolator_t *f;
-    dx = p->dx;
-    dy = p->dy;
-    dz = p->dz;
-    ii = p->i;
-    f = args->f0 + ii;
-    hax = qdt_2mc * ((f->ex + dy * f->dexdy) + dz * (f->dexdz + dy * f->d2exdydz));
-    hay = qdt_2mc * ((f->ey + dz * f->deydz) + dx * (f->deydx + dz * f->d2eydzdx));
-    haz = qdt_2mc * ((f->ez + dx * f->dezdx) + dy * (f->dezdy + dx * f->d2ezdxdy));
-    cbx = f->cbx + dx * f->dcbxdx;
-    cby = f->cby + dy * f->dcbydy;
-    cbz = f->cbz + dz * f->dcbzdz;
-    ux = p->ux;
-    uy = p->uy;
-    uz = p->uz;
-    q = p->w;
-    ux += hax;
-    uy += hay;
-    uz += haz;
-    v0 = qdt_2mc / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
-    v1 = cbx * cbx + (cby * cby + cbz * cbz);
-    v2 = (v0 * v0) * v1;
-    v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
-    v4 = v3 / (one + v1 * (v3 * v3));
-    v4 += v4;
-    v0 = ux + v3 * (uy * cbz - uz * cby);
-    v1 = uy + v3 * (uz * cbx - ux * cbz);
-    v2 = uz + v3 * (ux * cby - uy * cbx);
-    ux += v4 * (v1 * cbz - v2 * cby);
-    uy += v4 * (v2 * cbx - v0 * cbz);
-    uz += v4 * (v0 * cby - v1 * cbx);
-    ux += hax;
-    uy += hay;
-    uz += haz;
-    p->ux = ux;
-    p->uy = uy;
-    p->uz = uz;
-    v0 = one / sqrtf(one + (ux * ux + (uy * uy + uz * uz)));
-    ux *= cdt_dx;
-    uy *= cdt_dy;
-    uz *= cdt_dz;
-    ux *= v0;
-    uy *= v0;
-    uz *= v0;
-    v0 = dx + ux;
-    v1 = dy + uy;
-    v2 = dz + uz;
-    v3 = v0 + ux;
-    v4 = v1 + uy;
-    v5 = v2 + uz;
-    if (v3 <= one && v4 <= one && v5 <= one && -v3 <= one && -v4 <= one &&
-        -v5 <= one) {
-      float *a;
-      q *= qsp;
-      p->dx = v3;
-      p->dy = v4;
-      p->dz = v5;
-      dx = v0;
-      dy = v1;
-      dz = v2;
-      v5 = q * ux * uy * uz * one_third;
-      a = (float *)(a0 + ii);
-      ORC_ACC(a + 0, q * ux, dy, dz, v5);
-      ORC_ACC(a + 4, q * uy, dz, dx, v5);
-      ORC_ACC(a + 8, q * uz, dx, dy, v5);
-    } else {
-      local_pm.dispx = ux;
-      local_pm.dispy = uy;
-      local_pm.dispz = uz;
-      local_pm.i = (int32_t)(p - p0);
-      if (orc_move_p(p0, &local_pm, a0, args->g, qsp)) {
-        if (nm < max_nm)
-          pm[nm++] = local_pm;
-        else {
-          itmp++;
-          p->i = p->i >> 3;
-        }
-      }
-    }
-  }
-  args->seg_pm[rank] = pm;
-  args->seg_nm[rank] = nm;
-  args->seg_ignored[rank] = itmp;
-}
-
-/* Mover compaction of advance_p_pipeline.cc:310-339. */
-static int orc_push_compact(orc_push_args_t *args, int P, int *n_ignored) {
-  int nm = 0, ign = 0;
-  for (int r = 0; r <= P; r++) {
-    ign += args->seg_ignored[r];
-    if (args->pm + nm != args->seg_pm[r])
-      memmove(args->pm + nm, args->seg_pm[r],
-              sizeof(vpic_particle_mover_t) * (size_t)args->seg_nm[r]);
-    nm += args->seg_nm[r];
-  }
-  if (n_ignored) *n_ignored = ign;
-  return nm;
-}
-
-int orc_advance_p(vpic_particle_t *p, int np, float q, float m,
-                  vpic_particle_mover_t *pm, int max_nm, int *n_ignored,
-                  vpic_accumulator_t *a, const vpic_interpolator_t *f,
-                  const vpic_grid_t *g) {
-  float c[4];
-  orc_push_args_t args;
-  int64_t nv = g->nv;
-  vpic_accumulator_t *acc =
-      (vpic_accumulator_t *)calloc((size_t)(2 * nv), sizeof(*acc));
-  float *a0, *a1;
-  int nm;
-
-  orc_push_coefficients(q, m, g, c);
-  memset(&args, 0, sizeof(args));
-  args.p0 = p;
-  args.pm = pm;
-  args.a0 = acc;
-  args.f0 = f;
-  args.g = g;
-  args.qdt_2mc = c[0];
-  args.cdt_dx = c[1];
-  args.cdt_dy = c[2];
-  args.cdt_dz = c[3];
-  args.qsp = q;
-  args.np = np;
-  args.max_nm = max_nm;
-  args.stride = nv;
-  /* Copy 0 starts from the caller's accumulator (the host pipeline's
-   * copy is a->a[0] in the reference). */
-  memcpy(acc, a, sizeof(*acc) * (size_t)nv);
-  orc_push_pipeline(&args, 0, 1); /* pipeline 0 -> copy 1 */
-  orc_push_pipeline(&args, 1, 1); /* host tail   -> copy 0 */
-  nm = orc_push_compact(&args, 1, n_ignored);
-  /* reduce_array_pipeline.cc O1: a0 = a0 + a1. */
-  a0 = (float *)acc;
-  a1 = (float *)(acc + nv);
-  for (int64_t k = 0; k < 12 * nv; k++) a0[k] = a0[k] + a1[k];
-  memcpy(a, acc, sizeof(*acc) * (size_t)nv);
-  free(acc);
-  return nm;
-}
+/* Float-only parts below use the FP32 instantiation's names. */
 
 /* Threaded CPU baseline ------------------------------------------------- */
 
@@ -470,436 +227,6 @@ int orc_advance_p_threaded(vpic_particle_t *p, int np, float q, float m,
   return orc_push_compact(&args, n_thread, 0);
 }
 
-/* ------------------------------------------------------------------------ */
-/* Interpolator / accumulator (sf_interface)                                */
-/* ------------------------------------------------------------------------ */
-
-void orc_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
-                           const vpic_grid_t *g) {
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const float fourth = 0.25, half = 0.50;
-  float w0, w1, w2, w3;
-  for (int z = 1; z <= nz; z++)
-    for (int y = 1; y <= ny; y++)
-      for (int x = 1; x <= nx; x++) {
-        vpic_interpolator_t *pi = fi + VOX(x, y, z, nx, ny);
-        const vpic_field_t *pf0 = f + VOX(x, y, z, nx, ny);
-        const vpic_field_t *pfx = f + VOX(x + 1, y, z, nx, ny);
-        const vpic_field_t *pfy = f + VOX(x, y + 1, z, nx, ny);
-        const vpic_field_t *pfz = f + VOX(x, y, z + 1, nx, ny);
-        const vpic_field_t *pfyz = f + VOX(x, y + 1, z + 1, nx, ny);
-        const vpic_field_t *pfzx = f + VOX(x + 1, y, z + 1, nx, ny);
-        const vpic_field_t *pfxy = f + VOX(x + 1, y + 1, z, nx, ny);
-        w0 = pf0->ex; w1 = pfy->ex; w2 = pfz->ex; w3 = pfyz->ex;
-        pi->ex = fourth * ((w3 + w0) + (w1 + w2));
-        pi->dexdy = fourth * ((w3 - w0) + (w1 - w2));
-        pi->dexdz = fourth * ((w3 - w0) - (w1 - w2));
-        pi->d2exdydz = fourth * ((w3 + w0) - (w1 + w2));
-        w0 = pf0->ey; w1 = pfz->ey; w2 = pfx->ey; w3 = pfzx->ey;
-        pi->ey = fourth * ((w3 + w0) + (w1 + w2));
-        pi->deydz = fourth * ((w3 - w0) + (w1 - w2));
-        pi->deydx = fourth * ((w3 - w0) - (w1 - w2));
-        pi->d2eydzdx = fourth * ((w3 + w0) - (w1 + w2));
-        w0 = pf0->ez; w1 = pfx->ez; w2 = pfy->ez; w3 = pfxy->ez;
-        pi->ez = fourth * ((w3 + w0) + (w1 + w2));
-        pi->dezdx = fourth * ((w3 - w0) + (w1 - w2));
-        pi->dezdy = fourth * ((w3 - w0) - (w1 - w2));
-        pi->d2ezdxdy = fourth * ((w3 + w0) - (w1 + w2));
-        w0 = pf0->cbx; w1 = pfx->cbx;
-        pi->cbx = half * (w1 + w0);
-        pi->dcbxdx = half * (w1 - w0);
-        w0 = pf0->cby; w1 = pfy->cby;
-        pi->cby = half * (w1 + w0);
-        pi->dcbydy = half * (w1 - w0);
-        w0 = pf0->cbz; w1 = pfz->cbz;
-        pi->cbz = half * (w1 + w0);
-        pi->dcbzdz = half * (w1 - w0);
-      }
-}
-
-void orc_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
-                            const vpic_grid_t *g) {
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const float cx = 0.25 * g->rdy * g->rdz / g->dt;
-  const float cy = 0.25 * g->rdz * g->rdx / g->dt;
-  const float cz = 0.25 * g->rdx * g->rdy / g->dt;
-  for (int z = 1; z <= nz + 1; z++)
-    for (int y = 1; y <= ny + 1; y++)
-      for (int x = 1; x <= nx + 1; x++) {
-        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
-        const vpic_accumulator_t *a0 = a + VOX(x, y, z, nx, ny);
-        const vpic_accumulator_t *ax = a + VOX(x - 1, y, z, nx, ny);
-        const vpic_accumulator_t *ay = a + VOX(x, y - 1, z, nx, ny);
-        const vpic_accumulator_t *az = a + VOX(x, y, z - 1, nx, ny);
-        const vpic_accumulator_t *ayz = a + VOX(x, y - 1, z - 1, nx, ny);
-        const vpic_accumulator_t *azx = a + VOX(x - 1, y, z - 1, nx, ny);
-        const vpic_accumulator_t *axy = a + VOX(x - 1, y - 1, z, nx, ny);
-        f0->jfx += cx * (a0->jx[0] + ay->jx[1] + az->jx[2] + ayz->jx[3]);
-        f0->jfy += cy * (a0->jy[0] + az->jy[1] + ax->jy[2] + azx->jy[3]);
-        f0->jfz += cz * (a0->jz[0] + ax->jz[1] + ay->jz[2] + axy->jz[3]);
-      }
-}
-
-void orc_clear_jf(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int v = 0; v < g->nv; v++) f[v].jfx = f[v].jfy = f[v].jfz = 0;
-}
-
-void orc_clear_rhof(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int v = 0; v < g->nv; v++) f[v].rhof = 0;
-}
-
-/* ------------------------------------------------------------------------ */
-/* Field boundary helpers (standard/local.cc, standard/remote.cc)           */
-/* ------------------------------------------------------------------------ */
-
-/* Face k in -x,-y,-z,+x,+y,+z order -> axis and side. */
-#define FACE_AXIS(k) ((k) % 3)
-#define FACE_HIGH(k) ((k) >= 3)
-
-/* Coordinates (x,y,z) addressed through an axis permutation: c[X], c[Y],
- * c[Z] with (X,Y,Z) the cyclic rotation starting at `ax`. */
-static int fvox(const vpic_grid_t *g, const int c[3]) {
-  return VOX(c[0], c[1], c[2], g->nx, g->ny);
-}
-
-static int nd(const vpic_grid_t *g, int axis) {
-  return axis == 0 ? g->nx : axis == 1 ? g->ny : g->nz;
-}
-
-static float *fcomp(vpic_field_t *f, int which, int axis) {
-  /* which: 0 = e, 1 = cb, 2 = tca, 3 = jf */
-  float *base = which == 0 ? &f->ex : which == 1 ? &f->cbx
-                : which == 2 ? &f->tcax : &f->jfx;
-  return base + axis;
-}
-
-/* Iterate the "Y-oriented edges" of plane `pl` on the face normal to X:
- * X=plane, Y in 1..nY, Z in 1..nZ+1 (yz_EDGE_LOOP, local.cc:30). */
-typedef void (*edge_fn)(vpic_field_t *f, const vpic_grid_t *g, int v,
-                        int vsrc, void *ctx);
-
-static void edge_loop(vpic_field_t *f, const vpic_grid_t *g, int X, int Y,
-                      int pl, int yhi_extra, int zhi_extra, int src_off,
-                      edge_fn fn, void *ctx) {
-  int Z = 3 - X - Y;
-  int c[3], s[3];
-  for (int zz = 1; zz <= nd(g, Z) + zhi_extra; zz++)
-    for (int yy = 1; yy <= nd(g, Y) + yhi_extra; yy++) {
-      c[X] = pl;
-      c[Y] = yy;
-      c[Z] = zz;
-      s[0] = c[0];
-      s[1] = c[1];
-      s[2] = c[2];
-      s[X] += src_off;
-      fn(f, g, fvox(g, c), fvox(g, s), ctx);
-    }
-}
-
-typedef struct {
-  int comp;
-  float scale;
-} scale_ctx_t;
-
-static void fn_scale_jf(vpic_field_t *f, const vpic_grid_t *g, int v, int s,
-                        void *ctx) {
-  scale_ctx_t *c = (scale_ctx_t *)ctx;
-  (void)g;
-  (void)s;
-  float *x = fcomp(f + v, 3, c->comp);
-  if (c->scale == 0)
-    *x = 0;
-  else
-    *x *= 2.;
-}
-
-static void fn_zero_e_tca(vpic_field_t *f, const vpic_grid_t *g, int v, int s,
-                          void *ctx) {
-  int comp = *(int *)ctx;
-  (void)g;
-  (void)s;
-  *fcomp(f + v, 0, comp) = 0;
-  *fcomp(f + v, 2, comp) = 0;
-}
-
-typedef struct {
-  int comp;
-  float sign;
-} copy_ctx_t;
-
-static void fn_ghost_cb(vpic_field_t *f, const vpic_grid_t *g, int v, int s,
-                        void *ctx) {
-  copy_ctx_t *c = (copy_ctx_t *)ctx;
-  (void)g;
-  float src = *fcomp(f + s, 1, c->comp);
-  *fcomp(f + v, 1, c->comp) = c->sign > 0 ? src : -src;
-}
-
-/* For axis X with Y=(X+1)%3, Z=(X+2)%3 the reference's macros visit:
- *   tangential comp Y on "ZY" edges: Y in 1..nY+1? -- see below.
- * zy_EDGE_LOOP(x) = (x, 1..ny+1, 1..nz) carries cb_y / e_z / jf_z;
- * yz_EDGE_LOOP(x) = (x, 1..ny, 1..nz+1) carries cb_z / e_y / jf_y
- * (local.cc:30-36).  In rotated form for normal axis X: the loop over
- * (Y in 1..nY+1, Z in 1..nZ) holds cb_Y, e_Z, jf_Z and the loop over
- * (Y in 1..nY, Z in 1..nZ+1) holds cb_Z, e_Y, jf_Y. */
-
-static void local_ghost_tang_b(vpic_field_t *f, const vpic_grid_t *g) {
-  /* local.cc:116-121, order -x,-y,-z,+x,+y,+z. */
-  for (int k = 0; k < 6; k++) {
-    int bc = g->fbc[k];
-    if (bc == VPIC_BC_PERIODIC) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int hi = FACE_HIGH(k);
-    int ghost = hi ? nd(g, X) + 1 : 0;
-    int off = hi ? -1 : 1; /* source = ghost - (i,j,k) */
-    copy_ctx_t c;
-    if (bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS)
-      c.sign = 1;
-    else /* symmetric / pmc */
-      c.sign = -1;
-    c.comp = Y;
-    edge_loop(f, g, X, Y, ghost, 1, 0, off, fn_ghost_cb, &c);
-    c.comp = Z;
-    edge_loop(f, g, X, Y, ghost, 0, 1, off, fn_ghost_cb, &c);
-  }
-}
-
-/* Periodic ghost exchange with this same domain (remote.cc:61-134 with the
- * self-loop port of grid_comm.cc): every periodic face pair first sends
- * (face 1 towards -X, face nX towards +X), then receives in the order
- * -x,-y,-z,+x,+y,+z with rw = 1, lw = 0 (equal cell sizes). */
-static void remote_ghost_tang_b(vpic_field_t *f, const vpic_grid_t *g) {
-  float *buf[6] = {0};
-  for (int k = 0; k < 6; k++) {
-    if (g->fbc[k] != VPIC_BC_PERIODIC) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int face = FACE_HIGH(k) ? nd(g, X) : 1;
-    int n = (nd(g, Y) + 1) * nd(g, Z) + nd(g, Y) * (nd(g, Z) + 1), m = 0;
-    int c[3];
-    buf[k] = (float *)malloc(sizeof(float) * (size_t)n);
-    for (int zz = 1; zz <= nd(g, Z); zz++)
-      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        buf[k][m++] = *fcomp(f + fvox(g, c), 1, Y);
-      }
-    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-      for (int yy = 1; yy <= nd(g, Y); yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        buf[k][m++] = *fcomp(f + fvox(g, c), 1, Z);
-      }
-  }
-  for (int k = 0; k < 6; k++) {
-    if (g->fbc[k] != VPIC_BC_PERIODIC) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    /* Port k receives what the partner sent through the same-signed port,
-     * i.e. this domain's face on the opposite side. */
-    const float *p = buf[k];
-    const float rw = 1.0f, lw = 0.0f;
-    int lo = !FACE_HIGH(k);
-    int face = lo ? nd(g, X) + 1 : 0;
-    int inner = lo ? -1 : 1; /* field(x+i,...) */
-    int c[3], s[3], m = 0;
-    for (int zz = 1; zz <= nd(g, Z); zz++)
-      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        s[0] = c[0]; s[1] = c[1]; s[2] = c[2];
-        s[X] += inner;
-        *fcomp(f + fvox(g, c), 1, Y) =
-            rw * p[m++] + lw * *fcomp(f + fvox(g, s), 1, Y);
-      }
-    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-      for (int yy = 1; yy <= nd(g, Y); yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        s[0] = c[0]; s[1] = c[1]; s[2] = c[2];
-        s[X] += inner;
-        *fcomp(f + fvox(g, c), 1, Z) =
-            rw * p[m++] + lw * *fcomp(f + fvox(g, s), 1, Z);
-      }
-  }
-  for (int k = 0; k < 6; k++) free(buf[k]);
-}
-
-static void local_adjust_tang_e(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int k = 0; k < 6; k++) {
-    if (g->fbc[k] != VPIC_BC_ANTI_SYMMETRIC_FIELDS) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
-    int comp = Y;
-    edge_loop(f, g, X, Y, face, 0, 1, 0, fn_zero_e_tca, &comp);
-    comp = Z;
-    edge_loop(f, g, X, Y, face, 1, 0, 0, fn_zero_e_tca, &comp);
-  }
-}
-
-static void local_adjust_norm_b(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int k = 0; k < 6; k++) {
-    if (g->fbc[k] != VPIC_BC_SYMMETRIC_FIELDS) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
-    int c[3];
-    for (int zz = 1; zz <= nd(g, Z); zz++)
-      for (int yy = 1; yy <= nd(g, Y); yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        *fcomp(f + fvox(g, c), 1, X) = 0;
-      }
-  }
-}
-
-static void local_adjust_jf(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int k = 0; k < 6; k++) {
-    int bc = g->fbc[k];
-    if (bc == VPIC_BC_PERIODIC) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
-    scale_ctx_t c;
-    c.scale = bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS ? 0.0f : 2.0f;
-    c.comp = Y;
-    edge_loop(f, g, X, Y, face, 0, 1, 0, fn_scale_jf, &c);
-    c.comp = Z;
-    edge_loop(f, g, X, Y, face, 1, 0, 0, fn_scale_jf, &c);
-  }
-}
-
-void orc_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g) {
-  local_adjust_jf(f, g);
-  /* remote.cc:474-502: x faces, then y, then z; both sends of an axis
-   * are packed before either receive. */
-  for (int X = 0; X < 3; X++) {
-    if (g->fbc[X] != VPIC_BC_PERIODIC || g->fbc[X + 3] != VPIC_BC_PERIODIC)
-      continue;
-    int Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int n = nd(g, Y) * (nd(g, Z) + 1) + nd(g, Z) * (nd(g, Y) + 1);
-    float *snd[2];
-    int c[3];
-    for (int side = 0; side < 2; side++) {
-      int face = side == 0 ? 1 : nd(g, X) + 1, m = 0;
-      snd[side] = (float *)malloc(sizeof(float) * (size_t)n);
-      for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-        for (int yy = 1; yy <= nd(g, Y); yy++) {
-          c[X] = face; c[Y] = yy; c[Z] = zz;
-          snd[side][m++] = *fcomp(f + fvox(g, c), 3, Y);
-        }
-      for (int zz = 1; zz <= nd(g, Z); zz++)
-        for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-          c[X] = face; c[Y] = yy; c[Z] = zz;
-          snd[side][m++] = *fcomp(f + fvox(g, c), 3, Z);
-        }
-    }
-    /* END_RECV(-1): face nX+1 gets the face-1 send; END_RECV(+1): face 1
-     * gets the face nX+1 send.  lw = rw = 1 for equal cell sizes. */
-    for (int side = 0; side < 2; side++) {
-      const float *p = side == 0 ? snd[0] : snd[1];
-      int face = side == 0 ? nd(g, X) + 1 : 1, m = 0;
-      const float lw = 1.0f, rw = 1.0f;
-      for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-        for (int yy = 1; yy <= nd(g, Y); yy++) {
-          c[X] = face; c[Y] = yy; c[Z] = zz;
-          float *x = fcomp(f + fvox(g, c), 3, Y);
-          *x = lw * *x + rw * p[m++];
-        }
-      for (int zz = 1; zz <= nd(g, Z); zz++)
-        for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-          c[X] = face; c[Y] = yy; c[Z] = zz;
-          float *x = fcomp(f + fvox(g, c), 3, Z);
-          *x = lw * *x + rw * p[m++];
-        }
-    }
-    free(snd[0]);
-    free(snd[1]);
-  }
-}
-
-/* synchronize_rho (remote.cc:534-625): local_adjust_rhof / rhob
- * (local.cc:376-440) on local faces, then the x, y, z node-plane folds with
- * lw = rw = 1 for rhof and hlw = hrw = 1/2 for rhob (equal cell sizes). */
-void orc_synchronize_rho(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int k = 0; k < 6; k++) {
-    int bc = g->fbc[k];
-    if (bc == VPIC_BC_PERIODIC) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
-    int c[3];
-    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        vpic_field_t *fp = f + fvox(g, c);
-        if (bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS) fp->rhof = 0;
-        else fp->rhof *= 2;
-      }
-  }
-  for (int k = 0; k < 6; k++) {
-    if (g->fbc[k] != VPIC_BC_ANTI_SYMMETRIC_FIELDS) continue;
-    int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
-    int c[3];
-    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        f[fvox(g, c)].rhob = 0;
-      }
-  }
-  for (int X = 0; X < 3; X++) {
-    if (g->fbc[X] != VPIC_BC_PERIODIC || g->fbc[X + 3] != VPIC_BC_PERIODIC)
-      continue;
-    int Y = (X + 1) % 3, Z = (X + 2) % 3;
-    int n = (nd(g, Y) + 1) * (nd(g, Z) + 1);
-    float *snd[2];
-    int c[3];
-    for (int side = 0; side < 2; side++) {
-      int face = side == 0 ? 1 : nd(g, X) + 1, m = 0;
-      snd[side] = (float *)malloc(sizeof(float) * 2 * (size_t)n);
-      for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-        for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-          c[X] = face; c[Y] = yy; c[Z] = zz;
-          snd[side][m++] = f[fvox(g, c)].rhof;
-          snd[side][m++] = f[fvox(g, c)].rhob;
-        }
-    }
-    for (int side = 0; side < 2; side++) {
-      const float *p = snd[side];
-      int face = side == 0 ? nd(g, X) + 1 : 1, m = 0;
-      const float hlw = 0.5f, hrw = 0.5f, lw = hlw + hlw, rw = hrw + hrw;
-      for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-        for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-          c[X] = face; c[Y] = yy; c[Z] = zz;
-          vpic_field_t *fp = f + fvox(g, c);
-          fp->rhof = lw * fp->rhof + rw * p[m++];
-          fp->rhob = hlw * fp->rhob + hrw * p[m++];
-        }
-    }
-    free(snd[0]);
-    free(snd[1]);
-  }
-}
-
-/* ------------------------------------------------------------------------ */
-/* FDTD                                                                     */
-/* ------------------------------------------------------------------------ */
-
-void orc_advance_b(vpic_field_t *f, const vpic_grid_t *g, float frac) {
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const float px = (nx > 1) ? frac * g->cvac * g->dt * g->rdx : 0;
-  const float py = (ny > 1) ? frac * g->cvac * g->dt * g->rdy : 0;
-  const float pz = (nz > 1) ? frac * g->cvac * g->dt * g->rdz : 0;
-  /* Every cb update reads only E, so the bulk/leftover split of
-   * advance_b_pipeline.cc:69-122 has no effect on the values. */
-  for (int z = 1; z <= nz + 1; z++)
-    for (int y = 1; y <= ny + 1; y++)
-      for (int x = 1; x <= nx + 1; x++) {
-        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
-        const vpic_field_t *fx = f + VOX(x + 1, y, z, nx, ny);
-        const vpic_field_t *fy = f + VOX(x, y + 1, z, nx, ny);
-        const vpic_field_t *fz = f + VOX(x, y, z + 1, nx, ny);
-        if (y <= ny && z <= nz)
-          f0->cbx -= (py * (fy->ez - f0->ez) - pz * (fz->ey - f0->ey));
-        if (x <= nx && z <= nz)
-          f0->cby -= (pz * (fz->ex - f0->ex) - px * (fx->ez - f0->ez));
-        if (x <= nx && y <= ny)
-          f0->cbz -= (px * (fx->ey - f0->ey) - py * (fy->ex - f0->ex));
-      }
-  local_adjust_norm_b(f, g);
-}
-
 void orc_field_params_vacuum(vpic_field_params_t *mp, float damp) {
   /* sfa.cc:111-126 with sigma = 0, eps = mu = 1. */
   mp->decayx = mp->decayy = mp->decayz = (float)exp(-0.0);
@@ -908,115 +235,6 @@ void orc_field_params_vacuum(vpic_field_params_t *mp, float damp) {
   mp->epsx = mp->epsy = mp->epsz = 1.0f;
   mp->damp = damp;
   mp->nonconductive = 1.0f; /* sfa.cc:144 with ax = ay = az = 0 */
-}
-
-void orc_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
-                          const vpic_field_params_t *m) {
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const float decayx = m->decayx, drivex = m->drivex;
-  const float decayy = m->decayy, drivey = m->drivey;
-  const float decayz = m->decayz, drivez = m->drivez;
-  const float damp = m->damp;
-  const float px_muz = ((nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuz;
-  const float px_muy = ((nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuy;
-  const float py_mux = ((ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmux;
-  const float py_muz = ((ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmuz;
-  const float pz_muy = ((nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmuy;
-  const float pz_mux = ((nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmux;
-  const float cj = g->dt / g->eps0;
-
-  remote_ghost_tang_b(f, g);
-  local_ghost_tang_b(f, g);
-  /* Each E update reads cb (not written here) and its own voxel, so the
-   * pipeline/host split of vacuum_advance_e_pipeline.cc:87-329 does not
-   * change any value. */
-  for (int z = 1; z <= nz + 1; z++)
-    for (int y = 1; y <= ny + 1; y++)
-      for (int x = 1; x <= nx + 1; x++) {
-        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
-        const vpic_field_t *fx = f + VOX(x - 1, y, z, nx, ny);
-        const vpic_field_t *fy = f + VOX(x, y - 1, z, nx, ny);
-        const vpic_field_t *fz = f + VOX(x, y, z - 1, nx, ny);
-        if (x <= nx) {
-          f0->tcax = (py_muz * (f0->cbz - fy->cbz) - pz_muy * (f0->cby - fz->cby)) -
-                     damp * f0->tcax;
-          f0->ex = decayx * f0->ex + drivex * (f0->tcax - cj * f0->jfx);
-        }
-        if (y <= ny) {
-          f0->tcay = (pz_mux * (f0->cbx - fz->cbx) - px_muz * (f0->cbz - fx->cbz)) -
-                     damp * f0->tcay;
-          f0->ey = decayy * f0->ey + drivey * (f0->tcay - cj * f0->jfy);
-        }
-        if (z <= nz) {
-          f0->tcaz = (px_muy * (f0->cby - fx->cby) - py_mux * (f0->cbx - fy->cbx)) -
-                     damp * f0->tcaz;
-          f0->ez = decayz * f0->ez + drivez * (f0->tcaz - cj * f0->jfz);
-        }
-      }
-  local_adjust_tang_e(f, g);
-}
-
-/* ------------------------------------------------------------------------ */
-/* sort_p (sort_p_pipeline.cc:143-214, 345-370)                            */
-/* ------------------------------------------------------------------------ */
-
-void orc_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
-                int *partition, const vpic_grid_t *g) {
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const int vl = VOX(1, 1, 1, nx, ny), vh = VOX(nx, ny, nz, nx, ny);
-  const int nv = g->nv;
-  int *next = (int *)calloc((size_t)nv, sizeof(int));
-  int sum = 0;
-  for (int i = 0; i < np; i++) next[p[i].i]++;
-  for (int v = vl; v <= vh; v++) {
-    int count = next[v];
-    next[v] = sum;
-    partition[v] = sum;
-    sum += count;
-  }
-  partition[vh + 1] = sum;
-  for (int i = 0; i < np; i++) aux[next[p[i].i]++] = p[i];
-  for (int v = 0; v < vl; v++) partition[v] = 0;
-  for (int v = vh + 1; v <= nv; v++) partition[v] = np;
-  memcpy(p, aux, sizeof(*p) * (size_t)np);
-  free(next);
-}
-
-/* ------------------------------------------------------------------------ */
-/* Diagnostics                                                              */
-/* ------------------------------------------------------------------------ */
-
-static double energy_p_range(const vpic_particle_t *p, int n0, int n1,
-                             float qdt_2mc, float msp,
-                             const vpic_interpolator_t *f) {
-  const float one = 1.0;
-  double en = 0.0;
-  for (int n = n0; n < n1; n++) {
-    float dx = p[n].dx, dy = p[n].dy, dz = p[n].dz, v0, v1, v2;
-    int i = p[n].i;
-    v0 = p[n].ux + qdt_2mc * ((f[i].ex + dy * f[i].dexdy) + dz * (f[i].dexdz + dy * f[i].d2exdydz));
-    v1 = p[n].uy + qdt_2mc * ((f[i].ey + dz * f[i].deydz) + dx * (f[i].deydx + dz * f[i].d2eydzdx));
-    v2 = p[n].uz + qdt_2mc * ((f[i].ez + dx * f[i].dezdx) + dy * (f[i].dezdy + dx * f[i].d2ezdxdy));
-    v0 = v0 * v0 + v1 * v1 + v2 * v2;
-    v0 = (msp * p[n].w) * (v0 / (one + sqrtf(one + v0)));
-    en += (double)v0;
-  }
-  return en;
-}
-
-double orc_energy_p(const vpic_particle_t *p, int np, float q, float m,
-                    const vpic_interpolator_t *f, const vpic_grid_t *g) {
-  const float qdt_2mc = (q * g->dt) / (2 * m * g->cvac);
-  int i0, n0, i1, n1;
-  double en[2], local;
-  distribute(np, 16, 0, 1, &i0, &n0);
-  distribute(np, 16, 1, 1, &i1, &n1);
-  en[0] = energy_p_range(p, i0, i0 + n0, qdt_2mc, m, f);
-  en[1] = energy_p_range(p, i1, i1 + n1, qdt_2mc, m, f);
-  local = 0.0;
-  local += en[0];
-  local += en[1];
-  return local * ((double)g->cvac * (double)g->cvac);
 }
 
 /* center_p / uncenter_p, per particle (center_p_pipeline.cc:15-97,
@@ -1069,78 +287,6 @@ void orc_center_p(vpic_particle_t *p, int np, float q, float m,
     pp->ux = ux;
     pp->uy = uy;
     pp->uz = uz;
-  }
-}
-
-static void energy_f_range(double en[6], const vpic_field_t *f,
-                           const vpic_grid_t *g,
-                           const vpic_field_params_t *m, int l0, int cnt) {
-  const int nx = g->nx, ny = g->ny;
-  const float qepsx = 0.25 * m->epsx, qepsy = 0.25 * m->epsy,
-              qepsz = 0.25 * m->epsz;
-  const float hrmux = 0.50 * m->rmux, hrmuy = 0.50 * m->rmuy,
-              hrmuz = 0.50 * m->rmuz;
-  for (int k = 0; k < 6; k++) en[k] = 0;
-  for (int l = l0; l < l0 + cnt; l++) {
-    int x = 1 + l % nx, y = 1 + (l / nx) % ny, z = 1 + l / (nx * ny);
-    const vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
-    const vpic_field_t *fx = f + VOX(x + 1, y, z, nx, ny);
-    const vpic_field_t *fy = f + VOX(x, y + 1, z, nx, ny);
-    const vpic_field_t *fz = f + VOX(x, y, z + 1, nx, ny);
-    const vpic_field_t *fyz = f + VOX(x, y + 1, z + 1, nx, ny);
-    const vpic_field_t *fzx = f + VOX(x + 1, y, z + 1, nx, ny);
-    const vpic_field_t *fxy = f + VOX(x + 1, y + 1, z, nx, ny);
-    en[0] += qepsx * (f0->ex * f0->ex + fy->ex * fy->ex + fz->ex * fz->ex + fyz->ex * fyz->ex);
-    en[1] += qepsy * (f0->ey * f0->ey + fz->ey * fz->ey + fx->ey * fx->ey + fzx->ey * fzx->ey);
-    en[2] += qepsz * (f0->ez * f0->ez + fx->ez * fx->ez + fy->ez * fy->ez + fxy->ez * fxy->ez);
-    en[3] += hrmux * (f0->cbx * f0->cbx + fx->cbx * fx->cbx);
-    en[4] += hrmuy * (f0->cby * f0->cby + fy->cby * fy->cby);
-    en[5] += hrmuz * (f0->cbz * f0->cbz + fz->cbz * fz->cbz);
-  }
-}
-
-void orc_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
-                  const vpic_field_params_t *m) {
-  int ncell = g->nx * g->ny * g->nz;
-  int blk = 16 * (ncell / 16);
-  double e0[6], e1[6];
-  energy_f_range(e0, f, g, m, 0, blk);
-  energy_f_range(e1, f, g, m, blk, ncell - blk);
-  double v0 = 0.5 * g->eps0 * g->dV;
-  for (int k = 0; k < 6; k++) en[k] = (e0[k] + e1[k]) * v0;
-}
-
-void orc_accumulate_rho_p(vpic_field_t *f, const vpic_particle_t *p, int np,
-                          float q, const vpic_grid_t *g) {
-  const float q_8V = q * g->r8V;
-  const int sy = g->sy, sz = g->sz;
-  for (int n = 0; n < np; n++) {
-    float w0 = p[n].dx, w1 = p[n].dy, dz = p[n].dz;
-    int v = p[n].i;
-    float w7 = p[n].w * q_8V, w2, w3, w4, w5, w6;
-    /* rho_p.cc:64-74: FMA(x,y,z) = z + x*y, FNMS(x,y,z) = z - x*y. */
-    w6 = w7 - w0 * w7;
-    w7 = w7 + w0 * w7;
-    w4 = w6 - w1 * w6;
-    w5 = w7 - w1 * w7;
-    w6 = w6 + w1 * w6;
-    w7 = w7 + w1 * w7;
-    w0 = w4 - dz * w4;
-    w1 = w5 - dz * w5;
-    w2 = w6 - dz * w6;
-    w3 = w7 - dz * w7;
-    w4 = w4 + dz * w4;
-    w5 = w5 + dz * w5;
-    w6 = w6 + dz * w6;
-    w7 = w7 + dz * w7;
-    f[v].rhof += w0;
-    f[v + 1].rhof += w1;
-    f[v + sy].rhof += w2;
-    f[v + sy + 1].rhof += w3;
-    f[v + sz].rhof += w4;
-    f[v + sz + 1].rhof += w5;
-    f[v + sz + sy].rhof += w6;
-    f[v + sz + sy + 1].rhof += w7;
   }
 }
 
@@ -1275,147 +421,6 @@ void orc_inject_p(vpic_particle_t *p, int *np, int max_np,
 /* Divergence cleaning and initialization kernels of the vacuum field table */
 /* (sfa.cc:203-210)                                                         */
 /* ------------------------------------------------------------------------ */
-
-static float *fslot(vpic_field_t *f, int k) { return (float *)f + k; }
-
-/* Normal-E ghosts (and tca for local faces) of every face: periodic faces
- * take the opposite interior node plane with rw = 1, lw = 0
- * (remote.cc:137-207); local faces copy (anti-symmetric) or negate
- * (symmetric, pmc) the adjacent plane (local.cc:129-179).  Ghost node
- * planes only read interior planes, so the order does not matter. */
-static void ghost_norm_e(vpic_field_t *f, const vpic_grid_t *g) {
-  for (int k = 0; k < 6; k++) {
-    const int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    const int hi = FACE_HIGH(k), bc = g->fbc[k];
-    const int ghost = hi ? nd(g, X) + 1 : 0;
-    int c[3], s[3];
-    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-        c[X] = ghost; c[Y] = yy; c[Z] = zz;
-        s[0] = c[0]; s[1] = c[1]; s[2] = c[2];
-        vpic_field_t *f0 = f + fvox(g, c);
-        if (bc == VPIC_BC_PERIODIC) {
-          /* the opposite interior plane; lw multiplies the adjacent one */
-          s[X] = hi ? 1 : nd(g, X);
-          int a[3] = {c[0], c[1], c[2]};
-          a[X] = hi ? nd(g, X) : 1;
-          const float rw = 1.0f, lw = 0.0f;
-          *fcomp(f0, 0, X) = rw * *fcomp(f + fvox(g, s), 0, X) +
-                             lw * *fcomp(f + fvox(g, a), 0, X);
-        } else {
-          s[X] = hi ? nd(g, X) : 1;
-          vpic_field_t *f1 = f + fvox(g, s);
-          const float e = *fcomp(f1, 0, X), t = *fcomp(f1, 2, X);
-          if (bc == VPIC_BC_ANTI_SYMMETRIC_FIELDS) {
-            *fcomp(f0, 0, X) = e;
-            *fcomp(f0, 2, X) = t;
-          } else if (bc == VPIC_BC_ABSORB_FIELDS) {
-            s[X] = hi ? nd(g, X) - 1 : 2;
-            vpic_field_t *f2 = f + fvox(g, s);
-            *fcomp(f0, 0, X) = 2 * e - *fcomp(f2, 0, X);
-            *fcomp(f0, 2, X) = 2 * t - *fcomp(f2, 2, X);
-          } else {
-            *fcomp(f0, 0, X) = -e;
-            *fcomp(f0, 2, X) = -t;
-          }
-        }
-      }
-  }
-}
-
-/* Zero field slot `slot` (floats into field_t) on the nodes of every
- * anti-symmetric face, and of every absorbing face when `absorb`
- * (local_adjust_div_e, local.cc:299-334; local_adjust_rhob,
- * local.cc:415-440). */
-static void zero_node_faces(vpic_field_t *f, const vpic_grid_t *g, int slot,
-                            int absorb) {
-  for (int k = 0; k < 6; k++) {
-    if (g->fbc[k] != VPIC_BC_ANTI_SYMMETRIC_FIELDS &&
-        !(absorb && g->fbc[k] == VPIC_BC_ABSORB_FIELDS))
-      continue;
-    const int X = FACE_AXIS(k), Y = (X + 1) % 3, Z = (X + 2) % 3;
-    const int face = FACE_HIGH(k) ? nd(g, X) + 1 : 1;
-    int c[3];
-    for (int zz = 1; zz <= nd(g, Z) + 1; zz++)
-      for (int yy = 1; yy <= nd(g, Y) + 1; yy++) {
-        c[X] = face; c[Y] = yy; c[Z] = zz;
-        *fslot(f + fvox(g, c), slot) = 0;
-      }
-  }
-}
-
-void orc_compute_div_e_err(vpic_field_t *f, const vpic_grid_t *g,
-                           const vpic_field_params_t *m) {
-  /* vacuum_compute_div_e_err_pipeline.h:17-50, .cc:61-174 */
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const float nc = m->nonconductive;
-  const float px = ((nx > 1) ? g->rdx : 0) * m->epsx;
-  const float py = ((ny > 1) ? g->rdy : 0) * m->epsy;
-  const float pz = ((nz > 1) ? g->rdz : 0) * m->epsz;
-  const float cj = 1. / g->eps0;
-  ghost_norm_e(f, g);
-  for (int z = 1; z <= nz + 1; z++)
-    for (int y = 1; y <= ny + 1; y++)
-      for (int x = 1; x <= nx + 1; x++) {
-        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
-        const vpic_field_t *fx = f + VOX(x - 1, y, z, nx, ny);
-        const vpic_field_t *fy = f + VOX(x, y - 1, z, nx, ny);
-        const vpic_field_t *fz = f + VOX(x, y, z - 1, nx, ny);
-        f0->div_e_err = nc * (px * (f0->ex - fx->ex) + py * (f0->ey - fy->ey) +
-                              pz * (f0->ez - fz->ez) - cj * (f0->rhof + f0->rhob));
-      }
-  zero_node_faces(f, g, 3, 1);
-}
-
-void orc_compute_rhob(vpic_field_t *f, const vpic_grid_t *g,
-                      const vpic_field_params_t *m) {
-  /* vacuum_compute_rhob_pipeline.h:17-52, .cc:41-175 */
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const float nc = m->nonconductive;
-  const float px = (nx > 1) ? g->eps0 * m->epsx * g->rdx : 0;
-  const float py = (ny > 1) ? g->eps0 * m->epsy * g->rdy : 0;
-  const float pz = (nz > 1) ? g->eps0 * m->epsz * g->rdz : 0;
-  ghost_norm_e(f, g);
-  for (int z = 1; z <= nz + 1; z++)
-    for (int y = 1; y <= ny + 1; y++)
-      for (int x = 1; x <= nx + 1; x++) {
-        vpic_field_t *f0 = f + VOX(x, y, z, nx, ny);
-        const vpic_field_t *fx = f + VOX(x - 1, y, z, nx, ny);
-        const vpic_field_t *fy = f + VOX(x, y - 1, z, nx, ny);
-        const vpic_field_t *fz = f + VOX(x, y, z - 1, nx, ny);
-        f0->rhob = nc * (px * (f0->ex - fx->ex) + py * (f0->ey - fy->ey) +
-                         pz * (f0->ez - fz->ez) - f0->rhof);
-      }
-  zero_node_faces(f, g, 11, 0);
-}
-
-/* Weighted sum of div_e_err^2 over the nodes 1..n+1: interior nodes in
- * FP32 products, boundary nodes in FP64 with 1/2, 1/4, 1/8 on faces,
- * edges and corners (compute_rms_div_e_err_pipeline.cc:16-180). */
-double orc_compute_rms_div_e_err(const vpic_field_t *f, const vpic_grid_t *g) {
-  const int nx = g->nx, ny = g->ny, nz = g->nz;
-  double err = 0;
-  for (int z = 2; z <= nz; z++)
-    for (int y = 2; y <= ny; y++)
-      for (int x = 2; x <= nx; x++) {
-        const float d = f[VOX(x, y, z, nx, ny)].div_e_err;
-        err += d * d;
-      }
-  for (int z = 1; z <= nz + 1; z++)
-    for (int y = 1; y <= ny + 1; y++)
-      for (int x = 1; x <= nx + 1; x++) {
-        const int bx = x == 1 || x == nx + 1, by = y == 1 || y == ny + 1,
-                  bz = z == 1 || z == nz + 1;
-        const int nb = bx + by + bz;
-        if (!nb) continue;
-        const double w = nb == 1 ? 0.5 : nb == 2 ? 0.25 : 0.125;
-        const double d = f[VOX(x, y, z, nx, ny)].div_e_err;
-        err += w * d * d;
-      }
-  double local0 = err * g->dV;
-  double local1 = (g->nx * g->ny * g->nz) * g->dV;
-  return g->eps0 * sqrt(local0 / local1);
-}
 
 void orc_clean_div_e(vpic_field_t *f, const vpic_grid_t *g,
                      const vpic_field_params_t *m) {
@@ -1804,10 +809,13 @@ static float ta_frandn(ta_rng_t *r) {
 }
 
 /* takizuka_abe_collision (takizuka_abe_pipeline.cc:14-79), with the same
- * operations in the same order (float, except the double sin/cos of the
- * scattering phase as there). */
-static void ta_collide(vpic_particle_t *pi, vpic_particle_t *pj, float mu_i,
-                       float mu_j, float std, ta_rng_t *rng) {
+ * operations in the same order, all in float.  The macro takes three draws
+ * per collision, in this order: frandn (the scattering angle), frand_c0
+ * (the phase) and frand_c0 (the unequal-weight test); they are passed in
+ * as dn, u_phase and u_w. */
+static void ta_collide_draws(vpic_particle_t *pi, vpic_particle_t *pj,
+                             float mu_i, float mu_j, float std, float dn,
+                             float u_phase, float u_w) {
   float dd, ur, urx, ury, urz, tx, ty, tz, t0, t1, t2, wi, wj, stack[3];
   int d0, d1, d2;
   urx = pi->ux - pj->ux;
@@ -1838,18 +846,21 @@ static void ta_collide(vpic_particle_t *pi, vpic_particle_t *pj, float mu_i,
   t2 = 1 / ur;
   t1 = std * sqrtf(t2) * t2;
   if (t0 < t1) t1 = t0;
-  dd = t1 * ta_frandn(rng);
+  dd = t1 * dn;
   t0 = 2 * dd / (1 + dd * dd);
-  t1 = (float)(2 * M_PI * ta_frand_c0(rng));
-  t2 = (float)(t0 * sin((double)t1));
-  t1 = (float)(t0 * ur * cos((double)t1));
+  t1 = (float)(2 * M_PI * u_phase);
+  /* sin(t1), cos(t1) of a float are the float overloads in the reference's
+   * C++; here the double functions rounded to float (the correctly rounded
+   * value, as the device forms it), multiplied in float. */
+  t2 = t0 * (float)sin((double)t1);
+  t1 = t0 * ur * (float)cos((double)t1);
   t0 *= -dd;
   /* (1 - cos theta) u + |u| sin theta T (:60-63) */
   stack[0] = (t0 * urx + t1 * tx) + t2 * (ury * tz - urz * ty);
   stack[1] = (t0 * ury + t1 * ty) + t2 * (urz * tx - urx * tz);
   stack[2] = (t0 * urz + t1 * tz) + t2 * (urx * ty - ury * tx);
   /* unequal weights (:65-70) */
-  t0 = ta_frand_c0(rng);
+  t0 = u_w;
   t1 = mu_i;
   t2 = mu_j;
   if (wj < wi && wi * t0 > wj) t1 = 0;
@@ -1860,6 +871,15 @@ static void ta_collide(vpic_particle_t *pi, vpic_particle_t *pj, float mu_i,
   pj->ux -= t2 * stack[0];
   pj->uy -= t2 * stack[1];
   pj->uz -= t2 * stack[2];
+}
+
+static void ta_collide(vpic_particle_t *pi, vpic_particle_t *pj, float mu_i,
+                       float mu_j, float std, ta_rng_t *rng) {
+  /* the macro's draws, in the order it takes them */
+  const float dn = ta_frandn(rng);
+  const float u_phase = ta_frand_c0(rng);
+  const float u_w = ta_frand_c0(rng);
+  ta_collide_draws(pi, pj, mu_i, mu_j, std, dn, u_phase, u_w);
 }
 
 void orc_takizuka_abe(vpic_particle_t *spi_p, const int *spi_partition,

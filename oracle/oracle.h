@@ -203,6 +203,106 @@ void orc_takizuka_abe(vpic_particle_t *pi, const int *part_i, float qi,
                       float qj, float mj, double cvar0, int interval,
                       uint64_t seed, uint64_t call, const vpic_grid_t *g);
 
+/* ------------------------------------------------------------------------ */
+/* FP64 oracle: the step functions of oracle_step.inc compiled over double */
+/* ------------------------------------------------------------------------ */
+
+/* The FP64 twins of the reference's types: the same members in the same
+ * order, every float a double (ints unchanged). */
+typedef struct {
+  double dx, dy, dz;
+  int32_t i, pad_;
+  double ux, uy, uz, w;
+} orc64_particle_t;
+
+typedef struct {
+  double dispx, dispy, dispz;
+  int32_t i, pad_;
+} orc64_particle_mover_t;
+
+typedef struct {
+  double ex, dexdy, dexdz, d2exdydz;
+  double ey, deydz, deydx, d2eydzdx;
+  double ez, dezdx, dezdy, d2ezdxdy;
+  double cbx, dcbxdx;
+  double cby, dcbydy;
+  double cbz, dcbzdz;
+  double _pad[2];
+} orc64_interpolator_t;
+
+typedef struct {
+  double jx[4];
+  double jy[4];
+  double jz[4];
+} orc64_accumulator_t;
+
+typedef struct {
+  double ex, ey, ez, div_e_err;
+  double cbx, cby, cbz, div_b_err;
+  double tcax, tcay, tcaz, rhob;
+  double jfx, jfy, jfz, rhof;
+  int16_t ematx, ematy, ematz, nmat;
+  int16_t fmatx, fmaty, fmatz, cmat;
+} orc64_field_t;
+
+typedef struct {
+  double dt, cvac, eps0;
+  double dx, dy, dz, dV;
+  double rdx, rdy, rdz, r8V;
+  int32_t nx, ny, nz;
+  int32_t sx, sy, sz, nv;
+  int64_t rangel, rangeh;
+  const int64_t *neighbor; /* host, 6*nv */
+  int32_t pbc[6];
+  int32_t fbc[6];
+} orc64_grid_t;
+
+typedef struct {
+  double decayx, drivex, decayy, drivey, decayz, drivez;
+  double rmux, rmuy, rmuz;
+  double epsx, epsy, epsz;
+  double damp;
+  double nonconductive;
+} orc64_field_params_t;
+
+/* The grid's constants widened to double (the FP64 run then differs from
+ * the FP32 one only in its arithmetic). */
+void orc64_grid_from(orc64_grid_t *g64, const vpic_grid_t *g);
+
+int orc64_move_p(orc64_particle_t *p0, orc64_particle_mover_t *pm,
+                 orc64_accumulator_t *a0, const orc64_grid_t *g, double qsp);
+void orc64_push_coefficients(double q, double m, const orc64_grid_t *g,
+                             double out[4]);
+int orc64_advance_p(orc64_particle_t *p, int np, double q, double m,
+                    orc64_particle_mover_t *pm, int max_nm, int *n_ignored,
+                    orc64_accumulator_t *a, const orc64_interpolator_t *f,
+                    const orc64_grid_t *g);
+void orc64_load_interpolator(orc64_interpolator_t *fi, const orc64_field_t *f,
+                             const orc64_grid_t *g);
+void orc64_unload_accumulator(orc64_field_t *f, const orc64_accumulator_t *a,
+                              const orc64_grid_t *g);
+void orc64_clear_jf(orc64_field_t *f, const orc64_grid_t *g);
+void orc64_clear_rhof(orc64_field_t *f, const orc64_grid_t *g);
+void orc64_synchronize_jf(orc64_field_t *f, const orc64_grid_t *g);
+void orc64_synchronize_rho(orc64_field_t *f, const orc64_grid_t *g);
+void orc64_advance_b(orc64_field_t *f, const orc64_grid_t *g, double frac);
+void orc64_vacuum_advance_e(orc64_field_t *f, const orc64_grid_t *g,
+                            const orc64_field_params_t *m);
+void orc64_sort_p(orc64_particle_t *p, orc64_particle_t *aux, int np,
+                  int *partition, const orc64_grid_t *g);
+double orc64_energy_p(const orc64_particle_t *p, int np, double q, double m,
+                      const orc64_interpolator_t *f, const orc64_grid_t *g);
+void orc64_energy_f(double en[6], const orc64_field_t *f,
+                    const orc64_grid_t *g, const orc64_field_params_t *m);
+void orc64_accumulate_rho_p(orc64_field_t *f, const orc64_particle_t *p,
+                            int np, double q, const orc64_grid_t *g);
+void orc64_compute_div_e_err(orc64_field_t *f, const orc64_grid_t *g,
+                             const orc64_field_params_t *m);
+void orc64_compute_rhob(orc64_field_t *f, const orc64_grid_t *g,
+                        const orc64_field_params_t *m);
+double orc64_compute_rms_div_e_err(const orc64_field_t *f,
+                                   const orc64_grid_t *g);
+
 #ifdef __cplusplus
 }
 #endif

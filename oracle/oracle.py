@@ -327,6 +327,11 @@ class Ref:
             getattr(lib, name).argtypes = [_vp, _pg]
             getattr(lib, name).restype = ctypes.c_double
         lib.ref_sort_p.argtypes = [_vp, ctypes.c_int, _vp, _pg]
+        lib.ref_takizuka_abe.argtypes = [_vp, _vp, _f32, _f32, _vp, _vp,
+                                         _f32, _f32, ctypes.c_double, _i32,
+                                         ctypes.c_uint64, ctypes.c_uint64,
+                                         _pg]
+        lib.ref_takizuka_abe.restype = None
         lib.ref_load_interpolator.argtypes = [_vp, _vp, _pg]
         lib.ref_unload_accumulator.argtypes = [_vp, _vp, _pg]
         lib.ref_advance_b_bulk.argtypes = [_vp, _pg, _f32]
@@ -379,6 +384,17 @@ class Ref:
         part = np.zeros(g.nv + 1, dtype=np.int32)
         self.lib.ref_sort_p(_ptr(p), len(p), _ptr(part), ctypes.byref(g))
         return part
+
+    def takizuka_abe(self, pi, part_i, qi, mi, pj, part_j, qj, mj, cvar0,
+                     interval, seed, call, g):
+        """The reference's takizuka_abe_pipeline_scalar on the oracle's
+        per-voxel Philox stream (oracle/ref/ref_ta.cc), in place."""
+        same = pj is pi
+        self.lib.ref_takizuka_abe(_ptr(pi), _ptr(part_i), qi, mi,
+                                  _ptr(pi if same else pj),
+                                  _ptr(part_i if same else part_j), qj, mj,
+                                  cvar0, interval, seed, call,
+                                  ctypes.byref(g))
 
     def load_interpolator(self, fi, f, g):
         self.lib.ref_load_interpolator(_ptr(fi), _ptr(f), ctypes.byref(g))

@@ -10,8 +10,9 @@
 // reference's, in its order: the Fisher-Yates shuffle of species i with
 // rejection sampling, the three-particle start of an odd intra-species
 // voxel, the pairing of the larger set round the smaller, and
-// takizuka_abe_collision's float arithmetic with the double sin/cos of the
-// scattering phase.  Compiled without FMA contraction, like the rest.
+// takizuka_abe_collision's float arithmetic (the scattering phase's sin and
+// cos as correctly rounded floats).  Compiled without FMA contraction, like
+// the rest.
 #include "common.h"
 
 #include <cfloat>
@@ -88,8 +89,16 @@ __device__ void ta_collide(vpic_particle_t *pi, vpic_particle_t *pj,
   dd = t1 * rng.frandn();
   t0 = 2 * dd / (1 + dd * dd);
   t1 = (float)(2 * M_PI * rng.frand_c0());
-  t2 = (float)(t0 * sin((double)t1));
-  t1 = (float)(t0 * ur * cos((double)t1));
+  // The reference's sin(t1)/cos(t1) of a float take the float overloads
+  // (C++ <cmath>); they are formed here as the double functions rounded to
+  // float (the correctly rounded float value, identical on host and
+  // device), then multiplied in float as there.
+  {
+    double sn, cs;
+    sincos((double)t1, &sn, &cs);
+    t2 = t0 * (float)sn;
+    t1 = t0 * ur * (float)cs;
+  }
   t0 *= -dd;
   const float s0 = (t0 * urx + t1 * tx) + t2 * (ury * tz - urz * ty);
   const float s1 = (t0 * ury + t1 * ty) + t2 * (urz * tx - urx * tz);

@@ -40,6 +40,10 @@ class Species:
         self.last_sorted = -1
         self.partition = device.DeviceArray((grid.nv + 1,), np.int32)
         self.dev_nm = device.DeviceArray((4,), np.int32)  # last push's movers
+        # sort_p's out-of-place buffer (sort_p.cc:91 takes a fresh max_np
+        # buffer per sort); allocated by the first sort, then p and aux
+        # swap roles at every sort, so both always hold max_np records
+        self.aux = None
 
 
 class Simulation:
@@ -58,15 +62,12 @@ class Simulation:
         self.acc = device.accumulators(grid)
         self.acc.zero()
         self.species = []
-        self.aux = None
         self.step = 0
         self.stream = None  # the default stream
         self.nm_log = []
 
     def add_species(self, sp):
         self.species.append(sp)
-        if self.aux is None or self.aux.shape[0] < sp.np:
-            self.aux = device.particles(max(sp.np, 1))
         return sp
 
     # -- the reference-named operations ------------------------------------
@@ -74,11 +75,19 @@ class Simulation:
         return ctypes.byref(self.g)
 
     def sort_p(self, sp):
-        """sort_p (sort_p.cc:154-165): stable by voxel, into the shared aux
-        buffer, then the two buffers swap roles."""
-        _lib.call("vpic_hip_sort_p", sp.p.ptr, self.aux.ptr, sp.np,
+        """sort_p (sort_p.cc:154-165): stable by voxel, into the species'
+        own aux buffer of max_np records, then the two buffers swap roles
+        (the species' particle array stays max_np records long)."""
+        if sp.aux is None:
+            sp.aux = device.particles(max(sp.max_np, 1))
+        if sp.np > sp.max_np or sp.aux.shape[0] < sp.np or \
+                sp.p.shape[0] < sp.np:
+            raise _lib.VpicError(
+                "sort_p: species %s has %d particles, its buffers hold %d "
+                "and %d" % (sp.name, sp.np, sp.p.shape[0], sp.aux.shape[0]))
+        _lib.call("vpic_hip_sort_p", sp.p.ptr, sp.aux.ptr, sp.np,
                   sp.partition.ptr, self._G(), 0, self.stream)
-        sp.p, self.aux = self.aux, sp.p
+        sp.p, sp.aux = sp.aux, sp.p
         sp.last_sorted = self.step
 
     def takizuka_abe(self, spi, spj, cvar0, interval, seed=0):
@@ -86,9 +95,14 @@ class Simulation:
         (src/collision/takizuka_abe.cc:50-75) added to the step's collision
         operators (define_collision_op): every interval steps, both species
         sorted at this step first (apply_takizuka_abe, takizuka_abe.cc:6-12),
-        then vpic_hip_takizuka_abe with the step as the stream's counter."""
+        then vpic_hip_takizuka_abe with the step as the stream's counter.
+        The reference's operators draw in turn from one advancing generator
+        pool; here each operator gets its own Philox key, collision_key(seed,
+        its index among the operators), so two operators applied at the
+        same step never share draws."""
         if interval < 1:
             raise _lib.VpicError("takizuka_abe: interval must be >= 1")
+        key = collision_key(seed, len(self.collisions))
 
         def apply(sim):
             if sim.step % interval:
@@ -99,7 +113,7 @@ class Simulation:
             _lib.call("vpic_hip_takizuka_abe", spi.p.ptr, spi.partition.ptr,
                       spi.np, spi.q, spi.m, spj.p.ptr, spj.partition.ptr,
                       spj.np, spj.q, spj.m, float(cvar0), int(interval),
-                      int(seed), int(sim.step), sim._G(), sim.stream)
+                      key, int(sim.step), sim._G(), sim.stream)
         self.collisions.append(apply)
         return apply
 
@@ -321,6 +335,17 @@ class Simulation:
         with t("interp"):
             self.load_interpolator_array()
         self.step += 1
+
+
+def collision_key(seed, index):
+    """Philox key of collision operator `index` for a user seed: the
+    splitmix64 finaliser of seed * 2^16 + index, so different operators
+    (and different seeds) get unrelated keys."""
+    m = (1 << 64) - 1
+    z = ((int(seed) << 16) + int(index) + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
 
 
 def sp_id(sim, sp):
