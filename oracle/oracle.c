@@ -237,58 +237,6 @@ void orc_field_params_vacuum(vpic_field_params_t *mp, float damp) {
   mp->nonconductive = 1.0f; /* sfa.cc:144 with ax = ay = az = 0 */
 }
 
-/* center_p / uncenter_p, per particle (center_p_pipeline.cc:15-97,
- * uncenter_p_pipeline.cc:15-97).  Particles are independent, so the
- * DISTRIBUTE order does not matter. */
-void orc_center_p(vpic_particle_t *p, int np, float q, float m,
-                  const vpic_interpolator_t *f0, const vpic_grid_t *g,
-                  int uncenter) {
-  const float qdt_2mc_sp = (q * g->dt) / (2 * m * g->cvac);
-  const float qdt_2mc = uncenter ? -qdt_2mc_sp : qdt_2mc_sp;
-  const float qdt_4mc = uncenter ? -0.5 * qdt_2mc_sp : 0.5 * qdt_2mc_sp;
-  const float one = 1.0, one_third = 1.0 / 3.0, two_fifteenths = 2.0 / 15.0;
-  for (int n = 0; n < np; n++) {
-    vpic_particle_t *pp = p + n;
-    const float dx = pp->dx, dy = pp->dy, dz = pp->dz;
-    const vpic_interpolator_t *f = f0 + pp->i;
-    const float hax = qdt_2mc * ((f->ex + dy * f->dexdy) +
-                                 dz * (f->dexdz + dy * f->d2exdydz));
-    const float hay = qdt_2mc * ((f->ey + dz * f->deydz) +
-                                 dx * (f->deydx + dz * f->d2eydzdx));
-    const float haz = qdt_2mc * ((f->ez + dx * f->dezdx) +
-                                 dy * (f->dezdy + dx * f->d2ezdxdy));
-    const float cbx = f->cbx + dx * f->dcbxdx;
-    const float cby = f->cby + dy * f->dcbydy;
-    const float cbz = f->cbz + dz * f->dcbzdz;
-    float ux = pp->ux, uy = pp->uy, uz = pp->uz;
-    float v0, v1, v2, v3, v4;
-    if (!uncenter) {
-      ux += hax;
-      uy += hay;
-      uz += haz;
-    }
-    v0 = qdt_4mc / (float)sqrt(one + (ux * ux + (uy * uy + uz * uz)));
-    v1 = cbx * cbx + (cby * cby + cbz * cbz);
-    v2 = (v0 * v0) * v1;
-    v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
-    v4 = v3 / (one + v1 * (v3 * v3));
-    v4 += v4;
-    v0 = ux + v3 * (uy * cbz - uz * cby);
-    v1 = uy + v3 * (uz * cbx - ux * cbz);
-    v2 = uz + v3 * (ux * cby - uy * cbx);
-    ux += v4 * (v1 * cbz - v2 * cby);
-    uy += v4 * (v2 * cbx - v0 * cbz);
-    uz += v4 * (v0 * cby - v1 * cbx);
-    if (uncenter) {
-      ux += hax;
-      uy += hay;
-      uz += haz;
-    }
-    pp->ux = ux;
-    pp->uy = uy;
-    pp->uz = uz;
-  }
-}
 
 /* accumulate_rhob (rho_p.cc:126-173): the particle's charge on the 8 nodes
  * of its voxel, FMA/FNMS written out as z + x*y and z - x*y as the file's

@@ -302,6 +302,9 @@ void orc64_compute_rhob(orc64_field_t *f, const orc64_grid_t *g,
                         const orc64_field_params_t *m);
 double orc64_compute_rms_div_e_err(const orc64_field_t *f,
                                    const orc64_grid_t *g);
+void orc64_center_p(orc64_particle_t *p, int np, double q, double m,
+                    const orc64_interpolator_t *f0, const orc64_grid_t *g,
+                    int uncenter);
 
 #ifdef __cplusplus
 }

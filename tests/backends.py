@@ -128,12 +128,15 @@ def make(name):
     return OracleBackend() if name == "oracle" else GpuBackend()
 
 
-def inject(g, x, y, z, ux, uy, uz, w):
+def inject(g, x, y, z, ux, uy, uz, w, lo=(0.0, 0.0, 0.0), hi=None):
     """inject_particle's position -> (voxel, cell offset) mapping, in double
     then rounded to float (src/vpic/misc.cc:43-92).  Arrays in, particles
-    out.  Assumes a one-rank box from (0,0,0) to (nx*dx, ny*dy, nz*dz)."""
-    x0, y0, z0 = 0.0, 0.0, 0.0
-    x1, y1, z1 = g.nx * float(g.dx), g.ny * float(g.dy), g.nz * float(g.dz)
+    out.  A one-rank box from `lo` to `hi` (default: lo + n*d per axis)."""
+    x0, y0, z0 = (float(v) for v in lo)
+    if hi is None:
+        hi = (x0 + g.nx * float(g.dx), y0 + g.ny * float(g.dy),
+              z0 + g.nz * float(g.dz))
+    x1, y1, z1 = (float(v) for v in hi)
     out = np.zeros(len(x), dtype=T.PARTICLE)
     coords = []
     for r, lo, hi, n in ((x, x0, x1, g.nx), (y, y0, y1, g.ny),

@@ -163,6 +163,68 @@ class Simulation:
                   self.interp.ptr, self._G(), ctypes.byref(en), self.stream)
         return en.value
 
+    def rho(self):
+        """rhof of every species: clear_rhof, accumulate_rho_p,
+        synchronize_rho (initialize.cc:33-36, advance.cc:147-150)."""
+        _lib.call("vpic_hip_clear_rhof", self.fields.ptr, self._G(),
+                  self.stream)
+        for sp in self.species:
+            _lib.call("vpic_hip_accumulate_rho_p", self.fields.ptr, sp.p.ptr,
+                      sp.np, sp.q, self._G(), self.stream)
+        _lib.call("vpic_hip_synchronize_rho", self.fields.ptr, self._G(),
+                  self.stream)
+
+    def div_e_rms(self):
+        """The charge-conservation diagnostic of advance.cc:146-160: rho,
+        compute_div_e_err, compute_rms_div_e_err (FP64 reduction on the
+        device)."""
+        self.rho()
+        _lib.call("vpic_hip_compute_div_e_err", self.fields.ptr, self._G(),
+                  ctypes.byref(self.mp), self.stream)
+        err = ctypes.c_double()
+        _lib.call("vpic_hip_compute_rms_div_e_err", ctypes.byref(err),
+                  self.fields.ptr, self._G(), self.stream)
+        return err.value
+
+    def initialize(self):
+        """vpic_simulation::initialize after the user's set-up
+        (initialize.cc:16-60): synchronize_tang_e_norm_b, one div B
+        cleaning pass, compute_curl_b, the bound charge rhob from rho and
+        the initial E, one div E cleaning pass, synchronize_tang_e_norm_b
+        again, load_interpolator_array and uncenter_p of every species.
+        Returns the diagnostics the reference logs."""
+        G, S_ = self._G(), self.stream
+        out = {}
+        err = ctypes.c_double()
+        _lib.call("vpic_hip_synchronize_tang_e_norm_b", ctypes.byref(err),
+                  self.fields.ptr, G, S_)
+        out["sync_error"] = err.value
+        _lib.call("vpic_hip_compute_div_b_err", self.fields.ptr, G, S_)
+        _lib.call("vpic_hip_compute_rms_div_b_err", ctypes.byref(err),
+                  self.fields.ptr, G, S_)
+        out["div_b_rms"] = err.value
+        _lib.call("vpic_hip_clean_div_b", self.fields.ptr, G, S_)
+        _lib.call("vpic_hip_compute_curl_b", self.fields.ptr, G,
+                  ctypes.byref(self.mp), S_)
+        self.rho()
+        _lib.call("vpic_hip_compute_rhob", self.fields.ptr, G,
+                  ctypes.byref(self.mp), S_)
+        _lib.call("vpic_hip_compute_div_e_err", self.fields.ptr, G,
+                  ctypes.byref(self.mp), S_)
+        _lib.call("vpic_hip_compute_rms_div_e_err", ctypes.byref(err),
+                  self.fields.ptr, G, S_)
+        out["div_e_rms"] = err.value
+        _lib.call("vpic_hip_clean_div_e", self.fields.ptr, G,
+                  ctypes.byref(self.mp), S_)
+        _lib.call("vpic_hip_synchronize_tang_e_norm_b", ctypes.byref(err),
+                  self.fields.ptr, G, S_)
+        out["sync_error_after"] = err.value
+        self.load_interpolator_array()
+        for sp in self.species:
+            _lib.call("vpic_hip_center_p", sp.p.ptr, sp.np, sp.q, sp.m,
+                      self.interp.ptr, G, 1, S_)
+        return out
+
     def check_movers(self, sp=None):
         """(nm, movers lost to max_nm, bad voxels) of the last advance_p of
         sp (default: the last species; host read, synchronises)."""
@@ -389,6 +451,18 @@ class PhaseTimer:
         torch.cuda.synchronize()
         return {k: sum(a.elapsed_time(b) for a, b in v)
                 for k, v in self.ev.items()}
+
+
+def host_species(sim, name, q, m, p, sort_interval=0, max_np=None):
+    """A species whose particles (a host particle array) are uploaded."""
+    n = len(p)
+    sp = Species(name, q, m, max(n, max_np or 0), sim.g,
+                 sort_interval=sort_interval)
+    sp.np = n
+    if n:
+        src = torch.from_numpy(np.ascontiguousarray(p).view(np.uint8))
+        sp.p.t[:src.numel()].copy_(src)
+    return sim.add_species(sp)
 
 
 def uniform_plasma(sim, ppc, q, m, uth, seed, name, drift=(0.0, 0.0, 0.0),
