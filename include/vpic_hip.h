@@ -619,6 +619,42 @@ int vpic_hip_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
                                  void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Differentiable run (no counterpart in the reference)                    */
+/* ------------------------------------------------------------------------ */
+
+/* The magnetic field energy after n_step steps of the hot path and, when
+ * u_bar is given, its gradient by a hand-written reverse-mode adjoint.
+ * The step is vpic_simulation::advance (src/vpic/advance.cc:25-185) for one
+ * pushed species on a periodic box (immobile ions are a neutralising
+ * background the step never sees): clear_accumulator_array, advance_p
+ * (advance_p_pipeline.cc:96-238, move_p.cc:233-375), unload_accumulator +
+ * synchronize_jf, advance_b(0.5), vacuum_advance_e with no damping,
+ * advance_b(0.5), load_interpolator_array; the energy is the magnetic part
+ * of energy_f (vacuum_energy_f_pipeline.h:44-75), FP64 sums.
+ *
+ * precision 4: p0 is vpic_particle_t[np] and f0 vpic_field_t[nv] (the
+ * product run, FP32, the reference's push arithmetic); precision 8: the
+ * same members as doubles (64-B particles: dx,dy,dz, int32 i, pad, ux,uy,uz,
+ * w; 144-B fields: 16 doubles then the 8 int16 material ids), the FP64 run
+ * that finite-difference checks use.  Only E and cB of f0 are read; the
+ * initial momenta are taken as given (uncentred already).  The forward
+ * keeps a checkpoint every ckpt_interval steps; the reverse recomputes each
+ * segment from its checkpoint and runs the transposed field operations and
+ * the reverse push (the cell-crossing branch held fixed) backwards.
+ * Outputs (device arrays in the run's precision, may be NULL except u_bar
+ * when gradients are wanted): u_bar[np][3] = dW/d(ux,uy,uz), d_bar[np][3] =
+ * dW/d(dx,dy,dz), f_bar[nv][6] = dW/d(ex,ey,ez,cbx,cby,cbz) of the initial
+ * state.  *host_energy receives W.  All grid faces must be periodic.
+ * Synchronous; allocates its checkpoints (about (n_step/ckpt_interval +
+ * ckpt_interval) particle and field arrays). */
+int vpic_hip_adjoint_bfield_energy(const void *p0, int32_t np, float q,
+                                   float m, const void *f0,
+                                   const vpic_grid_t *g, int32_t n_step,
+                                   int32_t ckpt_interval, int32_t precision,
+                                   double *host_energy, void *u_bar,
+                                   void *d_bar, void *f_bar, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Timing of the push kernel (hipEvents on the launch stream)               */
 /* ------------------------------------------------------------------------ */
 

@@ -90,6 +90,9 @@ _SIGS = {
     "vpic_hip_grid_neighbor_table": [_pg, _vp],
     "vpic_hip_load_uniform_plasma": [_vp, _pg, _i32, _f32, _f32, _f32, _f32,
                                      _f32, _f32, _f32, ctypes.c_uint64, _vp],
+    "vpic_hip_adjoint_bfield_energy": [_vp, _i32, _f32, _f32, _vp, _pg, _i32,
+                                       _i32, _i32, _pf64, _vp, _vp, _vp,
+                                       _vp],
     "vpic_hip_push_timing_enable": [ctypes.c_int],
     "vpic_hip_push_timing": [_pi64, _pf64, _pi64],
 }
