@@ -55,64 +55,176 @@ def build_sim(nx, ny, nz, ppc, seed=2026, sort_interval=20):
     return s
 
 
-def cpu_baseline(nx=64, ny=64, nz=64, ppc=100, budget_s=12.0):
-    """The reference's advance_p on the host cores: N pthreads, each one
-    DISTRIBUTE(np,16) block with a private accumulator copy, the caller on
-    the tail (advance_p_pipeline.cc:253-340).  Sample: the same plasma on a
-    nx*ny*nz-cell box (both species), zero fields."""
+def host_info():
+    """The host's CPU model (lscpu), nproc, and the CPUs this process may
+    run on."""
+    import subprocess
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=20).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"lscpu_model": model, "nproc": os.cpu_count() or 1,
+            "cpus_allowed": allowed}
+
+
+def cpu_threads(info):
+    """Threads for the CPU baseline: every CPU this process may use, capped
+    at the host's share per GPU (OMP_NUM_THREADS where set; the GPU hosts
+    here allot 16 CPUs to each GPU)."""
+    n = info["cpus_allowed"]
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, cap) if cap > 0 else n)
+
+
+def cpu_baseline(s, budget_s=12.0):
+    """The reference's own advance_p (oracle/_ref: advance_p_pipeline.cc
+    compiled from the lanl/vpic sources; else the oracle's restatement) on
+    the host cores, on the benchmark's own arrays: the electron species of
+    the 256x128x128 plasma (419,430,400 particles) as the GPU left it, with
+    the GPU's current interpolators.  N pthreads, each one DISTRIBUTE(np,16)
+    block with a private accumulator copy, the caller on the tail, then the
+    copies reduced (advance_p_pipeline.cc:253-340,
+    reduce_array_pipeline.cc); 3 warm-up calls, then timed calls until
+    budget_s, as sample/bench/advance_p:46-52 times it."""
     import numpy as np
     import torch
     from oracle.oracle import Oracle, Ref, host_grid
     from vpic_amd import types as T
 
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, int(os.environ.get("OMP_NUM_THREADS",
-                                                     threads)), 16))
-    s = build_sim(nx, ny, nz, ppc, seed=99, sort_interval=0)
+    info = host_info()
+    threads = cpu_threads(info)
+    sp = s.species[0]
     torch.cuda.synchronize()
-    parts = [sp.p.download().copy() for sp in s.species]
-    qm = [(sp.q, sp.m) for sp in s.species]
+    log("cpu baseline: copying %d particles to the host" % sp.np)
+    p = sp.p.download()[:sp.np].copy()
+    fi = s.interp.download().copy()
     g = s.g
     hg = host_grid(g)
-    fi = np.zeros(g.nv, dtype=T.INTERPOLATOR)
-    stride = (g.nv + 1) & ~1
     use_ref = Ref.available()
     if use_ref:
         ref = Ref.get()
+        stride = (g.nv + 1) & ~1
         acc = np.zeros(((threads + 1) * stride, 12), dtype=np.float32)
-        pm = np.zeros(max(2 * len(parts[0]) // 25, 16 * 273), dtype=T.MOVER)
+        pm = np.zeros(max(2 * len(p) // 25, 16 * 273), dtype=T.MOVER)
 
-        def push(p, q, m):
-            ref.advance_p_pipelines(p, q, m, acc, fi, hg, threads, pm)
+        def push():
+            ref.advance_p_pipelines(p, sp.q, sp.m, acc, fi, hg, threads, pm)
     else:
         orc = Oracle.get()
         acc = np.zeros(((threads + 1) * g.nv, 12), dtype=np.float32)
 
-        def push(p, q, m):
-            orc.advance_p_threaded(p, q, m, acc, fi, hg, threads,
+        def push():
+            orc.advance_p_threaded(p, sp.q, sp.m, acc, fi, hg, threads,
                                    max(2 * len(p) // 25, 16 * 273))
-    # 3 warm-up calls, then timed calls (sample/bench/advance_p:46-52)
-    for _ in range(3):
-        for p, (q, m) in zip(parts, qm):
-            push(p, q, m)
+    for k in range(3):
+        push()
+        log("cpu baseline: warm-up %d" % (k + 1))
     n_step, t0 = 0, time.perf_counter()
     while True:
-        for p, (q, m) in zip(parts, qm):
-            push(p, q, m)
+        push()
         n_step += 1
         el = time.perf_counter() - t0
         if el >= budget_s or n_step >= 50:
             break
-    npart = sum(len(p) for p in parts)
-    return {"value": npart * n_step / el, "unit": "particle-steps/s",
-            "cores": threads, "kind": "reference" if use_ref else "port",
-            "sample": ("advance_p of both species of the same plasma on a "
-                       "%dx%dx%d-cell box (%d particles), zero fields, %d "
-                       "timed steps after 3 warm-ups, %.1f s"
-                       % (nx, ny, nz, npart, n_step, el))}
+    out = {"value": len(p) * n_step / el, "unit": "particle-steps/s",
+           "cores": threads, "kind": "reference" if use_ref else "port",
+           "sample": ("advance_p of the electron species of the benchmark "
+                      "plasma (%d particles, 256x128x128 cells) as the GPU "
+                      "left it, with its interpolators; %d timed calls after "
+                      "3 warm-ups, %.1f s" % (len(p), n_step, el))}
+    out.update(info)
+    out["threads_note"] = ("threads = the CPUs this process may use, capped "
+                           "at OMP_NUM_THREADS (the host's CPU share per "
+                           "GPU)")
+    return out
+
+
+def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3):
+    """The batched ensemble: nbox counter-streaming beam boxes drawn from
+    the seed (sides from {32,64,96,128}, 16-256 ppc, drift 0.05-0.3c,
+    spread 0.01c, immobile ions), tuned, then `steps` graph-replayed steps
+    timed; runs/s and aggregate particle-steps/s."""
+    import torch
+    from vpic_amd import ensemble as E
+    cfg = E.beam_config(nbox, seed)
+    t0 = time.perf_counter()
+    e = E.beam_ensemble(cfg, seed)
+    npart = sum(e.np)
+    log("ensemble: %d boxes, %d particles, loaded in %.1f s"
+        % (nbox, npart, time.perf_counter() - t0))
+    shape, tuned_ms = e.tune(tune_steps)
+    log("ensemble: tuned shape %s (%.2f ms per step)" % (shape, tuned_ms))
+    e.step(2)  # settle
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps)
+    el = time.perf_counter() - t0
+    e.close()
+    return {"boxes": nbox, "particles": npart, "steps_per_run": steps,
+            "runs_per_s": 1.0 / el, "s_per_run": el,
+            "particle_steps_per_s": npart * steps / el,
+            "ms_per_step": el * 1e3 / steps,
+            "tuned_shape": {"push_bs": shape[0], "push_chunk": shape[1],
+                            "field_bs": shape[2], "field_chunk": shape[3],
+                            "blocks_per_cu": shape[4]},
+            "config": "256 2D periodic counter-streaming electron-beam boxes, "
+                      "seed %d, hipGraph-replayed step" % seed}
+
+
+def adjoint_probe(n=64, ppc=32, steps=200, ckpt=20):
+    """The differentiable Weibel run: n^3 cells (dx = 0.1 d_e), ppc
+    bi-Maxwellian electrons (T_perp/T_par = 4, uth_par = 0.05c), immobile
+    ions, `steps` steps; the forward alone and the forward with the
+    checkpointed adjoint (gradient of the final magnetic energy with respect
+    to every initial momentum), FP32, timed on the device."""
+    import ctypes
+    import numpy as np
+    import torch
+    from vpic_amd import _lib
+    from vpic_amd import device
+    from vpic_amd import types as T
+    L = n * 0.1
+    dt = 0.95 * T.courant_length(L, L, L, n, n, n)
+    g = T.grid_periodic(0, 0, 0, L, L, L, n, n, n, dt)
+    npart = ppc * n ** 3
+    p = device.particles(npart)
+    _lib.call("vpic_hip_load_uniform_plasma", p.ptr, ctypes.byref(g), ppc,
+              0.05, 0.1, 0.1, 0.0, 0.0, 0.0, 0.001 / ppc, 17, None)
+    f = device.fields(g)
+    f.zero()
+    ub = torch.empty((npart, 3), dtype=torch.float32, device="cuda")
+    en = ctypes.c_double()
+
+    def call(grads):
+        _lib.call("vpic_hip_adjoint_bfield_energy", p.ptr, npart, -1.0, 1.0,
+                  f.ptr, ctypes.byref(g), steps, ckpt, 4, ctypes.byref(en),
+                  ub.data_ptr() if grads else None, None, None, None)
+    call(False)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    call(False)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    call(True)
+    t_adj = time.perf_counter() - t0
+    gnorm = float(ub.double().norm())
+    return {"cells": n ** 3, "particles": npart, "steps": steps,
+            "checkpoint_interval": ckpt, "forward_s": t_fwd,
+            "forward_and_adjoint_s": t_adj,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "energy": en.value, "grad_norm": gnorm,
+            "config": "3D Weibel 64^3, 32 ppc electrons, T_perp/T_par=4, "
+                      "uth_par=0.05c, immobile ions, FP32"}
 
 
 def main():
@@ -125,6 +237,10 @@ def main():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiable-run measurement")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tile", type=str, default=None,
                     help="push tile tx,ty,tz (cells)")
@@ -336,9 +452,17 @@ def main():
             out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch")
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(budget_s=args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(s, budget_s=args.cpu_budget)
     else:
         out["cpu_baseline"] = None
+    del s
+    torch.cuda.empty_cache()
+    if not args.no_ensemble:
+        log("ensemble ...")
+        out["ensemble"] = ensemble_probe()
+    if not args.no_adjoint:
+        log("adjoint ...")
+        out["adjoint"] = adjoint_probe()
     print(json.dumps(out))
 
 

@@ -274,6 +274,14 @@ double energy_p(const species_t *sp, const interpolator_array_t *ia);
  * density into fa->f[].rhof. */
 void accumulate_rho_p(field_array_t *fa, const species_t *sp);
 
+/* species_advance.h:133-136 (rho_p.cc:126-173): the charge of one
+ * particle (voxel index restored, not 8*voxel+face) on the 8 nodes of its
+ * voxel into f[].rhob, doubled on the local domain's faces; called by
+ * vpic_simulation::advance for movers boundary_p could not place
+ * (advance.cc:78-101).  f is the field array's host array. */
+void accumulate_rhob(field_t *f, const particle_t *p, const grid_t *g,
+                     const float qsp);
+
 /* species_advance.h:84-100 (center_p_pipeline.cc, uncenter_p_pipeline.cc):
  * half Boris step forward / backward in time of the momenta. */
 void center_p(species_t *sp, const interpolator_array_t *ia);

@@ -276,6 +276,17 @@ int vpic_hip_takizuka_abe(vpic_particle_t *pi, const int32_t *part_i,
                           int32_t interval, uint64_t seed, uint64_t call,
                           const vpic_grid_t *g, void *stream);
 
+/* accumulate_rhob (rho_p.cc:126-173; species_advance.h:133-136) of the n
+ * particles p (device): each particle's charge q w on the 8 nodes of its
+ * voxel into f[].rhob, a node weight doubled on each face of the local
+ * domain the voxel touches -- what boundary_p does for an absorbed
+ * particle, and vpic_simulation::advance for a mover it could not place
+ * (advance.cc:78-101).  Synchronous (particles whose nodes are not on the
+ * grid are skipped and reported). */
+int vpic_hip_accumulate_rhob(vpic_field_t *f, const vpic_particle_t *p,
+                             int32_t n, float qsp, const vpic_grid_t *g,
+                             void *stream);
+
 /* boundary_p for one species on a domain with no face shared with another
  * rank (boundary_p.cc:42-750; the loop of :257-375).  pm[0..nm) are the
  * movers advance_p left, in increasing particle order; each one's particle
@@ -608,6 +619,11 @@ int vpic_hip_field_params_vacuum(vpic_field_params_t *mp, float damp);
  * build for this one-rank grid (src/grid/ops.cc:45-165). */
 int vpic_hip_grid_neighbor_table(const vpic_grid_t *g, int64_t *host_nb);
 
+/* Copy `bytes` between any two host or device buffers (hipMemcpyDefault),
+ * synchronous on `stream`: how a host caller fills or reads arrays it only
+ * holds device pointers to (e.g. an ensemble box's). */
+int vpic_hip_copy(void *dst, const void *src, int64_t bytes, void *stream);
+
 /* Load a uniform thermal plasma: `ppc` particles per interior cell,
  * positions uniform in the cell, momenta N(drift, uth) per axis, weight w;
  * counter-based RNG keyed by `seed`.  Particle k belongs to cell
@@ -617,6 +633,57 @@ int vpic_hip_load_uniform_plasma(vpic_particle_t *p, const vpic_grid_t *g,
                                  float uth_z, float ud_x, float ud_y,
                                  float ud_z, float w, uint64_t seed,
                                  void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched ensemble of independent boxes (no counterpart in the reference) */
+/* ------------------------------------------------------------------------ */
+
+/* Many small independent periodic boxes advanced together.  Box b has its
+ * own grid (any size; all faces periodic, no neighbor table) and one pushed
+ * species of np[b] particles with charge q[b] and mass m[b] (immobile ions
+ * are a neutralising background the step never sees).  The boxes' arrays
+ * are packed one after another in HBM, each in the reference's single-grid
+ * layout (particle_t with box-local voxel indices, field_t, interpolator_t,
+ * accumulator_t); vpic_hip_ensemble_box returns a box's device pointers so
+ * its particles and fields can be loaded and read with the single-grid
+ * entry points.  A step is vpic_simulation::advance's hot path
+ * (src/vpic/advance.cc:25-185) for every box: one persistent launch per
+ * phase over a work queue of (box, range) items, the whole step captured in
+ * a hipGraph and replayed.  Fields start at zero; after loading the
+ * particles (and any initial fields) call vpic_hip_ensemble_prepare (the
+ * interpolators).  `stream` carries every launch (NULL: the ensemble
+ * creates a stream of its own, as a graph cannot be captured on the legacy
+ * default stream). */
+typedef struct vpic_ensemble vpic_ensemble_t;
+
+int vpic_hip_ensemble_create(vpic_ensemble_t **e, int32_t nbox,
+                             const vpic_grid_t *grids, const int32_t *np,
+                             const float *q, const float *m, void *stream);
+int vpic_hip_ensemble_destroy(vpic_ensemble_t *e);
+/* Device pointers of box b's particles, fields, interpolators and
+ * accumulators inside the packed arrays (any may be NULL). */
+int vpic_hip_ensemble_box(vpic_ensemble_t *e, int32_t b, vpic_particle_t **p,
+                          int32_t *np, vpic_field_t **f,
+                          vpic_interpolator_t **fi, vpic_accumulator_t **a);
+/* load_interpolator_array of every box (after loading particles/fields). */
+int vpic_hip_ensemble_prepare(vpic_ensemble_t *e);
+/* nstep steps of every box; use_graph replays the captured step (captured
+ * at the first call and after every shape change), else the phases are
+ * launched one by one.  Synchronous; fails if a particle left its box. */
+int vpic_hip_ensemble_step(vpic_ensemble_t *e, int32_t nstep,
+                           int32_t use_graph);
+/* Launch shapes: push block size (64..256) and particles per work item,
+ * field block size (64..512) and voxels / face elements per work item, and
+ * persistent blocks per compute unit. */
+int vpic_hip_ensemble_shape(vpic_ensemble_t *e, int32_t push_bs,
+                            int32_t push_chunk, int32_t field_bs,
+                            int32_t field_chunk, int32_t blocks_per_cu);
+int vpic_hip_ensemble_get_shape(const vpic_ensemble_t *e, int32_t out[5]);
+/* The tuner: times each candidate shape over trial_steps graph replays from
+ * the same saved state (restored afterwards), keeps the fastest and
+ * returns its ms per step. */
+int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
+                           double *host_best_ms);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiable run (no counterpart in the reference)                    */

@@ -51,6 +51,8 @@ void ref_energy_f(double en[6], const vpic_field_t *f, const vpic_grid_t *g,
 void ref_accumulate_hydro_p(vpic_hydro_t *h, const vpic_particle_t *p, int np,
                             float q, float m, const vpic_interpolator_t *f,
                             const vpic_grid_t *g);
+void ref_accumulate_rhob(vpic_field_t *f, const vpic_particle_t *p, int np,
+                         float qsp, const vpic_grid_t *g);
 }
 
 static int g_fail = 0;
@@ -433,6 +435,19 @@ static void check_species_kernels() {
     }
     report(num <= 1e-5 * den, "accumulate_rho_p + synchronize_rho",
            "max |d rhof| %.3g of %.3g", num, den);
+  }
+  // accumulate_rhob of single particles (as advance.cc:78-101 calls it)
+  // against the reference's own function, one particle after another
+  {
+    std::vector<field_t> f0(A.f);
+    const int n = e.sp.np < 64 ? e.sp.np : 64;
+    for (int k = 0; k < n; k++)
+      accumulate_rhob(A.f.data(), &e.p[k], e.sp.g, e.sp.q);
+    ref_accumulate_rhob((vpic_field_t *)f0.data(),
+                        (const vpic_particle_t *)e.p.data(), n, e.sp.q,
+                        &b.G);
+    report(same(A.f.data(), f0.data(), sizeof(field_t) * b.G.nv),
+           "accumulate_rhob", "fields differ");
   }
   // hydro moments in the sequence of dump.cc:240-242 (synchronize reduces
   // first, hydro_array.cc:145-147); sorted (tiled) and unsorted (flat)

@@ -20,13 +20,17 @@ field energies and each species' kinetic energy, FP64 sums; and the rms
 div E error) and their bounds, all measured first on the CPU (FP32 oracle
 against FP64 oracle) and then stated here:
   * thermal plasma (the 32^3, 2 x 64 ppc, 100-step box and reduced boxes):
-    electric and magnetic energy sums within 2e-4 relative of the FP64
+    electric and magnetic energy sums within 1e-3 relative of the FP64
     trace, kinetic energy within 2e-5, total energy within 2e-5, rms div E
     error below 2e-6 of the mean charge density (FP32 rounding of the node
-    sums, flat over the run) -- measured 1.5e-5, 2.7e-5, 2.4e-6, 2.3e-6
-    and 5e-7 on the 8^3 box;
-  * Harris sheet (walls, nonuniform density, heavy crossing): as above
-    with the field sums within 1e-3.
+    sums, flat over the run).  Measured, oracle FP32 against FP64: 1.6e-4,
+    2.5e-4, 3.3e-6, 3.0e-6 and 6.1e-7 on the 32^3 box (1.5e-5, 2.7e-5,
+    2.4e-6, 2.3e-6 and 5.9e-7 on the 8^3 box); the field energies are the
+    plasma's noise, whose FP32 and FP64 realizations drift apart as the
+    particle orbits do;
+  * Harris sheet (walls, nonuniform density, heavy crossing): the same
+    bounds (measured 2.0e-4, 1.6e-6, 1.8e-6, 5.1e-7, 2.1e-7 on a 64 x 32
+    sheet over 200 steps).
 The GPU traces (FP32 push, FP64 device reductions) must meet the same
 bounds against the FP64 oracle, and their own rms div E error the same
 ceiling.
@@ -64,8 +68,8 @@ GOLD_3D = os.path.join(os.path.dirname(__file__), "golden",
                        "energies_gold.3d_test")
 
 # trace bounds (see the module docstring)
-THERMAL = {"E": 2e-4, "B": 2e-4, "KE": 2e-5, "total": 2e-5, "div": 2e-6}
-HARRIS = {"E": 1e-3, "B": 1e-3, "KE": 2e-5, "total": 2e-5, "div": 2e-6}
+THERMAL = {"E": 1e-3, "B": 1e-3, "KE": 2e-5, "total": 2e-5, "div": 2e-6}
+HARRIS = THERMAL
 
 
 def say(*a):

@@ -68,6 +68,7 @@ _SIGS = {
     "vpic_hip_synchronize_hydro_array": [_vp, _pg, _vp],
     "vpic_hip_boundary_p": [_vp, _pi32, _vp, _i32, _vp, _f32, _pg, _vp,
                             _pi32, _vp],
+    "vpic_hip_accumulate_rhob": [_vp, _vp, _i32, _f32, _pg, _vp],
     "vpic_hip_inject_p": [_vp, _pi32, _i32, _vp, _pi32, _i32, _vp, _i32, _vp,
                           _f32, _pg, _pi32, _vp],
     "vpic_hip_write_energies": [ctypes.c_char_p, ctypes.c_int, _pdi, _f32,
@@ -88,11 +89,20 @@ _SIGS = {
                                _i32, _i32, _f32, _f32, _f32],
     "vpic_hip_field_params_vacuum": [_pmp, _f32],
     "vpic_hip_grid_neighbor_table": [_pg, _vp],
+    "vpic_hip_copy": [_vp, _vp, _i64, _vp],
     "vpic_hip_load_uniform_plasma": [_vp, _pg, _i32, _f32, _f32, _f32, _f32,
                                      _f32, _f32, _f32, ctypes.c_uint64, _vp],
     "vpic_hip_adjoint_bfield_energy": [_vp, _i32, _f32, _f32, _vp, _pg, _i32,
                                        _i32, _i32, _pf64, _vp, _vp, _vp,
                                        _vp],
+    "vpic_hip_ensemble_create": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
+    "vpic_hip_ensemble_destroy": [_vp],
+    "vpic_hip_ensemble_box": [_vp, _i32, _vp, _vp, _vp, _vp, _vp],
+    "vpic_hip_ensemble_prepare": [_vp],
+    "vpic_hip_ensemble_step": [_vp, _i32, _i32],
+    "vpic_hip_ensemble_shape": [_vp, _i32, _i32, _i32, _i32, _i32],
+    "vpic_hip_ensemble_get_shape": [_vp, _vp],
+    "vpic_hip_ensemble_tune": [_vp, _i32, _pf64],
     "vpic_hip_push_timing_enable": [ctypes.c_int],
     "vpic_hip_push_timing": [_pi64, _pf64, _pi64],
 }

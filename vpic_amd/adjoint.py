@@ -93,9 +93,9 @@ def host_particles64(p32):
     out = np.zeros((len(p32), 8), dtype=np.float64)
     for k, c in enumerate(("dx", "dy", "dz")):
         out[:, k] = p32[c]
-    iv = out[:, 3].view(np.int32).reshape(-1, 2)
-    iv[:, 0] = p32["i"]
-    iv[:, 1] = 0
+    iv = out.view(np.int32)  # (np, 16): words 6, 7 are column 3
+    iv[:, 6] = p32["i"]
+    iv[:, 7] = 0
     for k, c in enumerate(("ux", "uy", "uz", "w")):
         out[:, 4 + k] = p32[c]
     return out

@@ -292,6 +292,30 @@ extern "C" int vpic_hip_boundary_p(vpic_particle_t *p, int32_t *np,
   return launch_boundary_p(p, np, pm, nm, f, q, g, hits, counts, S(stream));
 }
 
+extern "C" int vpic_hip_accumulate_rhob(vpic_field_t *f,
+                                        const vpic_particle_t *p, int32_t n,
+                                        float qsp, const vpic_grid_t *g,
+                                        void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(n >= 0, "n < 0");
+  if (n == 0) return VPIC_OK;
+  CHECK(f && p, "NULL argument");
+  DeviceScratch &ns = nm_scratch();
+  if ((rc = scratch_reserve(ns, 64))) return rc;
+  VPIC_HIP(hipMemsetAsync(ns.ptr, 0, sizeof(int), S(stream)));
+  if ((rc = launch_accumulate_rhob(f, p, n, qsp, g, (int *)ns.ptr,
+                                   S(stream))))
+    return rc;
+  int bad = 0;
+  VPIC_HIP(hipMemcpyAsync(&bad, ns.ptr, sizeof(int), hipMemcpyDeviceToHost,
+                          S(stream)));
+  VPIC_HIP(hipStreamSynchronize(S(stream)));
+  CHECK(bad == 0, "accumulate_rhob: %d particles with a voxel whose nodes "
+        "are off the grid were skipped", bad);
+  return VPIC_OK;
+}
+
 extern "C" int vpic_hip_inject_p(vpic_particle_t *p, int32_t *np,
                                  int32_t max_np, vpic_particle_mover_t *pm,
                                  int32_t *nm, int32_t max_nm,
@@ -652,6 +676,16 @@ extern "C" int vpic_hip_field_params_vacuum(vpic_field_params_t *mp,
   mp->epsx = mp->epsy = mp->epsz = 1.0f;
   mp->damp = damp;
   mp->nonconductive = 1.0f;  // sfa.cc:144 with ax = ay = az = 0
+  return VPIC_OK;
+}
+
+extern "C" int vpic_hip_copy(void *dst, const void *src, int64_t bytes,
+                             void *stream) {
+  CHECK(bytes >= 0 && (bytes == 0 || (dst && src)), "copy: bad arguments");
+  if (!bytes) return VPIC_OK;
+  VPIC_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault,
+                          S(stream)));
+  VPIC_HIP(hipStreamSynchronize(S(stream)));
   return VPIC_OK;
 }
 

@@ -174,6 +174,38 @@ BoundaryScratch &boundary_scratch() {
 
 }  // namespace
 
+namespace {
+// accumulate_rhob (rho_p.cc:126-173) of particles p[0..n), one lane each;
+// a particle whose 8 nodes are not all on the grid is counted in bad[0].
+__global__ void k_accumulate_rhob(vpic_field_t *__restrict__ f,
+                                  const vpic_particle_t *__restrict__ p, int n,
+                                  float qsp, float r8V, int nx, int ny, int nz,
+                                  int sy, int sz, int nv, int *bad) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += gridDim.x * blockDim.x) {
+    const vpic_particle_t pp = p[k];
+    if (pp.i < 0 || pp.i + sz + sy + 1 >= nv) {
+      atomicAdd(bad, 1);
+      continue;
+    }
+    rhob_deposit(f, pp, qsp, r8V, pp.i, nx, ny, nz, sy, sz);
+  }
+}
+}  // namespace
+
+int launch_accumulate_rhob(vpic_field_t *f, const vpic_particle_t *p, int n,
+                           float qsp, const vpic_grid_t *g, int *dev_bad,
+                           hipStream_t s) {
+  if (n <= 0) return VPIC_OK;
+  int blk = (n + 255) / 256;
+  if (blk > 4096) blk = 4096;
+  hipLaunchKernelGGL(k_accumulate_rhob, dim3(blk), dim3(256), 0, s, f, p, n,
+                     qsp, g->r8V, g->nx, g->ny, g->nz, g->sy, g->sz, g->nv,
+                     dev_bad);
+  VPIC_LAUNCHED("k_accumulate_rhob");
+  return VPIC_OK;
+}
+
 int launch_boundary_p(vpic_particle_t *p, int *np_host,
                       const vpic_particle_mover_t *pm, int nm,
                       vpic_field_t *f, float q, const vpic_grid_t *g,

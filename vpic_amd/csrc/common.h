@@ -135,6 +135,50 @@ enum {
   VPIC_EUNSUPPORTED = -4
 };
 
+// vacuum_advance_e couplings (vacuum_advance_e_pipeline.h:27-33).
+struct ECoef {
+  float decayx, drivex, decayy, drivey, decayz, drivez, damp;
+  float px_muz, px_muy, py_mux, py_muz, pz_muy, pz_mux, cj;
+};
+
+// A box of a batched ensemble (packed arrays: its voxels start at voff).
+struct EnsFieldBox {
+  int nx, ny, nz;
+  int64_t voff;
+  float bp[3];  // advance_b(0.5) couplings
+  float uc[3];  // unload couplings
+  ECoef ec;     // vacuum_advance_e couplings
+};
+
+// A work item of a batched phase: voxels (axis < 0), face elements of one
+// axis, or particles of one box; linear indices [first, first + count).
+struct EnsItem {
+  int box, axis, first, count;
+};
+
+enum { ENS_UNLOAD, ENS_ADVB, ENS_ADVE, ENS_LOAD, ENS_GHOST, ENS_FOLD };
+
+// A box of a batched ensemble as the push sees it.
+struct EnsPushBox {
+  float qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
+  GridDev G;
+  int64_t poff, voff;
+};
+
+void ens_make_field_box(EnsFieldBox *b, const vpic_grid_t *g,
+                        const vpic_field_params_t *m, int64_t voff);
+int launch_ens_field(int phase, vpic_field_t *F, vpic_interpolator_t *FI,
+                     const vpic_accumulator_t *A, const EnsFieldBox *boxes,
+                     const EnsItem *items, int nitems, int *queue,
+                     int nblocks, int bs, hipStream_t s);
+int launch_ens_reset(int *queues, int n, hipStream_t s);
+int launch_clear_accumulator_n(vpic_accumulator_t *a, int64_t nv,
+                               hipStream_t s);
+int launch_ens_push(vpic_particle_t *P, const vpic_interpolator_t *FI,
+                    vpic_accumulator_t *A, const EnsPushBox *boxes,
+                    const EnsItem *items, int nitems, int *queue, int *nbad,
+                    int nblocks, int bs, hipStream_t s);
+
 // Per-device scratch kept by the library (grown on demand).
 struct DeviceScratch {
   void *ptr = nullptr;
@@ -171,6 +215,9 @@ int launch_takizuka_abe(vpic_particle_t *pi, const int *part_i, int np_i,
                         const int *part_j, int np_j, float qj, float mj,
                         double cvar0, int interval, uint64_t seed,
                         uint64_t call, const vpic_grid_t *g, hipStream_t s);
+int launch_accumulate_rhob(vpic_field_t *f, const vpic_particle_t *p, int n,
+                           float qsp, const vpic_grid_t *g, int *dev_bad,
+                           hipStream_t s);
 int launch_inject_p(vpic_particle_t *p, int *np, int max_np,
                     vpic_particle_mover_t *pm, int *nm, int max_nm,
                     const vpic_particle_injector_t *pi, int n,

@@ -510,6 +510,32 @@ extern "C" void accumulate_rho_p(field_array_t *fa, const species_t *sp) {
   dev_wrote(fn, fa->f, fbytes(fa->g));
 }
 
+// accumulate_rhob (rho_p.cc:126-173) of one host particle into the host
+// field array f: the particle is staged to the device and deposited into
+// f's device copy.
+extern "C" void accumulate_rhob(field_t *f, const particle_t *p,
+                                const grid_t *g, const float qsp) {
+  const char *fn = "accumulate_rhob";
+  if (!f || !p || !g) {
+    fail(fn, "Bad args");
+    return;
+  }
+  if (!ensure_init(fn)) return;
+  vpic_grid_t G;
+  if (!make_grid(fn, g, &G)) return;
+  const size_t fb = sizeof(field_t) * (size_t)g->nv;
+  vpic_field_t *fd;
+  if (!dev_in(fn, f, fb, fb, &fd)) return;
+  HIPOK_V(fn, hipMemcpy(st().one, p, sizeof(particle_t),
+                        hipMemcpyHostToDevice));
+  if (vpic_hip_accumulate_rhob(fd, (const vpic_particle_t *)st().one, 1, qsp,
+                               &G, nullptr) != 0) {
+    fail(fn, "%s", vpic_hip_last_error());
+    return;
+  }
+  dev_wrote(fn, f, fb);
+}
+
 static void center(const char *fn, species_t *sp,
                    const interpolator_array_t *ia, int uncenter) {
   if (!sp || !ia || sp->g != ia->g || !ia->i) {

@@ -49,12 +49,9 @@ __device__ __forceinline__ const float4 *F4(const vpic_field_t *f, int v) {
 // ---------------------------------------------------------------------------
 // load_interpolator_array
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_load_interpolator(
+__device__ __forceinline__ void load_interpolator_at(
     vpic_interpolator_t *__restrict__ fi, const vpic_field_t *__restrict__ f,
-    Box b, int nx, int ny) {
-  int x, y, z;
-  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
-    return;
+    int x, int y, int z, int nx, int ny) {
   const float fourth = 0.25f, half = 0.50f;
   const int v = vox(x, y, z, nx, ny);
   const int sy = nx + 2, sz = (ny + 2) * sy;
@@ -98,6 +95,15 @@ __global__ __launch_bounds__(256) void k_load_interpolator(
       make_float2(half * (w1 + w0), half * (w1 - w0));
 }
 
+__global__ __launch_bounds__(256) void k_load_interpolator(
+    vpic_interpolator_t *__restrict__ fi, const vpic_field_t *__restrict__ f,
+    Box b, int nx, int ny) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  load_interpolator_at(fi, f, x, y, z, nx, ny);
+}
+
 // ---------------------------------------------------------------------------
 // accumulators and jf
 // ---------------------------------------------------------------------------
@@ -121,12 +127,9 @@ __global__ __launch_bounds__(256) void k_clear_jf(vpic_field_t *f, int nv,
   }
 }
 
-__global__ __launch_bounds__(256) void k_unload_accumulator(
+__device__ __forceinline__ void unload_accumulator_at(
     vpic_field_t *__restrict__ f, const vpic_accumulator_t *__restrict__ a,
-    Box b, int nx, int ny, float cx, float cy, float cz) {
-  int x, y, z;
-  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
-    return;
+    int x, int y, int z, int nx, int ny, float cx, float cy, float cz) {
   const int v = vox(x, y, z, nx, ny);
   const int sy = nx + 2, sz = (ny + 2) * sy;
   const vpic_accumulator_t *a0 = a + v, *ax = a + v - 1, *ay = a + v - sy,
@@ -137,6 +140,15 @@ __global__ __launch_bounds__(256) void k_unload_accumulator(
   jf.y += cy * (a0->jy[0] + az->jy[1] + ax->jy[2] + azx->jy[3]);
   jf.z += cz * (a0->jz[0] + ax->jz[1] + ay->jz[2] + axy->jz[3]);
   F4(f, v)[3] = jf;
+}
+
+__global__ __launch_bounds__(256) void k_unload_accumulator(
+    vpic_field_t *__restrict__ f, const vpic_accumulator_t *__restrict__ a,
+    Box b, int nx, int ny, float cx, float cy, float cz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  unload_accumulator_at(f, a, x, y, z, nx, ny, cx, cy, cz);
 }
 
 // Rotated face addressing: normal axis X, tangential Y=(X+1)%3, Z=(X+2)%3.
@@ -180,12 +192,12 @@ __global__ void k_adjust_jf(vpic_field_t *f, FaceGeom G, int k, int mode) {
 }
 
 // One axis of the periodic jf fold (remote.cc:474-502 with lw = rw = 1).
-__global__ void k_fold_jf(vpic_field_t *f, FaceGeom G, int X) {
+__device__ __forceinline__ void fold_jf_at(vpic_field_t *f, const FaceGeom &G,
+                                           int X, int l) {
   const int Y = (X + 1) % 3, Z = (X + 2) % 3;
-  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const int wy = G.n[Y] + 1;
   const float lw = 1.0f, rw = 1.0f;
-  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
-       l += gridDim.x * blockDim.x) {
+  {
     const int yy = 1 + l % wy, zz = 1 + l / wy;
     const int lo = face_vox(G, X, 1, yy, zz);
     const int hi = face_vox(G, X, G.n[X] + 1, yy, zz);
@@ -202,6 +214,14 @@ __global__ void k_fold_jf(vpic_field_t *f, FaceGeom G, int X) {
       a = lw * sa + rw * sb;
     }
   }
+}
+
+__global__ void k_fold_jf(vpic_field_t *f, FaceGeom G, int X) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x)
+    fold_jf_at(f, G, X, l);
 }
 
 // local_adjust_rhof + local_adjust_rhob (local.cc:376-440) on one local face:
@@ -246,13 +266,10 @@ __global__ void k_fold_rho(vpic_field_t *f, FaceGeom G, int X) {
 // ---------------------------------------------------------------------------
 // advance_b
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_advance_b(vpic_field_t *__restrict__ f,
-                                                   Box b, int nx, int ny,
-                                                   int nz, float px, float py,
-                                                   float pz) {
-  int x, y, z;
-  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
-    return;
+__device__ __forceinline__ void advance_b_at(vpic_field_t *__restrict__ f,
+                                             int x, int y, int z, int nx,
+                                             int ny, int nz, float px,
+                                             float py, float pz) {
   const int v = vox(x, y, z, nx, ny);
   const int sy = nx + 2, sz = (ny + 2) * sy;
   const float4 e0 = F4(f, v)[0];
@@ -268,6 +285,16 @@ __global__ __launch_bounds__(256) void k_advance_b(vpic_field_t *__restrict__ f,
   if (ux && uz) b0.y -= (pz * (ez1.x - e0.x) - px * (ex1.z - e0.z));
   if (ux && uy) b0.z -= (px * (ex1.y - e0.y) - py * (ey1.x - e0.x));
   if ((uy && uz) || (ux && uz) || (ux && uy)) F4(f, v)[1] = b0;
+}
+
+__global__ __launch_bounds__(256) void k_advance_b(vpic_field_t *__restrict__ f,
+                                                   Box b, int nx, int ny,
+                                                   int nz, float px, float py,
+                                                   float pz) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  advance_b_at(f, x, y, z, nx, ny, nz, px, py, pz);
 }
 
 // local_adjust_norm_b (local.cc:266-296): symmetric faces zero cb normal.
@@ -294,14 +321,14 @@ __global__ void k_adjust_norm_b(vpic_field_t *f, FaceGeom G, int4 m0,
 // take the opposite interior face with rw = 1, lw = 0 (remote.cc:105-116 for
 // equal cell sizes); local faces copy (anti-symmetric) or negate
 // (symmetric/pmc) the first interior plane (local.cc:67-81).
-__global__ void k_ghost_tang_b(vpic_field_t *f, FaceGeom G, int X, int mlo,
-                               int mhi) {
+__device__ __forceinline__ void ghost_tang_b_at(vpic_field_t *f,
+                                                const FaceGeom &G, int X,
+                                                int mlo, int mhi, int l) {
   const int Y = (X + 1) % 3, Z = (X + 2) % 3;
   const int nX = G.n[X];
-  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  const int wy = G.n[Y] + 1;
   const float rw = 1.0f, lw = 0.0f;
-  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
-       l += gridDim.x * blockDim.x) {
+  {
     const int yy = 1 + l % wy, zz = 1 + l / wy;
     const int g0 = face_vox(G, X, 0, yy, zz);
     const int g1 = face_vox(G, X, nX + 1, yy, zz);
@@ -331,17 +358,19 @@ __global__ void k_ghost_tang_b(vpic_field_t *f, FaceGeom G, int X, int mlo,
   }
 }
 
-struct ECoef {
-  float decayx, drivex, decayy, drivey, decayz, drivez, damp;
-  float px_muz, px_muy, py_mux, py_muz, pz_muy, pz_mux, cj;
-};
+__global__ void k_ghost_tang_b(vpic_field_t *f, FaceGeom G, int X, int mlo,
+                               int mhi) {
+  const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+  const int wy = G.n[Y] + 1, wz = G.n[Z] + 1;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < wy * wz;
+       l += gridDim.x * blockDim.x)
+    ghost_tang_b_at(f, G, X, mlo, mhi, l);
+}
 
-__global__ __launch_bounds__(256) void k_advance_e(vpic_field_t *__restrict__ f,
-                                                   Box b, int nx, int ny,
-                                                   int nz, ECoef c) {
-  int x, y, z;
-  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
-    return;
+
+__device__ __forceinline__ void advance_e_at(vpic_field_t *__restrict__ f,
+                                             int x, int y, int z, int nx,
+                                             int ny, int nz, const ECoef &c) {
   const int v = vox(x, y, z, nx, ny);
   const int sy = nx + 2, sz = (ny + 2) * sy;
   float4 e0 = F4(f, v)[0];
@@ -367,6 +396,15 @@ __global__ __launch_bounds__(256) void k_advance_e(vpic_field_t *__restrict__ f,
   }
   F4(f, v)[0] = e0;
   F4(f, v)[2] = t0;
+}
+
+__global__ __launch_bounds__(256) void k_advance_e(vpic_field_t *__restrict__ f,
+                                                   Box b, int nx, int ny,
+                                                   int nz, ECoef c) {
+  int x, y, z;
+  if (!box_coords(b, blockIdx.x * (int64_t)blockDim.x + threadIdx.x, x, y, z))
+    return;
+  advance_e_at(f, x, y, z, nx, ny, nz, c);
 }
 
 // local_adjust_tang_e (local.cc:224-264): PEC faces zero e and tca tangential.
@@ -752,7 +790,150 @@ EnergyScratch &energy_scratch() {
   return s[dev & 15];
 }
 
+// ---------------------------------------------------------------------------
+// Batched boxes: one persistent launch per phase over a work queue of
+// (box, range) items; each box's voxels run the single-grid kernels' bodies
+// (same arithmetic), at the box's offset in the packed arrays.
+// ---------------------------------------------------------------------------
+template <int PHASE>
+__global__ __launch_bounds__(512) void k_ens_field(
+    vpic_field_t *__restrict__ F, vpic_interpolator_t *__restrict__ FI,
+    const vpic_accumulator_t *__restrict__ A,
+    const EnsFieldBox *__restrict__ boxes, const EnsItem *__restrict__ items,
+    int nitems, int *queue) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const EnsItem I = items[it];
+    const EnsFieldBox b = boxes[I.box];
+    vpic_field_t *f = F + b.voff;
+    const int nx = b.nx, ny = b.ny, nz = b.nz;
+    for (int l = I.first + threadIdx.x; l < I.first + I.count;
+         l += blockDim.x) {
+      if (PHASE == ENS_GHOST || PHASE == ENS_FOLD) {
+        FaceGeom G;
+        G.n[0] = nx;
+        G.n[1] = ny;
+        G.n[2] = nz;
+        G.s[0] = 1;
+        G.s[1] = nx + 2;
+        G.s[2] = (nx + 2) * (ny + 2);
+        if (PHASE == ENS_GHOST)
+          ghost_tang_b_at(f, G, I.axis, VPIC_BC_PERIODIC, VPIC_BC_PERIODIC, l);
+        else
+          fold_jf_at(f, G, I.axis, l);
+        continue;
+      }
+      // voxel phases: l runs over (1..nx+1) x (1..ny+1) x (1..nz+1)
+      const int wx = nx + 1, wy = ny + 1;
+      const int x = 1 + l % wx, t = l / wx, y = 1 + t % wy, z = 1 + t / wy;
+      if (PHASE == ENS_UNLOAD) {
+        float4 *jf = reinterpret_cast<float4 *>(f + vox(x, y, z, nx, ny)) + 3;
+        const float4 j0 = *jf;
+        *jf = make_float4(0.0f, 0.0f, 0.0f, j0.w);  // clear_jf
+        unload_accumulator_at(f, A + b.voff, x, y, z, nx, ny, b.uc[0], b.uc[1],
+                              b.uc[2]);
+      } else if (PHASE == ENS_ADVB) {
+        advance_b_at(f, x, y, z, nx, ny, nz, b.bp[0], b.bp[1], b.bp[2]);
+      } else if (PHASE == ENS_ADVE) {
+        advance_e_at(f, x, y, z, nx, ny, nz, b.ec);
+      } else if (PHASE == ENS_LOAD) {
+        if (x <= nx && y <= ny && z <= nz)
+          load_interpolator_at(FI + b.voff, f, x, y, z, nx, ny);
+      }
+    }
+  }
+}
+
+__global__ void k_ens_reset(int *queues, int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) queues[k] = 0;
+}
+
 }  // namespace
+
+// The vacuum_advance_e couplings of grid g (vacuum_advance_e_pipeline.h:
+// 27-33, in float, as launch_vacuum_advance_e forms them).
+static ECoef make_ecoef(const vpic_grid_t *g, const vpic_field_params_t *m) {
+  ECoef c;
+  const float damp = m->damp;
+  c.decayx = m->decayx;
+  c.drivex = m->drivex;
+  c.decayy = m->decayy;
+  c.drivey = m->drivey;
+  c.decayz = m->decayz;
+  c.drivez = m->drivez;
+  c.damp = damp;
+  c.px_muz = ((g->nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuz;
+  c.px_muy = ((g->nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuy;
+  c.py_mux = ((g->ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmux;
+  c.py_muz = ((g->ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmuz;
+  c.pz_muy = ((g->nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmuy;
+  c.pz_mux = ((g->nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmux;
+  c.cj = g->dt / g->eps0;
+  return c;
+}
+
+void ens_make_field_box(EnsFieldBox *b, const vpic_grid_t *g,
+                        const vpic_field_params_t *m, int64_t voff) {
+  b->nx = g->nx;
+  b->ny = g->ny;
+  b->nz = g->nz;
+  b->voff = voff;
+  const float frac = 0.5f;
+  b->bp[0] = (g->nx > 1) ? frac * g->cvac * g->dt * g->rdx : 0;
+  b->bp[1] = (g->ny > 1) ? frac * g->cvac * g->dt * g->rdy : 0;
+  b->bp[2] = (g->nz > 1) ? frac * g->cvac * g->dt * g->rdz : 0;
+  b->uc[0] = 0.25 * g->rdy * g->rdz / g->dt;
+  b->uc[1] = 0.25 * g->rdz * g->rdx / g->dt;
+  b->uc[2] = 0.25 * g->rdx * g->rdy / g->dt;
+  b->ec = make_ecoef(g, m);
+}
+
+int launch_ens_field(int phase, vpic_field_t *F, vpic_interpolator_t *FI,
+                     const vpic_accumulator_t *A, const EnsFieldBox *boxes,
+                     const EnsItem *items, int nitems, int *queue,
+                     int nblocks, int bs, hipStream_t s) {
+  if (nitems <= 0) return VPIC_OK;
+  const dim3 grid(nblocks < nitems ? nblocks : nitems), block(bs);
+  switch (phase) {
+    case ENS_UNLOAD:
+      hipLaunchKernelGGL(k_ens_field<ENS_UNLOAD>, grid, block, 0, s, F, FI, A,
+                         boxes, items, nitems, queue);
+      break;
+    case ENS_ADVB:
+      hipLaunchKernelGGL(k_ens_field<ENS_ADVB>, grid, block, 0, s, F, FI, A,
+                         boxes, items, nitems, queue);
+      break;
+    case ENS_ADVE:
+      hipLaunchKernelGGL(k_ens_field<ENS_ADVE>, grid, block, 0, s, F, FI, A,
+                         boxes, items, nitems, queue);
+      break;
+    case ENS_LOAD:
+      hipLaunchKernelGGL(k_ens_field<ENS_LOAD>, grid, block, 0, s, F, FI, A,
+                         boxes, items, nitems, queue);
+      break;
+    case ENS_GHOST:
+      hipLaunchKernelGGL(k_ens_field<ENS_GHOST>, grid, block, 0, s, F, FI, A,
+                         boxes, items, nitems, queue);
+      break;
+    default:
+      hipLaunchKernelGGL(k_ens_field<ENS_FOLD>, grid, block, 0, s, F, FI, A,
+                         boxes, items, nitems, queue);
+      break;
+  }
+  VPIC_LAUNCHED("k_ens_field");
+  return VPIC_OK;
+}
+
+int launch_ens_reset(int *queues, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_ens_reset, dim3(1), dim3(64), 0, s, queues, n);
+  VPIC_LAUNCHED("k_ens_reset");
+  return VPIC_OK;
+}
 
 int launch_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
                              const vpic_grid_t *g, hipStream_t s) {
@@ -767,6 +948,17 @@ int launch_load_interpolator(vpic_interpolator_t *fi, const vpic_field_t *f,
 int launch_clear_accumulator(vpic_accumulator_t *a, const vpic_grid_t *g,
                              hipStream_t s) {
   int64_t n4 = (int64_t)g->nv * 3;
+  int blk = blocks_for(n4, 256);
+  if (blk > 65536) blk = 65536;
+  hipLaunchKernelGGL(k_fill_zero4, dim3(blk), dim3(256), 0, s, (float4 *)a,
+                     n4);
+  VPIC_LAUNCHED("k_fill_zero4");
+  return VPIC_OK;
+}
+
+int launch_clear_accumulator_n(vpic_accumulator_t *a, int64_t nv,
+                               hipStream_t s) {
+  const int64_t n4 = nv * 3;
   int blk = blocks_for(n4, 256);
   if (blk > 65536) blk = 65536;
   hipLaunchKernelGGL(k_fill_zero4, dim3(blk), dim3(256), 0, s, (float4 *)a,
@@ -880,22 +1072,7 @@ int launch_vacuum_advance_e(vpic_field_t *f, const vpic_grid_t *g,
     VPIC_LAUNCHED("k_ghost_tang_b");
   }
   // vacuum_advance_e_pipeline.h:23-33, in float.
-  ECoef c;
-  c.decayx = m->decayx;
-  c.drivex = m->drivex;
-  c.decayy = m->decayy;
-  c.drivey = m->drivey;
-  c.decayz = m->decayz;
-  c.drivez = m->drivez;
-  c.damp = m->damp;
-  const float damp = m->damp;
-  c.px_muz = ((g->nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuz;
-  c.px_muy = ((g->nx > 1) ? (1 + damp) * g->cvac * g->dt * g->rdx : 0) * m->rmuy;
-  c.py_mux = ((g->ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmux;
-  c.py_muz = ((g->ny > 1) ? (1 + damp) * g->cvac * g->dt * g->rdy : 0) * m->rmuz;
-  c.pz_muy = ((g->nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmuy;
-  c.pz_mux = ((g->nz > 1) ? (1 + damp) * g->cvac * g->dt * g->rdz : 0) * m->rmux;
-  c.cj = g->dt / g->eps0;
+  const ECoef c = make_ecoef(g, m);
   Box b = {1, 1, 1, g->nx + 1, g->ny + 1, g->nz + 1};
   int64_t n = (int64_t)(g->nx + 1) * (g->ny + 1) * (g->nz + 1);
   hipLaunchKernelGGL(k_advance_e, dim3(blocks_for(n, 256)), dim3(256), 0, s, f,

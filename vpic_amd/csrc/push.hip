@@ -790,6 +790,57 @@ __device__ __forceinline__ void wave_deposit(int key, float j[12],
 }
 
 // --------------------------------------------------------------------------
+// Batched boxes: a persistent launch over a work queue of (box, particle
+// range) items.  Each box's particles are pushed by the flat kernel's
+// arithmetic with the box's coupling constants, deposits run-summed into
+// the box's accumulators; a crosser is moved right away by its lane (the
+// boxes are periodic: no movers come back).
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ens_push(
+    vpic_particle_t *__restrict__ P, const vpic_interpolator_t *__restrict__ FI,
+    float *__restrict__ ACC, const EnsPushBox *__restrict__ boxes,
+    const EnsItem *__restrict__ items, int nitems, int *queue,
+    int *__restrict__ nbad) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const EnsItem I = items[it];
+    const EnsPushBox b = boxes[I.box];
+    vpic_particle_t *p = P + b.poff;
+    const vpic_interpolator_t *f0 = FI + b.voff;
+    const GlobalSink sink{ACC + 12 * b.voff};
+    PushCoef C;
+    C.qdt_2mc = b.qdt_2mc;
+    C.cdt_dx = b.cdt_dx;
+    C.cdt_dy = b.cdt_dy;
+    C.cdt_dz = b.cdt_dz;
+    C.qsp = b.qsp;
+    const int end = I.first + I.count;
+    for (int base = I.first; base < end; base += blockDim.x) {
+      const int n = base + threadIdx.x;
+      const bool valid = n < end;
+      float4 A, B;
+      load_particle(p, n, valid, A, B);
+      PushResult R;
+      push_compute(A, B, valid, p, n, f0, C, b.G.nv, nbad, R);
+      wave_deposit(R.active ? R.voxel : -1, R.j, sink);
+      if (R.cross) {
+        vpic_particle_mover_t mv;
+        mv.dispx = R.dispx;
+        mv.dispy = R.dispy;
+        mv.dispz = R.dispz;
+        mv.i = n;
+        if (dev_move_p(p, mv, sink, b.G, C.qsp)) atomicAdd(nbad + 1, 1);
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // Flat kernel: particles [n0, np), any order, grid-stride over whole
 // wavefronts.  n0 = *start clamped to [0,np] (the end of the sort-time
 // partition when the tiled kernel ran first), or 0 when start is NULL.
@@ -2095,6 +2146,20 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
   hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
                      xcap, (int2 *)S.sortbuf.ptr, pm, max_nm, cnt, dev_nm);
   VPIC_LAUNCHED("k_order_movers");
+  return VPIC_OK;
+}
+
+int launch_ens_push(vpic_particle_t *P, const vpic_interpolator_t *FI,
+                    vpic_accumulator_t *A, const EnsPushBox *boxes,
+                    const EnsItem *items, int nitems, int *queue, int *nbad,
+                    int nblocks, int bs, hipStream_t s) {
+  if (nitems <= 0) return VPIC_OK;
+  if (bs < 64 || bs > 256 || (bs & 63))
+    return vpic_set_error(VPIC_EINVAL, "ensemble push: block size %d", bs);
+  hipLaunchKernelGGL(k_ens_push, dim3(nblocks < nitems ? nblocks : nitems),
+                     dim3(bs), 0, s, P, FI, (float *)A, boxes, items, nitems,
+                     queue, nbad);
+  VPIC_LAUNCHED("k_ens_push");
   return VPIC_OK;
 }
 
