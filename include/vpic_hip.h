@@ -205,6 +205,29 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
                              const vpic_grid_t *g, const int32_t *partition,
                              void *stream);
 
+/* vpic_hip_advance_p_async without a partition (the flat push), for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  Its crosser list, returned-mover list,
+ * counters and ordering buffer live in one fixed array of the library
+ * (2.5 MiB per GPU, held until the process ends; its device address is
+ * looked up by vpic_hip_init, or by the first call on a device), so it
+ * calls no hipMalloc, hipFree or allocating library routine.  Capacity:
+ * 65536 crossers listed per call (the rest are moved inside the push, as
+ * vpic_hip_advance_p_async does past its own list) and 65536 returned
+ * movers; max_nm above 65536 is rejected.  For np <= 327680 its lists are
+ * exactly as large as vpic_hip_advance_p_async's, and its results equal
+ * that call's (bit for bit except the summation order of the currents).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (on different streams or threads) race.  Writes {nm, n_ignored,
+ * n_bad} to dev_nm; does not synchronise. */
+int vpic_hip_advance_p_noalloc(vpic_particle_t *p, int32_t np, float q,
+                               float m, vpic_particle_mover_t *pm,
+                               int32_t max_nm, int32_t *dev_nm,
+                               vpic_accumulator_t *a,
+                               const vpic_interpolator_t *f,
+                               const vpic_grid_t *g, void *stream);
+
 /* Tile shape (cells) of the partition-driven push; its LDS window holds
  * (tx+2)(ty+2)(tz+2) accumulators.  Default 8x8x8.  No reference
  * counterpart (the reference's pipelines split particles, not cells). */

@@ -99,7 +99,8 @@ extern "C" int vpic_hip_init(int device) {
         device, n);
   VPIC_HIP(hipSetDevice(device));
   VPIC_HIP(hipFree(nullptr));
-  return VPIC_OK;
+  // the device address of the no-allocation push's fixed scratch
+  return push_noalloc_setup();
 }
 
 extern "C" const char *vpic_hip_last_error(void) { return g_err; }
@@ -205,6 +206,18 @@ extern "C" int vpic_hip_advance_p_async(
   CHECK(dev_nm, "dev_nm is NULL");
   return launch_advance_p(p, np, q, m, pm, max_nm, dev_nm, a, f, g, partition,
                           S(stream));
+}
+
+extern "C" int vpic_hip_advance_p_noalloc(
+    vpic_particle_t *p, int32_t np, float q, float m,
+    vpic_particle_mover_t *pm, int32_t max_nm, int32_t *dev_nm,
+    vpic_accumulator_t *a, const vpic_interpolator_t *f,
+    const vpic_grid_t *g, void *stream) {
+  int rc = check_push_args(p, np, pm, max_nm, a, f, g);
+  if (rc) return rc;
+  CHECK(dev_nm, "dev_nm is NULL");
+  return launch_advance_p_noalloc(p, np, q, m, pm, max_nm, dev_nm, a, f, g,
+                                  S(stream));
 }
 
 extern "C" int vpic_hip_push_group(int32_t gs) {

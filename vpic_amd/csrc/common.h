@@ -192,6 +192,15 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
                      const vpic_grid_t *g, const int *partition,
                      hipStream_t s);
+// The flat push on the library's fixed per-device scratch (no allocation):
+// vpic_hip_advance_p_noalloc.  push_noalloc_setup looks up that scratch's
+// device address on the current device (once per device).
+int launch_advance_p_noalloc(vpic_particle_t *p, int np, float q, float m,
+                             vpic_particle_mover_t *pm, int max_nm,
+                             int *dev_nm, vpic_accumulator_t *a,
+                             const vpic_interpolator_t *f,
+                             const vpic_grid_t *g, hipStream_t s);
+int push_noalloc_setup();
 int push_set_tile(int tx, int ty, int tz);
 int push_set_direct_runs(int runs);
 int push_set_group(int gs);

@@ -34,6 +34,8 @@
 // Both end with k_order_movers: movers the grid cannot handle, sorted by
 // particle index, truncated at max_nm (excess rolled back to the voxel
 // index, advance_p_pipeline.cc:228-236).
+#include <mutex>
+
 #include "common.h"
 
 namespace {
@@ -1957,18 +1959,137 @@ int push_set_direct_runs(int runs) {
   return e == hipSuccess ? VPIC_OK : vpic_check_hip(e, "hipMemcpyToSymbol");
 }
 
+namespace {
+
+// advance_p_pipeline.cc:279-283, in float.
+PushCoef push_coef(float q, float m, const vpic_grid_t *g) {
+  PushCoef C;
+  C.qdt_2mc = (q * g->dt) / (2 * m * g->cvac);
+  C.cdt_dx = g->cvac * g->dt * g->rdx;
+  C.cdt_dy = g->cvac * g->dt * g->rdy;
+  C.cdt_dz = g->cvac * g->dt * g->rdz;
+  C.qsp = q;
+  return C;
+}
+
+// Crossers listed for move_p (the rare excess is moved inline by the
+// push kernel) and movers returned to the caller: ~20% of the particles
+// per step, at least 65536, at most np.
+int crosser_capacity(int np) {
+  int64_t xcap64 = np / 5;
+  if (xcap64 < 65536) xcap64 = 65536;
+  if (xcap64 > np) xcap64 = np < 1 ? 1 : np;
+  return (int)xcap64;
+}
+
+// The flat push of particles [0, np) (k_advance_p), the listed crossers'
+// move_p and the ordering of the returned movers, on scratch the caller
+// provides: cnt (4 ints, cleared here by a kernel), xlist and rlist (xcap
+// movers each), sortbuf (pow2_at_least(xcap) int2).
+int launch_flat_push(vpic_particle_t *p, int np, float q, const PushCoef &C,
+                     const GridDev &G, vpic_particle_mover_t *pm, int max_nm,
+                     int *dev_nm, vpic_accumulator_t *a,
+                     const vpic_interpolator_t *f, int *cnt,
+                     vpic_particle_mover_t *xlist,
+                     vpic_particle_mover_t *rlist, int2 *sortbuf, int xcap,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
+  VPIC_LAUNCHED("k_clear_counters");
+  if (np > 0) {
+    int64_t want = ((int64_t)np + 255) / 256;
+    const int blocks = (int)(want < 16384 ? want : 16384);
+    hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p,
+                       (const int *)nullptr, np, f, (float *)a, xlist, xcap,
+                       rlist, cnt, C, G);
+    VPIC_LAUNCHED("k_advance_p");
+    int mblocks = (xcap + 255) / 256;
+    if (mblocks > 8192) mblocks = 8192;
+    hipLaunchKernelGGL(k_move_p_list, dim3(mblocks), dim3(256), 0, s, p,
+                       xlist, xcap, rlist, (float *)a, cnt, q, G);
+    VPIC_LAUNCHED("k_move_p_list");
+  }
+  hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
+                     xcap, sortbuf, pm, max_nm, cnt, dev_nm);
+  VPIC_LAUNCHED("k_order_movers");
+  return VPIC_OK;
+}
+
+}  // namespace
+
+// Scratch of the flat push for callers that cannot allocate
+// (vpic_hip_advance_p_noalloc): one array per GPU, held until the process
+// ends, laid out as 4 counters (in a 256-B slot), the crosser list and the
+// returned-mover list (NOALLOC_XCAP movers each) and the ordering buffer
+// (NOALLOC_XCAP int2): 2.5 MiB.
+constexpr int NOALLOC_XCAP = 65536;
+constexpr size_t NOALLOC_CNT_BYTES = 256;
+constexpr size_t NOALLOC_LIST_BYTES =
+    sizeof(vpic_particle_mover_t) * (size_t)NOALLOC_XCAP;
+constexpr size_t NOALLOC_BYTES = NOALLOC_CNT_BYTES + 2 * NOALLOC_LIST_BYTES +
+                                 sizeof(int2) * (size_t)NOALLOC_XCAP;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+// Device address of `scratch` per device (the host-side name is only a
+// stand-in and must never reach a kernel).
+constexpr int VPIC_MAX_DEVICES = 16;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[VPIC_MAX_DEVICES] = {nullptr};
+}  // namespace
+
+int push_noalloc_setup() {
+  int dev = 0;
+  VPIC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= VPIC_MAX_DEVICES)
+    return vpic_set_error(VPIC_EINVAL, "device %d beyond %d", dev,
+                          VPIC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[dev]) return VPIC_OK;
+  void *ptr = nullptr;
+  VPIC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr)
+    return vpic_set_error(VPIC_EHIP, "hipGetSymbolAddress(scratch) gave NULL");
+  g_noalloc_ptr[dev] = ptr;
+  return VPIC_OK;
+}
+
+int launch_advance_p_noalloc(vpic_particle_t *p, int np, float q, float m,
+                             vpic_particle_mover_t *pm, int max_nm,
+                             int *dev_nm, vpic_accumulator_t *a,
+                             const vpic_interpolator_t *f,
+                             const vpic_grid_t *g, hipStream_t s) {
+  if (max_nm > NOALLOC_XCAP)
+    return vpic_set_error(VPIC_EINVAL,
+                          "advance_p_noalloc: max_nm %d exceeds the %d movers "
+                          "its fixed scratch holds", max_nm, NOALLOC_XCAP);
+  int rc = push_noalloc_setup();
+  if (rc) return rc;
+  int dev = 0;
+  VPIC_HIP(hipGetDevice(&dev));
+  unsigned char *base = (unsigned char *)g_noalloc_ptr[dev];
+  int *cnt = (int *)base;
+  vpic_particle_mover_t *xlist =
+      (vpic_particle_mover_t *)(base + NOALLOC_CNT_BYTES);
+  vpic_particle_mover_t *rlist =
+      (vpic_particle_mover_t *)(base + NOALLOC_CNT_BYTES + NOALLOC_LIST_BYTES);
+  int2 *sortbuf =
+      (int2 *)(base + NOALLOC_CNT_BYTES + 2 * NOALLOC_LIST_BYTES);
+  int xcap = crosser_capacity(np);
+  if (xcap > NOALLOC_XCAP) xcap = NOALLOC_XCAP;
+  // a flat push moves particles without recording their tiles
+  push_invalidate_bins(p);
+  return launch_flat_push(p, np, q, push_coef(q, m, g), make_grid_dev(g), pm,
+                          max_nm, dev_nm, a, f, cnt, xlist, rlist, sortbuf,
+                          xcap, s);
+}
+
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
                      const vpic_grid_t *g, const int *partition,
                      hipStream_t s) {
   PushScratch &S = push_scratch();
-  // Crosser list sized for ~20% of the particles per step; the rare
-  // excess is moved inline by the push kernel.
-  int64_t xcap64 = np / 5;
-  if (xcap64 < 65536) xcap64 = 65536;
-  if (xcap64 > np) xcap64 = np < 1 ? 1 : np;
-  const int xcap = (int)xcap64;
+  const int xcap = crosser_capacity(np);
   int rc;
   if ((rc = scratch_reserve(S.counters, 64)) ||
       (rc = scratch_reserve(S.xlist, sizeof(vpic_particle_mover_t) * xcap)) ||
@@ -1979,13 +2100,7 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
   vpic_particle_mover_t *xlist = (vpic_particle_mover_t *)S.xlist.ptr;
   vpic_particle_mover_t *rlist = (vpic_particle_mover_t *)S.rlist.ptr;
 
-  // advance_p_pipeline.cc:279-283, in float.
-  PushCoef C;
-  C.qdt_2mc = (q * g->dt) / (2 * m * g->cvac);
-  C.cdt_dx = g->cvac * g->dt * g->rdx;
-  C.cdt_dy = g->cvac * g->dt * g->rdy;
-  C.cdt_dz = g->cvac * g->dt * g->rdz;
-  C.qsp = q;
+  const PushCoef C = push_coef(q, m, g);
   const GridDev G = make_grid_dev(g);
 
   hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
