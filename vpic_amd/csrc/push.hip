@@ -130,10 +130,12 @@ __device__ __forceinline__ int tile_code_1(int d, int n) {
 }
 
 struct WindowSink {
-  lds_u64 *lds;  // 12 * ncell
+  lds_u64 *lds;  // ncell cells of 12 components (cell-major)
   int ncell;
   double scale;  // 2^k, set once per workgroup before the first deposit
   float fscale;  // the same power of two
+  float ifscale;  // 2^-k
+  bool pre;       // register sums arrive scaled by 2^k (flush_scaled)
   float *acc;    // global accumulator
   int wx0, wy0, wz0, wnx, wny, wnz;
   int sx1, sy1;     // nx+2, ny+2
@@ -165,7 +167,7 @@ struct WindowSink {
     const int s = slot(voxel);
     float *g = acc + 12 * (int64_t)voxel;
     if (s >= 0) {
-      lds_u64 *a = lds + s;
+      lds_u64 *a = lds + 12 * s;
       // Scaling by a power of two is exact in float, so when every scaled
       // component is below 2^31 the rounding to an integer is done in FP32
       // (the same integer the FP64 path gives).
@@ -178,7 +180,7 @@ struct WindowSink {
       if (big * fs < 2147483520.0f) {
 #pragma unroll
         for (int k = 0; k < 12; k++)
-          __hip_atomic_fetch_add(a + k * ncell,
+          __hip_atomic_fetch_add(a + k,
                                  (unsigned long long)(long long)__float2int_rn(j[k] * fs),
                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         return;
@@ -187,7 +189,7 @@ struct WindowSink {
       for (int k = 0; k < 12; k++) {
         const double y = (double)j[k] * scale;
         if (fabs(y) < 2251799813685248.0)  // 2^51
-          __hip_atomic_fetch_add(a + k * ncell,
+          __hip_atomic_fetch_add(a + k,
                                  (unsigned long long)to_fixed(y),
                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
@@ -247,6 +249,36 @@ struct WindowSink {
         for (int c = 0; c < 12; c++) atomic_add_f32(g + c, j[c]);
       }
     }
+  }
+  // A lane's register sum js (12 components) for voxel `voxel`, whose
+  // window cell `cell` (-1 outside the window) the caller computed when
+  // the voxel was first seen.  With pre set, js is already scaled by 2^k
+  // (the deposits were formed with the charge times 2^k, an exact power-of-
+  // two scaling), so a sum whose components are all below 2^31 in magnitude
+  // converts with one truncating conversion per component (the truncation
+  // error, below 2^-k, is far below FP32 rounding of the sum).  Anything
+  // else is unscaled and takes deposit().
+  __device__ __forceinline__ void flush_scaled(int voxel, int cell,
+                                               const float js[12]) const {
+    if (pre && cell >= 0) {
+      // the sum of the magnitudes bounds every component and is NaN when
+      // one is, which then takes deposit()
+      float big = fabsf(js[0]);
+#pragma unroll
+      for (int k = 1; k < 12; k++) big += fabsf(js[k]);
+      if (big < 2147483520.0f) {
+        lds_u64 *a = lds + 12 * cell;
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+          __hip_atomic_fetch_add(a + k, (unsigned long long)(long long)(int)js[k],
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+      }
+    }
+    float j[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) j[k] = pre ? js[k] * ifscale : js[k];
+    deposit(voxel, j);
   }
 };
 
@@ -1210,10 +1242,12 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
 #pragma unroll
   for (int k = 0; k < 12; k++) jacc[k] = 0.0f;
   int jkey = -1;   // its voxel (-1: nothing held)
+  int jcell = -1;  // jkey's window cell (-1: outside the window)
   Interp F;        // coefficients of voxel fkey
   F.e0 = F.e1 = F.e2 = F.b0 = make_float4(0.f, 0.f, 0.f, 0.f);
   F.b1 = make_float2(0.f, 0.f);
   int fkey = -1;
+  int fcell = -1;  // fkey's window cell (-1: outside the window)
   int nbad = 0;  // slots with a voxel index outside the grid
 
   RowWalk rw;
@@ -1237,6 +1271,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   // of outstanding loads makes the wait for it drain both.)
   float4 A0, B0, A1, B1;
   load_particle(p, n, cur_ok, A0, B0);
+  PushCoef Cs = C;  // C with the deposit charge scaled by the window's 2^k
   {
     // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
     // the first particle of every lane.  One particle's current is below
@@ -1255,6 +1290,15 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     if (m > 0.0f) frexpf(8.0f * m, &e);  // 8|q w|max < 2^e
     W.scale = ldexp(1.0, 31 - e);
     W.fscale = (float)W.scale;
+    W.ifscale = (float)(1.0 / W.scale);
+    // The deposits of particles that stay in their cell are formed with the
+    // charge times 2^k (exact while the product is a normal float), so the
+    // register sums need no scaling when they are flushed.
+    Cs.qsp = C.qsp * W.fscale;
+    const float aq = fabsf(Cs.qsp);
+    W.pre = W.fscale > 0.0f && aq <= 3.0e38f && W.ifscale > 0.0f &&
+            (C.qsp == 0.0f || aq >= 1.0e-30f);
+    if (!W.pre) Cs.qsp = C.qsp;
   }
   int it = 0;
   auto slot = [&](const float4 A, const float4 B, float4 &An, float4 &Bn_) {
@@ -1266,6 +1310,12 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     if (active && ii != fkey) {
       int x, y, z;
       VD.coords(G, ii, x, y, z);
+      const int wx = x - W.wx0, wy = y - W.wy0, wz = z - W.wz0;
+      fcell = ((unsigned)wx < (unsigned)W.wnx &&
+               (unsigned)wy < (unsigned)W.wny &&
+               (unsigned)wz < (unsigned)W.wnz)
+                  ? wx + W.wnx * (wy + W.wny * wz)
+                  : -1;
       if (x >= x0 && x <= x1 && y >= y0 && y <= y1 && z >= z0 && z <= z1) {
         const int c = (x - x0) + cx * (y - y0) + cxy * (z - z0);
         F.e0 = ic_e0[c];
@@ -1299,7 +1349,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     // number of memory operations per iteration does not depend on the
     // particle and the loop never waits for its own stores
     PushResult R;
-    push_compute_u(A, B, active, F, C,
+    push_compute_u(A, B, active, F, Cs,
                    valid ? reinterpret_cast<float4 *>(p + n)
                          : reinterpret_cast<float4 *>(trash + t),
                    R);
@@ -1323,10 +1373,11 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     }
     if (R.active && !R.cross) {
       if (ii != jkey) {
-        if (jkey >= 0) W.deposit(jkey, jacc);
+        if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
 #pragma unroll
         for (int c = 0; c < 12; c++) jacc[c] = R.j[c];
         jkey = ii;
+        jcell = fcell;
       } else {
 #pragma unroll
         for (int c = 0; c < 12; c++) jacc[c] += R.j[c];
@@ -1362,7 +1413,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   }
   if (it < steps) slot(A0, B0, A1, B1);
   if (nbad) atomicAdd(cnt + 3, nbad);
-  if (jkey >= 0) W.deposit(jkey, jacc);
+  if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
   if (qn)
     drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt, BS);
   __syncthreads();
@@ -1372,8 +1423,12 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   // The window, as floats, to this tile's slot (component-major, ncell).
   float *wout = wbuf + (int64_t)blockIdx.x * wstride;
   const double inv = 1.0 / W.scale;
-  for (int kk = t; kk < 12 * ncell; kk += TILE_BS)
-    wout[kk] = (float)((double)(long long)lds_win[kk] * inv);
+  for (int c = t; c < ncell; c += TILE_BS) {
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      wout[k * ncell + c] =
+          (float)((double)(long long)lds_win[12 * c + k] * inv);
+  }
 }
 
 // Add the tiles' windows into the accumulator: voxel v receives the
