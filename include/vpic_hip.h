@@ -690,9 +690,10 @@ int vpic_hip_ensemble_box(vpic_ensemble_t *e, int32_t b, vpic_particle_t **p,
                           vpic_interpolator_t **fi, vpic_accumulator_t **a);
 /* load_interpolator_array of every box (after loading particles/fields). */
 int vpic_hip_ensemble_prepare(vpic_ensemble_t *e);
-/* nstep steps of every box; use_graph replays the captured step (captured
- * at the first call and after every shape change), else the phases are
- * launched one by one.  Synchronous; fails if a particle left its box. */
+/* nstep steps of every box (with the sorts vpic_hip_ensemble_sort_interval
+ * asks for); use_graph replays the captured step (captured at the first
+ * call and after every shape change), else the phases are launched one by
+ * one.  Synchronous; fails if a particle left its box. */
 int vpic_hip_ensemble_step(vpic_ensemble_t *e, int32_t nstep,
                            int32_t use_graph);
 /* Launch shapes: push block size (64..256) and particles per work item,
@@ -702,6 +703,13 @@ int vpic_hip_ensemble_shape(vpic_ensemble_t *e, int32_t push_bs,
                             int32_t push_chunk, int32_t field_bs,
                             int32_t field_chunk, int32_t blocks_per_cu);
 int vpic_hip_ensemble_get_shape(const vpic_ensemble_t *e, int32_t out[5]);
+/* sort_p of every box (src/species_advance/standard/sort_p.cc:154-165, the
+ * voxel order; particles of one voxel keep no particular order) before
+ * every step whose count of steps done is a multiple of `interval`, as
+ * advance.cc:31-37 sorts a species every sort_interval steps; 0 never
+ * sorts.  Default 20.  The sort is one counting sort over the packed
+ * arrays, captured in a graph of its own. */
+int vpic_hip_ensemble_sort_interval(vpic_ensemble_t *e, int32_t interval);
 /* The tuner: times each candidate shape over trial_steps graph replays from
  * the same saved state (restored afterwards), keeps the fastest and
  * returns its ms per step. */

@@ -105,6 +105,7 @@ _SIGS = {
     "vpic_hip_ensemble_shape": [_vp, _i32, _i32, _i32, _i32, _i32],
     "vpic_hip_ensemble_get_shape": [_vp, _vp],
     "vpic_hip_ensemble_tune": [_vp, _i32, _pf64],
+    "vpic_hip_ensemble_sort_interval": [_vp, _i32],
     "vpic_hip_push_timing_enable": [ctypes.c_int],
     "vpic_hip_push_timing": [_pi64, _pf64, _pi64],
 }

@@ -172,6 +172,15 @@ int launch_ens_field(int phase, vpic_field_t *F, vpic_interpolator_t *FI,
                      const EnsItem *items, int nitems, int *queue,
                      int nblocks, int bs, hipStream_t s);
 int launch_ens_reset(int *queues, int n, hipStream_t s);
+// Sort of every box's particles by voxel (sort_p per box on the packed
+// arrays), enqueued without a host synchronisation so that it can be
+// captured: keys (np_total ints), count and start (nv_total + 1 ints each),
+// sums (device_exclusive_scan's), aux (np_total particles).  Equal voxels
+// keep no particular order.
+int launch_ens_sort(vpic_particle_t *P, vpic_particle_t *aux,
+                    const EnsPushBox *boxes, const EnsItem *items, int nitems,
+                    int64_t np_total, int64_t nv_total, int *keys, int *count,
+                    int *start, int *sums, int *nbad, hipStream_t s);
 int launch_clear_accumulator_n(vpic_accumulator_t *a, int64_t nv,
                                hipStream_t s);
 int launch_ens_push(vpic_particle_t *P, const vpic_interpolator_t *FI,

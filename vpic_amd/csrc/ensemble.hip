@@ -69,6 +69,15 @@ struct vpic_ensemble {
   Shape shape = {256, 4096, 256, 2048, 4};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // sort_p of every box every sort_interval steps (0: never), before the
+  // step whose count of steps done is a multiple of it, as the reference's
+  // advance sorts a species (advance.cc:31-37); its own captured graph
+  int sort_interval = 20;
+  vpic_particle_t *aux = nullptr;
+  int *skeys = nullptr, *scount = nullptr, *sstart = nullptr,
+      *ssums = nullptr;
+  hipGraph_t sgraph = nullptr;
+  hipGraphExec_t sexec = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int64_t steps = 0;
@@ -76,8 +85,11 @@ struct vpic_ensemble {
   ~vpic_ensemble() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (sexec) (void)hipGraphExecDestroy(sexec);
+    if (sgraph) (void)hipGraphDestroy(sgraph);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
-    void *bufs[] = {P, F, FI, A, pbox, fbox, items, queues};
+    void *bufs[] = {P,   F,     FI,     A,      pbox,  fbox,  items,
+                    queues, aux, skeys, scount, sstart, ssums};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }
@@ -98,8 +110,42 @@ int ens_alloc(void **p, size_t bytes) {
 void drop_graph(vpic_ensemble *e) {
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->sexec) (void)hipGraphExecDestroy(e->sexec);
+  if (e->sgraph) (void)hipGraphDestroy(e->sgraph);
   e->exec = nullptr;
   e->graph = nullptr;
+  e->sexec = nullptr;
+  e->sgraph = nullptr;
+}
+
+// The sort's buffers: the aux particle array, keys, counts and offsets.
+int ensure_sort_buffers(vpic_ensemble *e) {
+  if (e->aux) return VPIC_OK;
+  const int64_t nslots = e->nv_total + 1;
+  const int64_t nsums = (nslots + 2047) / 2048 + 1;
+  int rc;
+  if ((rc = ens_alloc((void **)&e->aux,
+                      sizeof(vpic_particle_t) * e->np_total)) ||
+      (rc = ens_alloc((void **)&e->skeys, sizeof(int) * e->np_total)) ||
+      (rc = ens_alloc((void **)&e->scount, sizeof(int) * nslots)) ||
+      (rc = ens_alloc((void **)&e->sstart, sizeof(int) * nslots)) ||
+      (rc = ens_alloc((void **)&e->ssums, sizeof(int) * nsums))) {
+    void *bufs[] = {e->aux, e->skeys, e->scount, e->sstart, e->ssums};
+    for (void *b : bufs)
+      if (b) (void)hipFree(b);
+    e->aux = nullptr;
+    e->skeys = e->scount = e->sstart = e->ssums = nullptr;
+    return rc;
+  }
+  return VPIC_OK;
+}
+
+int enqueue_sort(vpic_ensemble *e, hipStream_t s) {
+  return launch_ens_sort(e->P, e->aux, e->pbox,
+                         e->items + e->item_off[Q_PUSH],
+                         e->item_off[Q_PUSH + 1] - e->item_off[Q_PUSH],
+                         e->np_total, e->nv_total, e->skeys, e->scount,
+                         e->sstart, e->ssums, e->queues + Q_COUNT, s);
 }
 
 // Work items of every phase for the current shape.
@@ -184,11 +230,12 @@ int enqueue_step(vpic_ensemble *e, hipStream_t s) {
   return VPIC_OK;
 }
 
-int capture(vpic_ensemble *e) {
-  drop_graph(e);
+// Capture what `enqueue` puts on the ensemble's stream into a graph.
+int capture_one(vpic_ensemble *e, int (*enqueue)(vpic_ensemble *, hipStream_t),
+                hipGraph_t *graph, hipGraphExec_t *exec) {
   hipStream_t s = e->stream;
   VPIC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(e, s);
+  int rc = enqueue(e, s);
   hipGraph_t g = nullptr;
   hipError_t ce = hipStreamEndCapture(s, &g);
   if (rc) {
@@ -196,21 +243,41 @@ int capture(vpic_ensemble *e) {
     return rc;
   }
   if (ce != hipSuccess) return vpic_check_hip(ce, "hipStreamEndCapture");
-  e->graph = g;
-  VPIC_HIP(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+  *graph = g;
+  VPIC_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return VPIC_OK;
+}
+
+int capture(vpic_ensemble *e) {
+  drop_graph(e);
+  int rc;
+  if ((rc = capture_one(e, enqueue_step, &e->graph, &e->exec))) return rc;
+  if (e->sort_interval > 0) {
+    if ((rc = ensure_sort_buffers(e)) ||
+        (rc = capture_one(e, enqueue_sort, &e->sgraph, &e->sexec)))
+      return rc;
+  }
   return VPIC_OK;
 }
 
 int run_steps(vpic_ensemble *e, int n, int use_graph) {
   int rc;
   if (use_graph && !e->exec && (rc = capture(e))) return rc;
+  if (e->sort_interval > 0 && (rc = ensure_sort_buffers(e))) return rc;
   for (int k = 0; k < n; k++) {
+    if (e->sort_interval > 0 && e->steps % e->sort_interval == 0) {
+      if (use_graph && !e->sexec && (rc = capture(e))) return rc;
+      if (use_graph)
+        VPIC_HIP(hipGraphLaunch(e->sexec, e->stream));
+      else if ((rc = enqueue_sort(e, e->stream)))
+        return rc;
+    }
     if (use_graph)
       VPIC_HIP(hipGraphLaunch(e->exec, e->stream));
     else if ((rc = enqueue_step(e, e->stream)))
       return rc;
+    e->steps++;
   }
-  e->steps += n;
   return VPIC_OK;
 }
 
@@ -273,6 +340,13 @@ extern "C" int vpic_hip_ensemble_create(vpic_ensemble_t **out, int32_t nbox,
   }
   e->nv_total = voff;
   e->np_total = poff;
+  if (voff >= INT32_MAX || poff >= INT32_MAX) {
+    delete e;
+    return vpic_set_error(VPIC_EINVAL,
+                          "ensemble_create: %lld voxels and %lld particles "
+                          "in all (each must stay below 2^31)",
+                          (long long)voff, (long long)poff);
+  }
   int rc;
   if ((rc = ens_alloc((void **)&e->P, sizeof(vpic_particle_t) * poff)) ||
       (rc = ens_alloc((void **)&e->F, sizeof(vpic_field_t) * voff)) ||
@@ -388,6 +462,17 @@ extern "C" int vpic_hip_ensemble_shape(vpic_ensemble_t *e, int32_t push_bs,
   return build_items(e);
 }
 
+extern "C" int vpic_hip_ensemble_sort_interval(vpic_ensemble_t *e,
+                                               int32_t interval) {
+  if (!e || interval < 0)
+    return vpic_set_error(VPIC_EINVAL, "ensemble_sort_interval: bad arguments");
+  if (interval != e->sort_interval) {
+    e->sort_interval = interval;
+    drop_graph(e);
+  }
+  return VPIC_OK;
+}
+
 extern "C" int vpic_hip_ensemble_get_shape(const vpic_ensemble_t *e,
                                            int32_t out[5]) {
   if (!e || !out) return vpic_set_error(VPIC_EINVAL, "bad arguments");
@@ -431,6 +516,10 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   (void)hipEventCreate(&t1);
   const Shape orig = e->shape;
   const int64_t steps0 = e->steps;
+  // the candidates are timed on the step alone (the sort's launch shape is
+  // not tuned)
+  const int interval0 = e->sort_interval;
+  e->sort_interval = 0;
   Shape best = orig;
   float best_ms = 1e30f;
   rc = VPIC_OK;
@@ -459,6 +548,7 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   }
   (void)hipStreamSynchronize(s);
   e->steps = steps0;  // the trials' steps were undone with the state
+  e->sort_interval = interval0;
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   (void)hipFree(saveP);
@@ -470,7 +560,7 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
     return rc;
   }
   e->shape = best;
-  if ((rc = build_items(e))) return rc;
+  if ((rc = build_items(e))) return rc;  // drops the trial graphs
   if (host_best_ms) *host_best_ms = best_ms;
   return VPIC_OK;
 }

@@ -591,6 +591,125 @@ int sort_p_counting(vpic_particle_t *p, vpic_particle_t *aux, int np,
 
 }  // namespace
 
+// --------------------------------------------------------------------------
+// Batched sort of an ensemble's boxes (no host synchronisation, so a graph
+// can hold it).  The key of a particle is its box's voxel offset plus its
+// voxel, so one counting sort over the packed arrays sorts every box within
+// its own particle range (the keys of box b lie between those of boxes b-1
+// and b+1, and every box keeps its particle count).  A voxel outside the
+// box is counted in nbad and sorted as the box's voxel 0.
+// --------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ int ens_key(const vpic_particle_t &p,
+                                       const EnsPushBox &b, int *nbad) {
+  const int v = p.i;
+  if ((unsigned)v >= (unsigned)b.G.nv) {
+    atomicAdd(nbad, 1);
+    return (int)b.voff;
+  }
+  return (int)b.voff + v;
+}
+
+// Runs of equal keys over consecutive lanes: `len` for the run's first
+// lane (0 elsewhere) and each lane's rank in its run.
+__device__ __forceinline__ void key_runs(int key, bool valid, int lane,
+                                         int &len, int &rank) {
+  const int kp = __shfl_up(key, 1);
+  const bool head = valid && (lane == 0 || kp != key);
+  const uint64_t heads = __ballot(head);
+  const uint64_t act = __ballot(valid);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int h0 = 63 - __clzll(heads & upto);  // this lane's run head
+  rank = lane - h0;
+  // the next head after this lane (or the end of the valid lanes)
+  const uint64_t later = heads & ~upto;
+  const int nxt = later ? __ffsll((unsigned long long)later) - 1
+                        : 64 - __clzll(act);
+  len = head ? nxt - lane : 0;
+}
+
+__global__ __launch_bounds__(256) void k_ens_count(
+    const vpic_particle_t *__restrict__ P, const EnsPushBox *__restrict__ boxes,
+    const EnsItem *__restrict__ items, int *__restrict__ keys,
+    int *__restrict__ count, int *__restrict__ nbad) {
+  const EnsItem I = items[blockIdx.x];
+  const EnsPushBox b = boxes[I.box];
+  const int lane = lane_id();
+  const int end = I.first + I.count;
+  for (int base = I.first; base < end; base += blockDim.x) {
+    const int n = base + threadIdx.x;
+    const bool valid = n < end;
+    int key = -1;
+    if (valid) {
+      const int64_t g = b.poff + n;
+      key = ens_key(P[g], b, nbad);
+      keys[g] = key;
+    }
+    int len, rank;
+    key_runs(key, valid, lane, len, rank);
+    if (len) atomicAdd(count + key, len);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ens_scatter(
+    const vpic_particle_t *__restrict__ P, const EnsPushBox *__restrict__ boxes,
+    const EnsItem *__restrict__ items, const int *__restrict__ keys,
+    int *__restrict__ cursor, vpic_particle_t *__restrict__ aux) {
+  const EnsItem I = items[blockIdx.x];
+  const EnsPushBox b = boxes[I.box];
+  const int lane = lane_id();
+  const int end = I.first + I.count;
+  for (int base = I.first; base < end; base += blockDim.x) {
+    const int n = base + threadIdx.x;
+    const bool valid = n < end;
+    const int64_t g = b.poff + n;
+    const int key = valid ? keys[g] : -1;
+    int len, rank;
+    key_runs(key, valid, lane, len, rank);
+    int at = 0;
+    if (len) at = atomicAdd(cursor + key, len);
+    // the run head's slot, to every lane of the run
+    const int kp = __shfl_up(key, 1);
+    const bool head = valid && (lane == 0 || kp != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h0 = 63 - __clzll(heads & upto);
+    at = __shfl(at, h0 < 0 ? 0 : h0);
+    if (valid) {
+      const float4 *src = reinterpret_cast<const float4 *>(P + g);
+      float4 *dst = reinterpret_cast<float4 *>(aux + at + rank);
+      dst[0] = src[0];
+      dst[1] = src[1];
+    }
+  }
+}
+
+}  // namespace
+
+int launch_ens_sort(vpic_particle_t *P, vpic_particle_t *aux,
+                    const EnsPushBox *boxes, const EnsItem *items, int nitems,
+                    int64_t np_total, int64_t nv_total, int *keys, int *count,
+                    int *start, int *sums, int *nbad, hipStream_t s) {
+  if (nitems <= 0 || np_total <= 0) return VPIC_OK;
+  const int64_t nslots = nv_total + 1;
+  int fb = blocks_for(nslots, 256);
+  if (fb > 65536) fb = 65536;
+  hipLaunchKernelGGL(k_fill_int, dim3(fb), dim3(256), 0, s, count, nslots, 0);
+  VPIC_LAUNCHED("k_fill_int");
+  hipLaunchKernelGGL(k_ens_count, dim3(nitems), dim3(256), 0, s, P, boxes,
+                     items, keys, count, nbad);
+  VPIC_LAUNCHED("k_ens_count");
+  int rc;
+  if ((rc = device_exclusive_scan(count, start, nslots, sums, s))) return rc;
+  hipLaunchKernelGGL(k_ens_scatter, dim3(nitems), dim3(256), 0, s, P, boxes,
+                     items, (const int *)keys, start, aux);
+  VPIC_LAUNCHED("k_ens_scatter");
+  VPIC_HIP(hipMemcpyAsync(P, aux, sizeof(vpic_particle_t) * (size_t)np_total,
+                          hipMemcpyDeviceToDevice, s));
+  return VPIC_OK;
+}
+
 // sort_p: the axis passes when every axis fits their LDS counters, else the
 // counting sort.
 int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,

@@ -104,6 +104,11 @@ class Ensemble:
             _lib.call("vpic_hip_ensemble_shape", self.h, *new)
         return tuple(new)
 
+    def sort_interval(self, n):
+        """Sort every box's particles by voxel before every n-th step (0:
+        never; default 20)."""
+        _lib.call("vpic_hip_ensemble_sort_interval", self.h, int(n))
+
     def tune(self, trial_steps=3):
         ms = ctypes.c_double()
         _lib.call("vpic_hip_ensemble_tune", self.h, int(trial_steps),
