@@ -149,16 +149,19 @@ def cpu_baseline(s, budget_s=12.0):
     return out
 
 
-def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3):
+def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3,
+                   sort_interval=20):
     """The batched ensemble: nbox counter-streaming beam boxes drawn from
     the seed (sides from {32,64,96,128}, 16-256 ppc, drift 0.05-0.3c,
-    spread 0.01c, immobile ions), tuned, then `steps` graph-replayed steps
-    timed; runs/s and aggregate particle-steps/s."""
+    spread 0.01c, immobile ions), every box sorted every sort_interval
+    steps, tuned, then `steps` graph-replayed steps timed; runs/s and
+    aggregate particle-steps/s."""
     import torch
     from vpic_amd import ensemble as E
     cfg = E.beam_config(nbox, seed)
     t0 = time.perf_counter()
     e = E.beam_ensemble(cfg, seed)
+    e.sort_interval(sort_interval)
     npart = sum(e.np)
     log("ensemble: %d boxes, %d particles, loaded in %.1f s"
         % (nbox, npart, time.perf_counter() - t0))
@@ -177,8 +180,10 @@ def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3):
             "tuned_shape": {"push_bs": shape[0], "push_chunk": shape[1],
                             "field_bs": shape[2], "field_chunk": shape[3],
                             "blocks_per_cu": shape[4]},
+            "sort_interval": sort_interval,
             "config": "256 2D periodic counter-streaming electron-beam boxes, "
-                      "seed %d, hipGraph-replayed step" % seed}
+                      "seed %d, hipGraph-replayed step, sort_p of every box "
+                      "every %d steps" % (seed, sort_interval)}
 
 
 def adjoint_probe(n=64, ppc=32, steps=200, ckpt=20):
@@ -218,13 +223,37 @@ def adjoint_probe(n=64, ppc=32, steps=200, ckpt=20):
     call(True)
     t_adj = time.perf_counter() - t0
     gnorm = float(ub.double().norm())
+    ub32 = ub.double()
+    # the same run in FP64 (the kernels over double) from the widened state
+    from vpic_amd import adjoint as A
+    p64 = torch.from_numpy(A.host_particles64(p.download())).cuda()
+    f64 = torch.zeros((g.nv, 18), dtype=torch.float64, device="cuda")
+    ub64 = torch.empty((npart, 3), dtype=torch.float64, device="cuda")
+
+    def call64(grads):
+        _lib.call("vpic_hip_adjoint_bfield_energy", p64.data_ptr(), npart,
+                  -1.0, 1.0, f64.data_ptr(), ctypes.byref(g), steps, ckpt, 8,
+                  ctypes.byref(en), ub64.data_ptr() if grads else None, None,
+                  None, None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    call64(False)
+    t_fwd64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    call64(True)
+    t_adj64 = time.perf_counter() - t0
+    diff = float((ub32 - ub64).norm() / ub64.norm())
     return {"cells": n ** 3, "particles": npart, "steps": steps,
             "checkpoint_interval": ckpt, "forward_s": t_fwd,
             "forward_and_adjoint_s": t_adj,
             "adjoint_over_forward": t_adj / t_fwd,
             "energy": en.value, "grad_norm": gnorm,
+            "fp64": {"forward_s": t_fwd64, "forward_and_adjoint_s": t_adj64,
+                     "adjoint_over_forward": t_adj64 / t_fwd64},
+            "fp32_grad_vs_fp64_rel_norm": diff,
             "config": "3D Weibel 64^3, 32 ppc electrons, T_perp/T_par=4, "
-                      "uth_par=0.05c, immobile ions, FP32"}
+                      "uth_par=0.05c, immobile ions, FP32 (and the same run "
+                      "in FP64)"}
 
 
 def main():
