@@ -59,22 +59,38 @@ def host_info():
     """The host's CPU model (lscpu), nproc, and the CPUs this process may
     run on."""
     import subprocess
-    model = ""
+    fields = {}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True,
                              timeout=20).stdout
         for line in out.splitlines():
-            if line.strip().startswith("Model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
     except Exception:
         pass
     try:
         allowed = len(os.sched_getaffinity(0))
     except AttributeError:
         allowed = os.cpu_count() or 1
-    return {"lscpu_model": model, "nproc": os.cpu_count() or 1,
-            "cpus_allowed": allowed}
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                   timeout=20).stdout.strip())
+    except Exception:
+        pass
+
+    def num(k):
+        try:
+            return int(fields.get(k, ""))
+        except ValueError:
+            return None
+    cores, sockets = num("Core(s) per socket"), num("Socket(s)")
+    return {"lscpu_model": fields.get("Model name", ""),
+            "lscpu_cpus": num("CPU(s)"),
+            "host_physical_cores": (cores * sockets) if cores and sockets
+            else None,
+            "nproc": nproc, "cpus_allowed": allowed}
 
 
 def cpu_threads(info):
@@ -143,9 +159,11 @@ def cpu_baseline(s, budget_s=12.0):
                       "left it, with its interpolators; %d timed calls after "
                       "3 warm-ups, %.1f s" % (len(p), n_step, el))}
     out.update(info)
-    out["threads_note"] = ("threads = the CPUs this process may use, capped "
-                           "at OMP_NUM_THREADS (the host's CPU share per "
-                           "GPU)")
+    out["threads_note"] = ("threads = the CPUs this process may use "
+                           "(nproc: the host's CPU share for this GPU), "
+                           "capped at OMP_NUM_THREADS; the host's physical "
+                           "cores beyond that share are not available to "
+                           "this process")
     return out
 
 

@@ -1,5 +1,5 @@
-// sort.hip -- sort_p on gfx950: stable counting sort of the particles by
-// voxel, and the voxel partition.
+// sort.hip -- sort_p on gfx950: stable sort of the particles by voxel, and
+// the voxel partition.
 //
 // Reference: src/species_advance/standard/pipeline/sort_p_pipeline.cc
 // (:143-214 fine subsort, :221-371 driver).  The reference's result is the
@@ -7,16 +7,23 @@
 // partition[v] = index of the first particle of voxel v, 0 below the first
 // interior voxel and np above the last (:335-340, :354-360).
 //
-// Device passes (np particles, nv voxels):
-//   k_count      key[k] = p[k].i; run-aggregated histogram (one atomic per run
-//                of equal keys in a wavefront)
-//   scan         partition = exclusive scan of the histogram (partition[nv]=np)
-//   k_scatter    perm[cursor[key]++] = k (one atomic per run; lanes of a run
-//                take consecutive slots in index order)
-//   k_fixup      equal-key segments sorted by particle index in LDS (one
-//                wavefront per voxel), which makes the order stable whatever
-//                order the atomics of different wavefronts took
-//   k_gather     out[j] = p[perm[j]]
+// Main path (every axis of the grid within RS_MAXB - 2 cells): three stable
+// counting passes over (key, index) pairs, x then y then z digit of the
+// voxel (least significant first; the voxel index is monotone in (z, y, x)):
+//   k_rs_hist    per block of RS_BLK elements, the count of each digit
+//                (LDS counters, runs of equal digits over consecutive lanes
+//                add once); the first pass reads the keys from the particles
+//   scan         exclusive scan of the (digit, block) counts
+//   k_rs_scatter every element to its digit's next slot of its block, in
+//                index order (ballot match on the digit: stable, no global
+//                atomics)
+//   k_gather_part  out[j] = p[perm[j]], and the partition from the sorted
+//                keys
+// Fallback for larger grids (k_count, scan, k_scatter, k_fixup, k_gather):
+// a counting sort with one global atomic per run of equal keys, whose
+// equal-key segments are then sorted by index in LDS.
+// Batched sort of an ensemble's boxes (k_ens_count, k_ens_scatter): one
+// counting sort over packed arrays, no order within a voxel.
 #include "common.h"
 
 namespace {
