@@ -184,7 +184,9 @@ def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3,
     log("ensemble: %d boxes, %d particles, loaded in %.1f s"
         % (nbox, npart, time.perf_counter() - t0))
     shape, tuned_ms = e.tune(tune_steps)
-    log("ensemble: tuned shape %s (%.2f ms per step)" % (shape, tuned_ms))
+    tile = e.tile()
+    log("ensemble: tuned shape %s, tile %s (%.2f ms per step)"
+        % (shape, tile, tuned_ms))
     e.step(2)  # settle
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -197,7 +199,8 @@ def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3,
             "ms_per_step": el * 1e3 / steps,
             "tuned_shape": {"push_bs": shape[0], "push_chunk": shape[1],
                             "field_bs": shape[2], "field_chunk": shape[3],
-                            "blocks_per_cu": shape[4]},
+                            "blocks_per_cu": shape[4],
+                            "tile": list(tile)},
             "sort_interval": sort_interval,
             "config": "256 2D periodic counter-streaming electron-beam boxes, "
                       "seed %d, hipGraph-replayed step, sort_p of every box "

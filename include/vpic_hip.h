@@ -710,6 +710,14 @@ int vpic_hip_ensemble_get_shape(const vpic_ensemble_t *e, int32_t out[5]);
  * sorts.  Default 20.  The sort is one counting sort over the packed
  * arrays, captured in a graph of its own. */
 int vpic_hip_ensemble_sort_interval(vpic_ensemble_t *e, int32_t interval);
+/* Tile of the ensemble's tiled push (cells per axis, clamped to each box;
+ * default 16x8x4): once a sort has written the partitions, every box is
+ * pushed tile by tile with LDS current windows, as vpic_hip_advance_p with a
+ * partition pushes one grid.  0,0,0 keeps the flat push (run-summed global
+ * atomics).  The tuner also picks the tile. */
+int vpic_hip_ensemble_tile(vpic_ensemble_t *e, int32_t tx, int32_t ty,
+                           int32_t tz);
+int vpic_hip_ensemble_get_tile(const vpic_ensemble_t *e, int32_t out[3]);
 /* The tuner: times each candidate shape over trial_steps graph replays from
  * the same saved state (restored afterwards), keeps the fastest and
  * returns its ms per step. */

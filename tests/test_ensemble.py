@@ -10,8 +10,11 @@ for every box, from the device's state before the step,
   * the fields equal the oracle's unload / synchronize_jf / advance_b /
     vacuum_advance_e / advance_b of the box from the device's accumulators,
     and the interpolators its load_interpolator_array, bit for bit.
-Steps alternate between graph replays (several in a row, with host work
-between) and phase-by-phase launches; every other step starts with the
+Run with the default tile of the tiled push (one tile per small box), with
+the flat push, and with 2x2x1 tiles (several tiles per box, so deposits
+cross tile windows and go through the out-of-window records and the
+merge).  Steps alternate between graph replays (several in a row, with host
+work between) and phase-by-phase launches; every other step starts with the
 batched sort_p of every box (the oracle sorts its copy too; the batched sort
 keeps no order within a voxel, so those steps compare the particles as a
 set, each pushed bit for bit, and check the voxel order).  The tuner keeps one of its
@@ -56,7 +59,9 @@ def _small_boxes(seed):
 
 
 @pytest.mark.gpu
-def test_ensemble_boxes_match_single_box_oracle(orc):
+@pytest.mark.parametrize("tile", [None, (0, 0, 0), (2, 2, 1)],
+                         ids=["tile-default", "flat", "tile-2x2x1"])
+def test_ensemble_boxes_match_single_box_oracle(orc, tile):
     import torch
     from oracle.oracle import host_grid
     from vpic_amd import _lib
@@ -69,6 +74,8 @@ def test_ensemble_boxes_match_single_box_oracle(orc):
         e.set_particles(b, p)
         e.set_fields(b, f)
     e.prepare()
+    if tile is not None:
+        e.tile(*tile)
     e.sort_interval(2)  # sorts before steps 0, 2 and 4
     mp = T.vacuum_params()
     for step, graph in enumerate((True, True, True, False, True)):

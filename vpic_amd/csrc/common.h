@@ -172,15 +172,35 @@ int launch_ens_field(int phase, vpic_field_t *F, vpic_interpolator_t *FI,
                      const EnsItem *items, int nitems, int *queue,
                      int nblocks, int bs, hipStream_t s);
 int launch_ens_reset(int *queues, int n, hipStream_t s);
+// The tiled push of a batched ensemble (push.hip): per-box tiles of
+// tx x ty x tz cells (clamped to each box), their LDS windows and
+// out-of-window sub-buckets, and the box-relative partitions (box b's at
+// ens_tiled_epart() + voff_b + b, nv_b + 1 ints, written by the batched
+// sort).  launch_ens_push_tiled enqueues the push and the merge of every
+// box tile; its counters (4 ints: [1] movers, [3] particles off their box)
+// accumulate until the caller clears them.
+struct EnsTiled;
+int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
+                     const int *np, const float *q, const float *m,
+                     const int64_t *voff, const int64_t *poff,
+                     vpic_particle_t *P, const vpic_interpolator_t *FI,
+                     vpic_accumulator_t *A, int tx, int ty, int tz);
+void ens_tiled_destroy(EnsTiled *t);
+int *ens_tiled_epart(EnsTiled *t);
+int *ens_tiled_counters(EnsTiled *t);
+int launch_ens_push_tiled(EnsTiled *et, int *queue, int nblocks,
+                          hipStream_t s);
 // Sort of every box's particles by voxel (sort_p per box on the packed
 // arrays), enqueued without a host synchronisation so that it can be
 // captured: keys (np_total ints), count and start (nv_total + 1 ints each),
 // sums (device_exclusive_scan's), aux (np_total particles).  Equal voxels
-// keep no particular order.
+// keep no particular order.  With epart, box b's partition (box-relative,
+// nv_b + 1 ints) is written at epart + voff_b + b.
 int launch_ens_sort(vpic_particle_t *P, vpic_particle_t *aux,
                     const EnsPushBox *boxes, const EnsItem *items, int nitems,
-                    int64_t np_total, int64_t nv_total, int *keys, int *count,
-                    int *start, int *sums, int *nbad, hipStream_t s);
+                    int nbox, int64_t np_total, int64_t nv_total, int *keys,
+                    int *count, int *start, int *sums, int *nbad, int *epart,
+                    hipStream_t s);
 int launch_clear_accumulator_n(vpic_accumulator_t *a, int64_t nv,
                                hipStream_t s);
 int launch_ens_push(vpic_particle_t *P, const vpic_interpolator_t *FI,

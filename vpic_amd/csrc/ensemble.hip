@@ -40,11 +40,18 @@ enum {
   Q_ADVE,
   Q_ADVB2,
   Q_LOAD,
+  Q_TILES,  // the tiled push's (box, tile) queue (items held by EnsTiled)
   Q_COUNT
 };
 
 struct Shape {
   int push_bs, push_chunk, field_bs, field_chunk, blocks_per_cu;
+};
+
+// Tile of the tiled push (cells per axis, clamped to each box); 0: the
+// flat push.
+struct TileShape {
+  int tx, ty, tz;
 };
 
 }  // namespace
@@ -78,6 +85,11 @@ struct vpic_ensemble {
       *ssums = nullptr;
   hipGraph_t sgraph = nullptr;
   hipGraphExec_t sexec = nullptr;
+  // the tiled push (after a first sort: it reads the sort's partitions)
+  TileShape tile = {16, 8, 4};
+  EnsTiled *tiled = nullptr;
+  bool sorted_once = false;
+  bool graph_tiled = false;  // the captured step uses the tiled push
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int64_t steps = 0;
@@ -87,6 +99,7 @@ struct vpic_ensemble {
     if (graph) (void)hipGraphDestroy(graph);
     if (sexec) (void)hipGraphExecDestroy(sexec);
     if (sgraph) (void)hipGraphDestroy(sgraph);
+    if (tiled) ens_tiled_destroy(tiled);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     void *bufs[] = {P,   F,     FI,     A,      pbox,  fbox,  items,
                     queues, aux, skeys, scount, sstart, ssums};
@@ -140,12 +153,27 @@ int ensure_sort_buffers(vpic_ensemble *e) {
   return VPIC_OK;
 }
 
+// The tiled push's state for the current tile shape (none for tile 0).
+int ensure_tiled(vpic_ensemble *e) {
+  if (e->tiled || e->tile.tx <= 0) return VPIC_OK;
+  return ens_tiled_create(&e->tiled, e->nbox, e->grids.data(), e->np.data(),
+                          e->q.data(), e->m.data(), e->voff.data(),
+                          e->poff.data(), e->P, e->FI, e->A, e->tile.tx,
+                          e->tile.ty, e->tile.tz);
+}
+
 int enqueue_sort(vpic_ensemble *e, hipStream_t s) {
   return launch_ens_sort(e->P, e->aux, e->pbox,
                          e->items + e->item_off[Q_PUSH],
                          e->item_off[Q_PUSH + 1] - e->item_off[Q_PUSH],
-                         e->np_total, e->nv_total, e->skeys, e->scount,
-                         e->sstart, e->ssums, e->queues + Q_COUNT, s);
+                         e->nbox, e->np_total, e->nv_total, e->skeys,
+                         e->scount, e->sstart, e->ssums, e->queues + Q_COUNT,
+                         ens_tiled_epart(e->tiled), s);
+}
+
+// The step uses the tiled push once a sort has written the partitions.
+bool use_tiled(const vpic_ensemble *e) {
+  return e->tiled && e->sorted_once;
 }
 
 // Work items of every phase for the current shape.
@@ -167,7 +195,9 @@ int build_items(vpic_ensemble *e) {
     for (int b = 0; b < e->nbox; b++) {
       const vpic_grid_t &g = e->grids[b];
       const int n[3] = {g.nx, g.ny, g.nz};
-      if (q == Q_PUSH) {
+      if (q == Q_TILES) {
+        // (box, tile) items live with the tiled push's state
+      } else if (q == Q_PUSH) {
         add_range(b, -1, e->np[b], e->shape.push_chunk);
       } else if (q == Q_FOLDX || q == Q_FOLDY || q == Q_FOLDZ) {
         const int X = q - Q_FOLDX;
@@ -211,12 +241,18 @@ int enqueue_step(vpic_ensemble *e, hipStream_t s) {
   if ((rc = launch_ens_reset(e->queues, Q_COUNT, s))) return rc;
   if ((rc = launch_clear_accumulator_n(e->A, e->nv_total, s))) return rc;
   const int nblk = e->shape.blocks_per_cu * e->ncu;
-  if ((rc = launch_ens_push(e->P, e->FI, e->A, e->pbox,
-                            e->items + e->item_off[Q_PUSH],
-                            e->item_off[Q_PUSH + 1] - e->item_off[Q_PUSH],
-                            e->queues + Q_PUSH, e->queues + Q_COUNT, nblk,
-                            e->shape.push_bs, s)))
+  if (use_tiled(e)) {
+    // one persistent workgroup per CU (the LDS window takes the CU)
+    if ((rc = launch_ens_push_tiled(e->tiled, e->queues + Q_TILES, e->ncu, s)))
+      return rc;
+  } else if ((rc = launch_ens_push(e->P, e->FI, e->A, e->pbox,
+                                   e->items + e->item_off[Q_PUSH],
+                                   e->item_off[Q_PUSH + 1] -
+                                       e->item_off[Q_PUSH],
+                                   e->queues + Q_PUSH, e->queues + Q_COUNT,
+                                   nblk, e->shape.push_bs, s))) {
     return rc;
+  }
   if ((rc = field_phase(e, Q_UNLOAD, ENS_UNLOAD, s)) ||
       (rc = field_phase(e, Q_FOLDX, ENS_FOLD, s)) ||
       (rc = field_phase(e, Q_FOLDY, ENS_FOLD, s)) ||
@@ -251,6 +287,7 @@ int capture_one(vpic_ensemble *e, int (*enqueue)(vpic_ensemble *, hipStream_t),
 int capture(vpic_ensemble *e) {
   drop_graph(e);
   int rc;
+  e->graph_tiled = use_tiled(e);
   if ((rc = capture_one(e, enqueue_step, &e->graph, &e->exec))) return rc;
   if (e->sort_interval > 0) {
     if ((rc = ensure_sort_buffers(e)) ||
@@ -262,8 +299,9 @@ int capture(vpic_ensemble *e) {
 
 int run_steps(vpic_ensemble *e, int n, int use_graph) {
   int rc;
-  if (use_graph && !e->exec && (rc = capture(e))) return rc;
-  if (e->sort_interval > 0 && (rc = ensure_sort_buffers(e))) return rc;
+  if (e->sort_interval > 0 &&
+      ((rc = ensure_sort_buffers(e)) || (rc = ensure_tiled(e))))
+    return rc;
   for (int k = 0; k < n; k++) {
     if (e->sort_interval > 0 && e->steps % e->sort_interval == 0) {
       if (use_graph && !e->sexec && (rc = capture(e))) return rc;
@@ -271,7 +309,12 @@ int run_steps(vpic_ensemble *e, int n, int use_graph) {
         VPIC_HIP(hipGraphLaunch(e->sexec, e->stream));
       else if ((rc = enqueue_sort(e, e->stream)))
         return rc;
+      e->sorted_once = true;
     }
+    // (re)capture when the step's push changed (first sort, shape change)
+    if (use_graph && (!e->exec || e->graph_tiled != use_tiled(e)) &&
+        (rc = capture(e)))
+      return rc;
     if (use_graph)
       VPIC_HIP(hipGraphLaunch(e->exec, e->stream));
     else if ((rc = enqueue_step(e, e->stream)))
@@ -436,11 +479,21 @@ extern "C" int vpic_hip_ensemble_step(vpic_ensemble_t *e, int32_t nstep,
   if (!e || nstep < 0)
     return vpic_set_error(VPIC_EINVAL, "ensemble_step: bad arguments");
   int rc;
+  int *tcnt = ens_tiled_counters(e->tiled);
+  if (tcnt)
+    VPIC_HIP(hipMemsetAsync(tcnt, 0, 4 * sizeof(int), e->stream));
   if ((rc = run_steps(e, nstep, use_graph))) return rc;
   VPIC_HIP(hipStreamSynchronize(e->stream));
   int bad[2] = {0, 0};
   VPIC_HIP(hipMemcpy(bad, e->queues + Q_COUNT, sizeof bad,
                      hipMemcpyDeviceToHost));
+  tcnt = ens_tiled_counters(e->tiled);  // (created by the first sorting step)
+  if (tcnt) {
+    int tc[4] = {0, 0, 0, 0};
+    VPIC_HIP(hipMemcpy(tc, tcnt, sizeof tc, hipMemcpyDeviceToHost));
+    bad[0] += tc[3];
+    bad[1] += tc[1];
+  }
   if (bad[0] || bad[1])
     return vpic_set_error(VPIC_EINVAL,
                           "ensemble_step: %d particles with a voxel off their "
@@ -460,6 +513,31 @@ extern "C" int vpic_hip_ensemble_shape(vpic_ensemble_t *e, int32_t push_bs,
     return vpic_set_error(VPIC_EINVAL, "ensemble_shape: out of range");
   e->shape = {push_bs, push_chunk, field_bs, field_chunk, blocks_per_cu};
   return build_items(e);
+}
+
+extern "C" int vpic_hip_ensemble_tile(vpic_ensemble_t *e, int32_t tx,
+                                      int32_t ty, int32_t tz) {
+  if (!e || tx < 0 || ty < 0 || tz < 0 || (tx == 0) != (ty == 0) ||
+      (tx == 0) != (tz == 0))
+    return vpic_set_error(VPIC_EINVAL, "ensemble_tile: bad arguments");
+  if (tx == e->tile.tx && ty == e->tile.ty && tz == e->tile.tz)
+    return VPIC_OK;
+  VPIC_HIP(hipStreamSynchronize(e->stream));
+  if (e->tiled) ens_tiled_destroy(e->tiled);
+  e->tiled = nullptr;
+  e->sorted_once = false;
+  e->tile = {tx, ty, tz};
+  drop_graph(e);
+  return VPIC_OK;
+}
+
+extern "C" int vpic_hip_ensemble_get_tile(const vpic_ensemble_t *e,
+                                          int32_t out[3]) {
+  if (!e || !out) return vpic_set_error(VPIC_EINVAL, "bad arguments");
+  out[0] = e->tile.tx;
+  out[1] = e->tile.ty;
+  out[2] = e->tile.tz;
+  return VPIC_OK;
 }
 
 extern "C" int vpic_hip_ensemble_sort_interval(vpic_ensemble_t *e,
@@ -484,9 +562,12 @@ extern "C" int vpic_hip_ensemble_get_shape(const vpic_ensemble_t *e,
   return VPIC_OK;
 }
 
-// The tuner: every candidate shape is timed over `trial_steps` graph
-// replays (hipEvents on the ensemble's stream) from the same saved state,
-// the state is restored after each, and the fastest shape is kept.
+// The tuner: every candidate is timed over `trial_steps` graph replays
+// (hipEvents on the ensemble's stream) from the same saved state, restored
+// after each; first the tile of the tiled push (with the current launch
+// shape), then the launch shape (with the best tile).  With sorting on,
+// each trial starts from a sort of the saved state (untimed; it writes the
+// partitions the tiled push reads) and runs without further sorts.
 extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
                                       double *host_best_ms) {
   if (!e || trial_steps < 1)
@@ -498,7 +579,10 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
       {64, 1024, 128, 1024, 16},  {256, 32768, 256, 16384, 1},
       {128, 4096, 128, 2048, 8},  {256, 8192, 512, 8192, 2},
   };
+  static const TileShape tiles[] = {{16, 16, 1}, {32, 8, 1}, {16, 8, 1},
+                                    {8, 8, 1},   {16, 8, 4}, {8, 8, 8}};
   const int ncand = sizeof(cands) / sizeof(cands[0]);
+  const int ntile = sizeof(tiles) / sizeof(tiles[0]);
   void *saveP = nullptr, *saveF = nullptr, *saveFI = nullptr;
   int rc;
   const size_t pbytes = sizeof(vpic_particle_t) * e->np_total;
@@ -515,36 +599,69 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
   const Shape orig = e->shape;
+  const TileShape orig_tile = e->tile;
   const int64_t steps0 = e->steps;
-  // the candidates are timed on the step alone (the sort's launch shape is
-  // not tuned)
   const int interval0 = e->sort_interval;
-  e->sort_interval = 0;
-  Shape best = orig;
-  float best_ms = 1e30f;
-  rc = VPIC_OK;
+  e->sort_interval = 0;  // no sorts inside a trial
   (void)hipMemcpyAsync(saveP, e->P, pbytes, hipMemcpyDeviceToDevice, s);
   (void)hipMemcpyAsync(saveF, e->F, fbytes, hipMemcpyDeviceToDevice, s);
   (void)hipMemcpyAsync(saveFI, e->FI, ibytes, hipMemcpyDeviceToDevice, s);
-  for (int c = 0; c < ncand && rc == VPIC_OK; c++) {
-    e->shape = cands[c];
-    if ((rc = build_items(e)) || (rc = capture(e))) break;
-    // one untimed replay (first-launch costs), then the timed ones
-    if ((rc = run_steps(e, 1, 1))) break;
+  auto restore = [&]() {
+    (void)hipMemcpyAsync(e->P, saveP, pbytes, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(e->F, saveF, fbytes, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(e->FI, saveFI, ibytes, hipMemcpyDeviceToDevice, s);
+  };
+  auto set_tile = [&](const TileShape &t) -> int {
+    if (t.tx == e->tile.tx && t.ty == e->tile.ty && t.tz == e->tile.tz &&
+        (e->tiled || t.tx <= 0))
+      return VPIC_OK;
+    (void)hipStreamSynchronize(s);
+    if (e->tiled) ens_tiled_destroy(e->tiled);
+    e->tiled = nullptr;
+    e->sorted_once = false;
+    e->tile = t;
+    return ensure_tiled(e);
+  };
+  // one trial of the current shape and tile: ms per step
+  auto trial = [&](float &ms) -> int {
+    int r;
+    if (interval0 > 0) {
+      if ((r = ensure_sort_buffers(e)) || (r = enqueue_sort(e, s))) return r;
+      e->sorted_once = true;
+    }
+    if ((r = build_items(e)) || (r = capture(e))) return r;
+    if ((r = run_steps(e, 1, 1))) return r;  // first-launch costs
     (void)hipEventRecord(t0, s);
-    if ((rc = run_steps(e, trial_steps, 1))) break;
+    if ((r = run_steps(e, trial_steps, 1))) return r;
     (void)hipEventRecord(t1, s);
     (void)hipEventSynchronize(t1);
-    float ms = 0;
     (void)hipEventElapsedTime(&ms, t0, t1);
     ms /= trial_steps;
+    restore();
+    return VPIC_OK;
+  };
+  Shape best = orig;
+  TileShape best_tile = orig_tile;
+  float best_ms = 1e30f;
+  rc = VPIC_OK;
+  // the tiles (the tiled push needs the sort's partitions)
+  for (int c = 0; c < (interval0 > 0 ? ntile : 0) && rc == VPIC_OK; c++) {
+    float ms = 0;
+    if ((rc = set_tile(tiles[c])) || (rc = trial(ms))) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_tile = tiles[c];
+    }
+  }
+  if (rc == VPIC_OK) rc = set_tile(best_tile);
+  for (int c = 0; c < ncand && rc == VPIC_OK; c++) {
+    float ms = 0;
+    e->shape = cands[c];
+    if ((rc = trial(ms))) break;
     if (ms < best_ms) {
       best_ms = ms;
       best = cands[c];
     }
-    (void)hipMemcpyAsync(e->P, saveP, pbytes, hipMemcpyDeviceToDevice, s);
-    (void)hipMemcpyAsync(e->F, saveF, fbytes, hipMemcpyDeviceToDevice, s);
-    (void)hipMemcpyAsync(e->FI, saveFI, ibytes, hipMemcpyDeviceToDevice, s);
   }
   (void)hipStreamSynchronize(s);
   e->steps = steps0;  // the trials' steps were undone with the state
@@ -556,10 +673,16 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   (void)hipFree(saveFI);
   if (rc) {
     e->shape = orig;
+    (void)set_tile(orig_tile);
+    e->sorted_once = false;
     (void)build_items(e);
     return rc;
   }
   e->shape = best;
+  // the partitions now describe the last trial's sort of the saved state,
+  // not the restored order: any sort's partition still gives every particle
+  // to exactly one tile, and the next sort refreshes it
+  e->sorted_once = interval0 > 0 && e->tiled != nullptr;
   if ((rc = build_items(e))) return rc;  // drops the trial graphs
   if (host_best_ms) *host_best_ms = best_ms;
   return VPIC_OK;

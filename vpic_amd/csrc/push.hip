@@ -35,6 +35,7 @@
 // particle index, truncated at max_nm (excess rolled back to the voxel
 // index, advance_p_pipeline.cc:228-236).
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -949,13 +950,50 @@ struct TileGeom {
   int tx, ty, tz;     // tile extent in cells
   int ntx, nty, ntz;  // tiles per axis
   unsigned mx, my;    // fast_div multipliers for nx+2, ny+2 (0: divide)
+  int stage_win;      // coefficients staged for the window (1) or the tile
 };
 
-constexpr int TILE_BS = 1024;             // threads per workgroup
+#ifndef VPIC_TILE_BS
+#define VPIC_TILE_BS 1024
+#endif
+#ifndef VPIC_PUSH_DEPTH
+#define VPIC_PUSH_DEPTH 1
+#endif
+// A box of a batched ensemble as the tiled push sees it: a single grid at
+// offsets of the packed arrays, with its partition (box-relative, nv + 1
+// entries), window slots and out-of-window sub-buckets.
+struct EnsTileBox {
+  vpic_particle_t *p;
+  const int *part;
+  const vpic_interpolator_t *f0;
+  float *acc;
+  float *wbuf;
+  OowBuckets OB;
+  PushCoef C;
+  GridDev G;
+  TileGeom T;
+  int np, wstride;
+};
+
+struct EnsTileItem {
+  int box, tile;
+};
+
+constexpr int TILE_BS = VPIC_TILE_BS;     // threads per workgroup
 constexpr int TILE_WAVES = TILE_BS / 64;
+// waves per SIMD the launch bounds ask for (one workgroup per CU)
+constexpr int TILE_MIN_WAVES = TILE_BS / 256;
+// slots a lane loads ahead of the one it pushes (1 to 3)
+constexpr int PUSH_DEPTH = VPIC_PUSH_DEPTH;
+static_assert(PUSH_DEPTH >= 1 && PUSH_DEPTH <= 3, "prefetch depth 1..3");
+static_assert(TILE_BS % 256 == 0, "whole waves on every SIMD");
 constexpr int TILE_MAXROWS = 256;         // ty * tz
-constexpr int TILE_MAXCELLS = 512;        // tx * ty * tz (LDS coefficients)
+constexpr int TILE_MAXCELLS = 512;        // tx * ty * tz
 constexpr int TILE_WIN_BYTES = 104 * 1024;  // largest LDS window
+// The dynamic LDS of the tiled push: the window (96 B per window cell)
+// followed by the staged interpolation coefficients (72 B per cell) of the
+// whole window when both fit here, else of the tile's cells only.
+constexpr int TILE_DYN_BYTES = 140 * 1024;
 constexpr int WAVE_Q = 64;                // crosser queue of one wavefront
 
 // Walks tile slots in increasing order: particle index of slot k (rows of
@@ -1093,13 +1131,19 @@ __device__ __forceinline__ void drain_wave_queue(
 // k_bin_decide), so a binning push launches all three instances and the two
 // that do not match return at once; compiling each state separately keeps
 // the unused paths' values out of the scalar registers of the particle loop.
+//
+// The body is a device function of the tile index, so that a batched
+// launch over many grids (k_ens_push_tiled) runs the same code per (grid,
+// tile) work item.
 template <int GS, int MODE>
-__global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
-    vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
-    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
-    float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
+__device__ __forceinline__ void tiled_push(
+    int tile, vpic_particle_t *__restrict__ p, int np,
+    const int *__restrict__ part, const vpic_interpolator_t *__restrict__ f0,
+    float *__restrict__ acc, float *__restrict__ wbuf, int wstride,
+    vpic_particle_t *__restrict__ trash,
     vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
-    PushCoef C, GridDev G, TileGeom T, BinArgs Bn, OowBuckets OB) {
+    const PushCoef &C, const GridDev &G, const TileGeom &T, const BinArgs &Bn,
+    const OowBuckets &OB) {
   extern __shared__ unsigned long long lds_win[];  // 12 * ncell
   __shared__ vpic_particle_mover_t wq[TILE_WAVES][WAVE_Q];
   __shared__ int row_start[TILE_MAXROWS];
@@ -1108,16 +1152,10 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   __shared__ int bin_cnt[27];
   __shared__ int oow_cnt;
   __shared__ int oow_dcnt[27];
-  // interpolation coefficients of the tile's cells (gathers from LDS cost
-  // far fewer cache requests than per-lane global gathers once the
-  // particles have drifted out of voxel order)
-  __shared__ float4 ic_e0[TILE_MAXCELLS], ic_e1[TILE_MAXCELLS],
-      ic_e2[TILE_MAXCELLS], ic_b0[TILE_MAXCELLS];
-  __shared__ float2 ic_b1[TILE_MAXCELLS];
 
   const int t = threadIdx.x;
   const int lane = lane_id();
-  int b = blockIdx.x;
+  int b = tile;
   const int btx = b % T.ntx;
   b /= T.ntx;
   const int bty = b % T.nty;
@@ -1161,7 +1199,7 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   W.btx = btx;
   W.bty = bty;
   W.btz = btz;
-  W.self = blockIdx.x;
+  W.self = tile;
   const int ncell = W.wnx * W.wny * W.wnz;
   W.ncell = ncell;
   VoxDiv VD;
@@ -1195,10 +1233,24 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     row_pref[r + 1] = e - s;
   }
   for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
-  const int cx = x1 - x0 + 1, cxy = cx * rows_y;
-  for (int c = t; c < cxy * (z1 - z0 + 1); c += TILE_BS) {
-    const int v = vox(x0 + c % cx, y0 + (c / cx) % rows_y, z0 + c / cxy,
-                      G.nx, G.ny);
+  // Interpolation coefficients staged in LDS (after the window) for the
+  // cells of the staging box: the window when the launch made room for it,
+  // else the tile.  A particle in the box reads its 18 coefficients from
+  // LDS, any other from global memory (a dependent global load, which the
+  // wavefront waits for, so the box should cover the drifted particles).
+  const int sw = T.stage_win ? 1 : 0;
+  const int sx0 = x0 - sw, sy0 = y0 - sw, sz0 = z0 - sw;
+  const int snx = x1 - x0 + 1 + 2 * sw, sny = y1 - y0 + 1 + 2 * sw,
+            snz = z1 - z0 + 1 + 2 * sw;
+  const int sncell = snx * sny * snz;
+  float4 *ic_e0 = reinterpret_cast<float4 *>(lds_win + 12 * ncell);
+  float4 *ic_e1 = ic_e0 + sncell;
+  float4 *ic_e2 = ic_e1 + sncell;
+  float4 *ic_b0 = ic_e2 + sncell;
+  float2 *ic_b1 = reinterpret_cast<float2 *>(ic_b0 + sncell);
+  for (int c = t; c < sncell; c += TILE_BS) {
+    const int v = vox(sx0 + c % snx, sy0 + (c / snx) % sny,
+                      sz0 + c / (snx * sny), G.nx, G.ny);
     const float4 *fp = reinterpret_cast<const float4 *>(f0 + v);
     ic_e0[c] = fp[0];
     ic_e1[c] = fp[1];
@@ -1211,16 +1263,37 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
   if (t == 0) oow_cnt = 0;
   if (t < 27) oow_dcnt[t] = 0;
   __syncthreads();
-  if (t == 0) {
-    row_pref[0] = 0;
-    for (int r = 0; r < nrow; r++) row_pref[r + 1] += row_pref[r];
+  if (t < 64) {
+    // inclusive prefix of the row lengths by the first wavefront: four rows
+    // per lane (nrow <= TILE_MAXROWS = 256), then the lanes' totals
+    static_assert(TILE_MAXROWS <= 256, "four rows per lane");
+    int v[4], run = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = 4 * t + i;
+      run += r < nrow ? row_pref[r + 1] : 0;
+      v[i] = run;
+    }
+    int incl = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(incl, d);
+      if (t >= d) incl += o;
+    }
+    const int off = incl - run;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = 4 * t + i;
+      if (r < nrow) row_pref[r + 1] = v[i] + off;
+    }
+    if (t == 0) row_pref[0] = 0;
   }
   __syncthreads();
   // the slots: the partition rows, or this tile's list
   int total = row_pref[nrow];
   const int *L = Bn.list;
   if (listed) {
-    int ls = Bn.lstart[blockIdx.x], le = Bn.lstart[blockIdx.x + 1];
+    int ls = Bn.lstart[tile], le = Bn.lstart[tile + 1];
     ls = min(max(ls, 0), np);
     le = min(max(le, ls), np);
     L += ls;
@@ -1252,25 +1325,39 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
 
   RowWalk rw;
   rw.start(row_start, row_pref);
-  int k = g0 + j;
-  // particle of the current slot (a list entry outside [0, np) is skipped)
-  int n = 0;
-  bool cur_ok = k < g1;
-  if (cur_ok) {
-    n = listed ? L[k] : rw.particle(row_start, row_pref, nrow, k);
-    cur_ok = (unsigned)n < (unsigned)np;
-    if (!cur_ok) n = 0;
+  int k = g0 + j;  // the slot being pushed
+  // The particle of slot ks (false past the group's run; a list entry
+  // outside [0, np) is skipped).  Asked for increasing ks only.
+  auto particle_of = [&](int ks, int &nn) -> bool {
+    nn = 0;
+    if (ks >= g1) return false;
+    nn = listed ? L[ks] : rw.particle(row_start, row_pref, nrow, ks);
+    const bool ok = (unsigned)nn < (unsigned)np;
+    if (!ok) nn = 0;
+    return ok;
+  };
+  // A lane's particles in flight: the one it pushes and the next
+  // PUSH_DEPTH - 1 loaded ahead; each slot loads the particle PUSH_DEPTH
+  // slots on into the set it frees, so a load is waited for PUSH_DEPTH
+  // slots after it is issued.
+  struct PSlot {
+    float4 A, B;
+    int n;
+    bool ok;
+  };
+  PSlot P0, P1, P2, P3;
+  P0.ok = particle_of(k, P0.n);
+  load_particle(p, P0.n, P0.ok, P0.A, P0.B);
+  if (PUSH_DEPTH >= 2) {
+    P1.ok = particle_of(k + GS, P1.n);
+    load_particle(p, P1.n, P1.ok, P1.A, P1.B);
   }
-  // list entry of the next slot, loaded one slot ahead
-  int nidx = (listed && k + GS < g1) ? L[k + GS] : 0;
-  // One slot of this lane: push (A, B) = particle n while (An, Bn) is
-  // loaded.  The loop below alternates two register pairs, so a prefetch
-  // is waited for only one full slot after it is issued.  (Loading two
-  // slots ahead gains nothing: a coefficient load for a particle outside
-  // the tile is younger than the older prefetch, and the in-order counter
-  // of outstanding loads makes the wait for it drain both.)
-  float4 A0, B0, A1, B1;
-  load_particle(p, n, cur_ok, A0, B0);
+  if (PUSH_DEPTH >= 3) {
+    P2.ok = particle_of(k + 2 * GS, P2.n);
+    load_particle(p, P2.n, P2.ok, P2.A, P2.B);
+  }
+  const float4 A0 = P0.A, B0 = P0.B;
+  const bool cur_ok = P0.ok;
   PushCoef Cs = C;  // C with the deposit charge scaled by the window's 2^k
   {
     // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
@@ -1301,8 +1388,10 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     if (!W.pre) Cs.qsp = C.qsp;
   }
   int it = 0;
-  auto slot = [&](const float4 A, const float4 B, float4 &An, float4 &Bn_) {
-    const bool valid = cur_ok;
+  auto slot = [&](const PSlot &cur, PSlot &pref) {
+    const float4 A = cur.A, B = cur.B;
+    const int n = cur.n;
+    const bool valid = cur.ok;
     const int iraw = __float_as_int(A.w);
     const bool active = valid && (unsigned)iraw < (unsigned)G.nv;
     nbad += valid && !active;
@@ -1316,8 +1405,10 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
                (unsigned)wz < (unsigned)W.wnz)
                   ? wx + W.wnx * (wy + W.wny * wz)
                   : -1;
-      if (x >= x0 && x <= x1 && y >= y0 && y <= y1 && z >= z0 && z <= z1) {
-        const int c = (x - x0) + cx * (y - y0) + cxy * (z - z0);
+      const int sx = x - sx0, sy = y - sy0, sz = z - sz0;
+      if ((unsigned)sx < (unsigned)snx && (unsigned)sy < (unsigned)sny &&
+          (unsigned)sz < (unsigned)snz) {
+        const int c = sx + snx * (sy + sny * sz);
         F.e0 = ic_e0[c];
         F.e1 = ic_e1[c];
         F.e2 = ic_e2[c];
@@ -1331,18 +1422,9 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
     // prefetch this lane's next particle (issued after the coefficients,
     // so waiting for them leaves the prefetch in flight), then the list
     // entry after it
-    const int kn = k + GS;
-    int nn = 0;
-    bool nok = kn < g1;
-    if (listed) {
-      nn = nidx;
-      nok = nok && (unsigned)nn < (unsigned)np;
-    } else if (nok) {
-      nn = rw.particle(row_start, row_pref, nrow, kn);
-    }
-    if (!nok) nn = 0;
-    load_particle(p, nn, nok, An, Bn_);
-    if (listed) nidx = kn + GS < g1 ? L[kn + GS] : 0;
+    const int kn = k + PUSH_DEPTH * GS;
+    pref.ok = particle_of(kn, pref.n);
+    load_particle(p, pref.n, pref.ok, pref.A, pref.B);
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch issued here
 
     // every lane stores (an idle lane to its own trash slot), so the
@@ -1402,26 +1484,46 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
       }
       qn += c;
     }
-    n = nn;
-    cur_ok = nok;
-    k = kn;
+    k += GS;
     it++;
   };
-  while (it + 1 < steps) {
-    slot(A0, B0, A1, B1);
-    slot(A1, B1, A0, B0);
+  // the sets rotate: slot `it` pushes set it % (PUSH_DEPTH + 1) and loads
+  // into set (it + PUSH_DEPTH) % (PUSH_DEPTH + 1)
+  if (PUSH_DEPTH == 1) {
+    while (it + 1 < steps) {
+      slot(P0, P1);
+      slot(P1, P0);
+    }
+    if (it < steps) slot(P0, P1);
+  } else if (PUSH_DEPTH == 2) {
+    while (it + 2 < steps) {
+      slot(P0, P2);
+      slot(P1, P0);
+      slot(P2, P1);
+    }
+    if (it < steps) slot(P0, P2);
+    if (it < steps) slot(P1, P0);
+  } else {
+    while (it + 3 < steps) {
+      slot(P0, P3);
+      slot(P1, P0);
+      slot(P2, P1);
+      slot(P3, P2);
+    }
+    if (it < steps) slot(P0, P3);
+    if (it < steps) slot(P1, P0);
+    if (it < steps) slot(P2, P1);
   }
-  if (it < steps) slot(A0, B0, A1, B1);
   if (nbad) atomicAdd(cnt + 3, nbad);
   if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
   if (qn)
     drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt, BS);
   __syncthreads();
-  if (keyed && t < 27) Bn.counts[blockIdx.x * 27 + t] = bin_cnt[t];
+  if (keyed && t < 27) Bn.counts[tile * 27 + t] = bin_cnt[t];
   if (Bn.oow && t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
-  if (OB.cap > 0 && t < 27) OB.cnt[27 * blockIdx.x + t] = oow_dcnt[t];
+  if (OB.cap > 0 && t < 27) OB.cnt[27 * tile + t] = oow_dcnt[t];
   // The window, as floats, to this tile's slot (component-major, ncell).
-  float *wout = wbuf + (int64_t)blockIdx.x * wstride;
+  float *wout = wbuf + (int64_t)tile * wstride;
   const double inv = 1.0 / W.scale;
   for (int c = t; c < ncell; c += TILE_BS) {
 #pragma unroll
@@ -1429,6 +1531,17 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
       wout[k * ncell + c] =
           (float)((double)(long long)lds_win[12 * c + k] * inv);
   }
+}
+
+template <int GS, int MODE>
+__global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_advance_p_tiled(
+    vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
+    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
+    float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
+    vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
+    PushCoef C, GridDev G, TileGeom T, BinArgs Bn, OowBuckets OB) {
+  tiled_push<GS, MODE>(blockIdx.x, p, np, part, f0, acc, wbuf, wstride, trash,
+                       rlist, rcap, cnt, C, G, T, Bn, OB);
 }
 
 // Add the tiles' windows into the accumulator: voxel v receives the
@@ -1442,13 +1555,14 @@ __global__ __launch_bounds__(TILE_BS, 4) void k_advance_p_tiled(
 // in LDS as 64-bit fixed point with a scale set from its largest component
 // and record count (sums stay below 2^50, so the rounding of each record
 // is exact to 2^-50 of that bound), and added with the windows.
-__global__ __launch_bounds__(256) void k_merge_windows(
-    float *__restrict__ acc, const float *__restrict__ wbuf, int wstride,
-    GridDev G, TileGeom T, OowBuckets OB) {
+__device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
+                                           const float *__restrict__ wbuf,
+                                           int wstride, const GridDev &G,
+                                           const TileGeom &T,
+                                           const OowBuckets &OB) {
   __shared__ unsigned long long bsum[12 * TILE_MAXCELLS];
   __shared__ float bmax[256];
-  int b = blockIdx.x;
-  const int tile = b;
+  int b = tile;
   const int bx = b % T.ntx;
   b /= T.ntx;
   const int by = b % T.nty;
@@ -1582,6 +1696,49 @@ __global__ __launch_bounds__(256) void k_merge_windows(
     a4[1] = v1;
     a4[2] = v2;
   }
+}
+
+__global__ __launch_bounds__(256) void k_merge_windows(
+    float *__restrict__ acc, const float *__restrict__ wbuf, int wstride,
+    GridDev G, TileGeom T, OowBuckets OB) {
+  merge_tile(blockIdx.x, acc, wbuf, wstride, G, T, OB);
+}
+
+// --------------------------------------------------------------------------
+// Batched boxes, tiled: one work item per (box, tile).  Each box is a
+// single-grid tiled push at an offset (its particles sorted by the batched
+// sort, its partition box-relative), run by the same device functions as
+// k_advance_p_tiled / k_merge_windows.  The boxes are periodic, so no mover
+// comes back; cnt[3] counts particles with a voxel off their box.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_ens_push_tiled(
+    const EnsTileBox *__restrict__ boxes, const EnsTileItem *__restrict__ items,
+    int nitems, int *__restrict__ queue, vpic_particle_t *__restrict__ trash,
+    vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt) {
+  __shared__ int s_item;
+  const BinArgs Bn = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= nitems) break;
+    const EnsTileItem I = items[it];
+    const EnsTileBox &B = boxes[__builtin_amdgcn_readfirstlane(I.box)];
+    tiled_push<16, 0>(__builtin_amdgcn_readfirstlane(I.tile), B.p, B.np,
+                      B.part, B.f0, B.acc, B.wbuf, B.wstride, trash, rlist, 0,
+                      cnt, B.C, B.G, B.T, Bn, B.OB);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ens_merge(
+    const EnsTileBox *__restrict__ boxes, const EnsTileItem *__restrict__ items,
+    int nitems) {
+  const EnsTileItem I = items[blockIdx.x];
+  const EnsTileBox &B = boxes[__builtin_amdgcn_readfirstlane(I.box)];
+  merge_tile(__builtin_amdgcn_readfirstlane(I.tile), B.acc, B.wbuf, B.wstride,
+             B.G, B.T, B.OB);
 }
 
 // --------------------------------------------------------------------------
@@ -2177,7 +2334,11 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       T.mx = magic_ok ? div_magic(sx1) : 0;
       T.my = magic_ok ? div_magic(sy1) : 0;
       const size_t wcells = (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2);
-      const size_t lds = sizeof(unsigned long long) * 12 * wcells;
+      const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
+      T.stage_win = win_bytes + 72 * wcells <= (size_t)TILE_DYN_BYTES;
+      const size_t lds =
+          win_bytes +
+          72 * (T.stage_win ? wcells : (size_t)T.tx * T.ty * T.tz);
       const int nblk = T.ntx * T.nty * T.ntz;
       const int wstride = (int)(12 * wcells);
       if ((rc = scratch_reserve(S.wbuf, sizeof(float) * (size_t)wstride *
@@ -2208,7 +2369,7 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         for (int mode = 0; mode < 3; mode++)
           VPIC_HIP(hipFuncSetAttribute(
               reinterpret_cast<const void *>(tiled_kernel(g_group, mode)),
-              hipFuncAttributeMaxDynamicSharedMemorySize, TILE_WIN_BYTES));
+              hipFuncAttributeMaxDynamicSharedMemorySize, TILE_DYN_BYTES));
         attr_set[ak] = true;
       }
       // Tile lists from the codes of the last tiled push of this array.
@@ -2316,6 +2477,171 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
   hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
                      xcap, (int2 *)S.sortbuf.ptr, pm, max_nm, cnt, dev_nm);
   VPIC_LAUNCHED("k_order_movers");
+  return VPIC_OK;
+}
+
+// --------------------------------------------------------------------------
+// The tiled push of a batched ensemble (k_ens_push_tiled + k_ens_merge):
+// per-box tile geometry, window slots, out-of-window sub-buckets and the
+// box-relative partitions the batched sort writes.
+// --------------------------------------------------------------------------
+struct EnsTiled {
+  int nbox = 0, nitems = 0;
+  size_t lds = 0;
+  EnsTileBox *boxes = nullptr;
+  EnsTileItem *items = nullptr;
+  int *epart = nullptr;  // box b's partition at voff_b + b (nv_b + 1 ints)
+  float *wbuf = nullptr;
+  int *ocnt = nullptr;
+  float4 *orec = nullptr;
+  vpic_particle_t *trash = nullptr;
+  vpic_particle_mover_t *rlist = nullptr;
+  int *cnt = nullptr;  // the push's counters: [1] movers, [3] bad voxels
+  ~EnsTiled() {
+    void *bufs[] = {boxes, items, epart, wbuf, ocnt, orec, trash, rlist, cnt};
+    for (void *b : bufs)
+      if (b) (void)hipFree(b);
+  }
+};
+
+namespace {
+int ens_dev_alloc(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return vpic_set_error(VPIC_ENOMEM, "ensemble tiles: hipMalloc(%zu): %s",
+                          bytes, hipGetErrorString(e));
+  }
+  return VPIC_OK;
+}
+}  // namespace
+
+int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
+                     const int *np, const float *q, const float *m,
+                     const int64_t *voff, const int64_t *poff,
+                     vpic_particle_t *P, const vpic_interpolator_t *FI,
+                     vpic_accumulator_t *A, int tx, int ty, int tz) {
+  *out = nullptr;
+  if (tx < 1 || ty < 1 || tz < 1)
+    return vpic_set_error(VPIC_EINVAL, "ensemble tiles: bad tile shape");
+  std::vector<EnsTileBox> hb(nbox);
+  std::vector<EnsTileItem> hi;
+  int64_t wtot = 0, otot = 0, rtot = 0, nvtot = 0;
+  size_t lds = 0;
+  for (int b = 0; b < nbox; b++) {
+    const vpic_grid_t &g = grids[b];
+    TileGeom T = {};
+    T.tx = tx < g.nx ? tx : g.nx;
+    T.ty = ty < g.ny ? ty : g.ny;
+    T.tz = tz < g.nz ? tz : g.nz;
+    if (T.ty * T.tz > TILE_MAXROWS || T.tx * T.ty * T.tz > TILE_MAXCELLS ||
+        (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2) * 96 > TILE_WIN_BYTES)
+      return vpic_set_error(VPIC_EINVAL, "ensemble tiles: tile %dx%dx%d of "
+                            "box %d does not fit the LDS window",
+                            T.tx, T.ty, T.tz, b);
+    T.ntx = (g.nx + T.tx - 1) / T.tx;
+    T.nty = (g.ny + T.ty - 1) / T.ty;
+    T.ntz = (g.nz + T.tz - 1) / T.tz;
+    const unsigned sx1 = g.nx + 2, sy1 = g.ny + 2;
+    if ((uint64_t)g.nv * sx1 >= ((uint64_t)1 << 32))
+      return vpic_set_error(VPIC_EINVAL, "ensemble tiles: box %d too large",
+                            b);
+    T.mx = div_magic(sx1);
+    T.my = div_magic(sy1);
+    const size_t wcells = (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2);
+    const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
+    T.stage_win = win_bytes + 72 * wcells <= (size_t)TILE_DYN_BYTES;
+    const size_t l = win_bytes +
+                     72 * (T.stage_win ? wcells : (size_t)T.tx * T.ty * T.tz);
+    if (l > lds) lds = l;
+    const int ntiles = T.ntx * T.nty * T.ntz;
+    int64_t cap = (int64_t)np[b] / ntiles / 32;
+    cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
+    EnsTileBox &B = hb[b];
+    B.p = P + poff[b];
+    B.part = nullptr;  // set below (device addresses)
+    B.f0 = FI + voff[b];
+    B.acc = (float *)(A + voff[b]);
+    B.wstride = (int)(12 * wcells);
+    B.wbuf = reinterpret_cast<float *>((intptr_t)wtot);    // offset for now
+    B.OB.cnt = reinterpret_cast<int *>((intptr_t)otot);
+    B.OB.rec = reinterpret_cast<float4 *>((intptr_t)rtot);
+    B.OB.cap = (int)cap;
+    B.C = push_coef(q[b], m[b], &g);
+    B.G = make_grid_dev(&g);
+    B.T = T;
+    B.np = np[b];
+    B.part = reinterpret_cast<const int *>((intptr_t)(voff[b] + b));
+    wtot += (int64_t)ntiles * B.wstride;
+    otot += 27 * (int64_t)ntiles;
+    rtot += 4 * 27 * cap * (int64_t)ntiles;
+    nvtot += g.nv + 1;
+    for (int t = 0; t < ntiles; t++) hi.push_back({b, t});
+  }
+  EnsTiled *et = new EnsTiled;
+  et->nbox = nbox;
+  et->nitems = (int)hi.size();
+  et->lds = lds;
+  int rc;
+  if ((rc = ens_dev_alloc((void **)&et->boxes, sizeof(EnsTileBox) * nbox)) ||
+      (rc = ens_dev_alloc((void **)&et->items,
+                          sizeof(EnsTileItem) * hi.size())) ||
+      (rc = ens_dev_alloc((void **)&et->epart, sizeof(int) * nvtot)) ||
+      (rc = ens_dev_alloc((void **)&et->wbuf, sizeof(float) * wtot)) ||
+      (rc = ens_dev_alloc((void **)&et->ocnt, sizeof(int) * otot)) ||
+      (rc = ens_dev_alloc((void **)&et->orec, sizeof(float4) * rtot)) ||
+      (rc = ens_dev_alloc((void **)&et->trash,
+                          sizeof(vpic_particle_t) * TILE_BS)) ||
+      (rc = ens_dev_alloc((void **)&et->rlist, sizeof(vpic_particle_mover_t))) ||
+      (rc = ens_dev_alloc((void **)&et->cnt, sizeof(int) * 64))) {
+    delete et;
+    return rc;
+  }
+  for (EnsTileBox &B : hb) {
+    B.wbuf = et->wbuf + (intptr_t)B.wbuf;
+    B.OB.cnt = et->ocnt + (intptr_t)B.OB.cnt;
+    B.OB.rec = et->orec + (intptr_t)B.OB.rec;
+    B.part = et->epart + (intptr_t)B.part;
+  }
+  hipError_t he;
+  if ((he = hipMemcpy(et->boxes, hb.data(), sizeof(EnsTileBox) * nbox,
+                      hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemcpy(et->items, hi.data(), sizeof(EnsTileItem) * hi.size(),
+                      hipMemcpyHostToDevice)) != hipSuccess ||
+      (he = hipMemset(et->cnt, 0, sizeof(int) * 64)) != hipSuccess) {
+    delete et;
+    return vpic_check_hip(he, "ensemble tiles");
+  }
+  *out = et;
+  return VPIC_OK;
+}
+
+void ens_tiled_destroy(EnsTiled *t) { delete t; }
+
+int *ens_tiled_epart(EnsTiled *t) { return t ? t->epart : nullptr; }
+
+int *ens_tiled_counters(EnsTiled *t) { return t ? t->cnt : nullptr; }
+
+int launch_ens_push_tiled(EnsTiled *et, int *queue, int nblocks,
+                          hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    VPIC_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void *>(k_ens_push_tiled),
+        hipFuncAttributeMaxDynamicSharedMemorySize, TILE_DYN_BYTES));
+    attr_set = true;
+  }
+  if (et->nitems <= 0) return VPIC_OK;
+  const int nb = nblocks < et->nitems ? nblocks : et->nitems;
+  hipLaunchKernelGGL(k_ens_push_tiled, dim3(nb), dim3(TILE_BS), et->lds, s,
+                     (const EnsTileBox *)et->boxes,
+                     (const EnsTileItem *)et->items, et->nitems, queue,
+                     et->trash, et->rlist, et->cnt);
+  VPIC_LAUNCHED("k_ens_push_tiled");
+  hipLaunchKernelGGL(k_ens_merge, dim3(et->nitems), dim3(256), 0, s,
+                     (const EnsTileBox *)et->boxes,
+                     (const EnsTileItem *)et->items, et->nitems);
+  VPIC_LAUNCHED("k_ens_merge");
   return VPIC_OK;
 }
 

@@ -692,12 +692,28 @@ __global__ __launch_bounds__(256) void k_ens_scatter(
   }
 }
 
+// Box b's partition, box-relative, at epart + voff_b + b (nv_b + 1 ints):
+// the offsets of its voxels in the packed order less the box's first
+// particle (start holds the exclusive scan over the packed keys, whose entry
+// after box b's last voxel is box b + 1's first particle).
+__global__ __launch_bounds__(256) void k_ens_part(
+    const EnsPushBox *__restrict__ boxes, const int *__restrict__ start,
+    int *__restrict__ epart) {
+  const int b = blockIdx.x;
+  const EnsPushBox B = boxes[b];
+  const int base = (int)B.voff;
+  const int first = (int)B.poff;
+  for (int v = threadIdx.x; v <= B.G.nv; v += blockDim.x)
+    epart[base + b + v] = start[base + v] - first;
+}
+
 }  // namespace
 
 int launch_ens_sort(vpic_particle_t *P, vpic_particle_t *aux,
                     const EnsPushBox *boxes, const EnsItem *items, int nitems,
-                    int64_t np_total, int64_t nv_total, int *keys, int *count,
-                    int *start, int *sums, int *nbad, hipStream_t s) {
+                    int nbox, int64_t np_total, int64_t nv_total, int *keys,
+                    int *count, int *start, int *sums, int *nbad, int *epart,
+                    hipStream_t s) {
   if (nitems <= 0 || np_total <= 0) return VPIC_OK;
   const int64_t nslots = nv_total + 1;
   int fb = blocks_for(nslots, 256);
@@ -709,6 +725,11 @@ int launch_ens_sort(vpic_particle_t *P, vpic_particle_t *aux,
   VPIC_LAUNCHED("k_ens_count");
   int rc;
   if ((rc = device_exclusive_scan(count, start, nslots, sums, s))) return rc;
+  if (epart) {
+    hipLaunchKernelGGL(k_ens_part, dim3(nbox), dim3(256), 0, s, boxes,
+                       (const int *)start, epart);
+    VPIC_LAUNCHED("k_ens_part");
+  }
   hipLaunchKernelGGL(k_ens_scatter, dim3(nitems), dim3(256), 0, s, P, boxes,
                      items, (const int *)keys, start, aux);
   VPIC_LAUNCHED("k_ens_scatter");

@@ -109,6 +109,17 @@ class Ensemble:
         never; default 20)."""
         _lib.call("vpic_hip_ensemble_sort_interval", self.h, int(n))
 
+    def tile(self, tx=None, ty=None, tz=None):
+        """The tiled push's tile (cells per axis; 0,0,0: the flat push);
+        returns the tile in use."""
+        cur = (ctypes.c_int32 * 3)()
+        _lib.call("vpic_hip_ensemble_get_tile", self.h, cur)
+        new = [v if v is not None else cur[k] for k, v in
+               enumerate((tx, ty, tz))]
+        if new != list(cur):
+            _lib.call("vpic_hip_ensemble_tile", self.h, *new)
+        return tuple(new)
+
     def tune(self, trial_steps=3):
         ms = ctypes.c_double()
         _lib.call("vpic_hip_ensemble_tune", self.h, int(trial_steps),
