@@ -40,6 +40,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def set_tile(spec):
+    """Push tile "tx,ty,tz" (one-cell window halo) or "tx,ty,tz,halo"."""
+    from vpic_amd import _lib
+    v = [int(x) for x in spec.split(",")]
+    if len(v) == 4:
+        _lib.call("vpic_hip_push_tile_window", *v)
+    else:
+        _lib.call("vpic_hip_push_tile", *v)
+
+
 def build_sim(nx, ny, nz, ppc, seed=2026, sort_interval=20):
     from vpic_amd import sim as S
     from vpic_amd import types as T
@@ -293,7 +303,7 @@ def main():
                     help="skip the differentiable-run measurement")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tile", type=str, default=None,
-                    help="push tile tx,ty,tz (cells)")
+                    help="push tile tx,ty,tz (cells) or tx,ty,tz,halo")
     ap.add_argument("--direct-runs", type=int, default=None,
                     help="push deposit threshold (runs per wavefront)")
     ap.add_argument("--sweep", type=str, default=None,
@@ -390,7 +400,7 @@ def main():
             tile, runs = f[0], int(f[1])
             gs = int(f[2]) if len(f) > 2 else 16
             bmode = int(f[3]) if len(f) > 3 else 0
-            _lib.call("vpic_hip_push_tile", *[int(v) for v in tile.split(",")])
+            set_tile(tile)
             _lib.call("vpic_hip_push_direct_runs", runs)
             _lib.call("vpic_hip_push_group", gs)
             _lib.call("vpic_hip_push_bin_mode", bmode)
@@ -414,7 +424,7 @@ def main():
                               "value": npart * args.steps / el}), flush=True)
         return
     if args.tile:
-        _lib.call("vpic_hip_push_tile", *[int(v) for v in args.tile.split(",")])
+        set_tile(args.tile)
     if args.direct_runs:
         _lib.call("vpic_hip_push_direct_runs", args.direct_runs)
     log("building %dx%dx%d, %d ppc/species" % (args.nx, args.ny, args.nz,

@@ -229,9 +229,18 @@ int vpic_hip_advance_p_noalloc(vpic_particle_t *p, int32_t np, float q,
                                const vpic_grid_t *g, void *stream);
 
 /* Tile shape (cells) of the partition-driven push; its LDS window holds
- * (tx+2)(ty+2)(tz+2) accumulators.  Default 8x8x8.  No reference
- * counterpart (the reference's pipelines split particles, not cells). */
+ * (tx+2)(ty+2)(tz+2) accumulators (a halo of one cell).  Default 8x8x8.
+ * No reference counterpart (the reference's pipelines split particles,
+ * not cells). */
 int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
+
+/* The same with the window's halo: 1 or 2 cells, (tx+2h)(ty+2h)(tz+2h)
+ * accumulators of 96 B, which with 72 B per tile cell must fit 140 KiB
+ * of LDS.  A wider halo keeps the deposits of particles that drifted
+ * further since the last sort_p in LDS.  Results do not depend on it
+ * beyond the order of the current sums.  No reference counterpart. */
+int vpic_hip_push_tile_window(int32_t tx, int32_t ty, int32_t tz,
+                              int32_t halo);
 
 /* Wavefronts whose 64 particles fall in at least `runs` runs of equal
  * voxels deposit lane by lane instead of summing runs first (default 65,

@@ -501,6 +501,38 @@ def test_tiled_push_parity(dev, orc, tile, tile_reset):
         assert H.acc_rel_err(ag, ao) <= ACC_TOL
 
 
+# tiles with a two-cell window halo (tx, ty, tz, halo): tiles narrower than
+# the halo put a voxel in the windows of up to five tiles per axis
+WINDOWS = [(8, 8, 4, 2), (6, 6, 6, 2), (1, 1, 1, 2), (3, 2, 5, 2),
+           (2, 3, 2, 2)]
+
+
+@pytest.mark.parametrize("win", WINDOWS,
+                         ids=["x".join(map(str, w[:3])) + "h%d" % w[3]
+                              for w in WINDOWS])
+def test_tiled_push_halo2_parity(dev, orc, win, tile_reset):
+    """The two-cell halo: particles drifted up to several cells since the
+    sort (their deposits in the halo of their tile's window, in the window
+    of a neighbouring tile's halo, or outside every window)."""
+    _lib.call("vpic_hip_push_tile_window", *win)
+    g = _box(9, 7, 6)
+    for (q, m, uth, steps) in ((-1.0, 1.0, 0.1, 0), (-1.0, 1.0, 0.3, 6),
+                               (-1.0, 1.0, 0.6, 5), (1.0, 25.0, 0.02, 2)):
+        p, fi = _inputs(orc, g, 11, uth, 33, shuffle=True)
+        hg = host_grid(g)
+        part = orc.sort_p(p, g)
+        for _ in range(steps):
+            orc.advance_p(p, q, m, H.acc_array(g), fi, hg)
+        po, ao = p.copy(), H.acc_array(g)
+        orc.advance_p(po, q, m, ao, fi, hg)
+        say("tiled advance_p window", win, "uth", uth, "drift steps", steps)
+        pg, ag, nmg, igg, _ = _gpu_advance_p(dev, p, q, m, fi, g,
+                                             partition=part)
+        assert H.same_bits(pg, po)
+        assert (nmg, igg) == (0, 0)
+        assert H.acc_rel_err(ag, ao) <= ACC_TOL
+
+
 def test_tiled_push_multistep_device_sort(dev, orc):
     """sort_p and several tiled pushes entirely on the device (the partition
     goes stale as the particles drift), each step against the oracle."""
@@ -632,3 +664,8 @@ def test_tiled_push_walls_movers(dev, orc):
 def test_tile_shape_rejected():
     with pytest.raises(_lib.VpicError, match="rejected"):
         _lib.call("vpic_hip_push_tile", 64, 64, 64)
+    # 8x8x8 with a two-cell halo: 12^3 window cells of 96 B exceed the LDS
+    with pytest.raises(_lib.VpicError, match="rejected"):
+        _lib.call("vpic_hip_push_tile_window", 8, 8, 8, 2)
+    with pytest.raises(_lib.VpicError, match="not 1 or 2"):
+        _lib.call("vpic_hip_push_tile_window", 4, 4, 4, 3)

@@ -33,6 +33,7 @@ _SIGS = {
     "vpic_hip_advance_p_noalloc": [_vp, _i32, _f32, _f32, _vp, _i32, _vp,
                                    _vp, _vp, _pg, _vp],
     "vpic_hip_push_tile": [_i32, _i32, _i32],
+    "vpic_hip_push_tile_window": [_i32, _i32, _i32, _i32],
     "vpic_hip_push_direct_runs": [_i32],
     "vpic_hip_push_group": [_i32],
     "vpic_hip_push_bin_mode": [_i32],

@@ -239,9 +239,20 @@ extern "C" int vpic_hip_push_direct_runs(int32_t runs) {
 }
 
 extern "C" int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz) {
-  CHECK(push_set_tile(tx, ty, tz) == VPIC_OK,
-        "push tile %dx%dx%d rejected (window must fit 96 KiB of LDS, "
-        "ty*tz <= 256)", tx, ty, tz);
+  CHECK(push_set_tile(tx, ty, tz, 1) == VPIC_OK,
+        "push tile %dx%dx%d rejected (window of 96 B per cell plus 72 B per "
+        "tile cell must fit 140 KiB of LDS, ty*tz <= 256, tx*ty*tz <= 512)",
+        tx, ty, tz);
+  return VPIC_OK;
+}
+
+extern "C" int vpic_hip_push_tile_window(int32_t tx, int32_t ty, int32_t tz,
+                                         int32_t halo) {
+  CHECK(halo == 1 || halo == 2, "push window halo %d is not 1 or 2", halo);
+  CHECK(push_set_tile(tx, ty, tz, halo) == VPIC_OK,
+        "push tile %dx%dx%d with halo %d rejected (window of 96 B per cell "
+        "plus 72 B per tile cell must fit 140 KiB of LDS, ty*tz <= 256, "
+        "tx*ty*tz <= 512)", tx, ty, tz, halo);
   return VPIC_OK;
 }
 

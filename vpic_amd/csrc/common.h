@@ -230,7 +230,7 @@ int launch_advance_p_noalloc(vpic_particle_t *p, int np, float q, float m,
                              const vpic_interpolator_t *f,
                              const vpic_grid_t *g, hipStream_t s);
 int push_noalloc_setup();
-int push_set_tile(int tx, int ty, int tz);
+int push_set_tile(int tx, int ty, int tz, int halo);
 int push_set_direct_runs(int runs);
 int push_set_group(int gs);
 int push_set_bin_mode(int mode);

@@ -14,11 +14,12 @@
 //  k_advance_p_tiled (used when the sort-time voxel partition is given):
 //    one workgroup per 3D tile of cells.  The tile's particles are the rows
 //    of the partition written by the last sort_p.  Currents go to an LDS
-//    window (the tile plus a one-cell halo, 48 B per cell) and the window is
-//    added to the global accumulator once at the end.  Cell crossers are
-//    queued in LDS and moved by the same workgroup, so their segments also
-//    land in the window.  Deposits outside the window (fast particles,
-//    periodic wrap) fall back to global atomics.
+//    window (the tile plus a halo of one or two cells, 96 B of 64-bit fixed
+//    point per cell) and the window is added to the global accumulator once
+//    at the end (k_merge_windows).  Cell crossers are queued in LDS and
+//    moved by the same workgroup, so their segments also land in the
+//    window.  Deposits outside the window (drifted particles, periodic
+//    wrap) go to per-tile sub-buckets that the merge adds.
 //
 //  k_advance_p (any particle order, and the particles after the partition):
 //    grid-stride over the particles, deposits straight to global atomics;
@@ -950,7 +951,9 @@ struct TileGeom {
   int tx, ty, tz;     // tile extent in cells
   int ntx, nty, ntz;  // tiles per axis
   unsigned mx, my;    // fast_div multipliers for nx+2, ny+2 (0: divide)
-  int stage_win;      // coefficients staged for the window (1) or the tile
+  int stage_win;      // coefficients staged for the tile and a one-cell
+                      // ring (1) or the tile only (0)
+  int halo;           // window cells beyond the tile on every side (1 or 2)
 };
 
 #ifndef VPIC_TILE_BS
@@ -989,7 +992,6 @@ static_assert(PUSH_DEPTH >= 1 && PUSH_DEPTH <= 3, "prefetch depth 1..3");
 static_assert(TILE_BS % 256 == 0, "whole waves on every SIMD");
 constexpr int TILE_MAXROWS = 256;         // ty * tz
 constexpr int TILE_MAXCELLS = 512;        // tx * ty * tz
-constexpr int TILE_WIN_BYTES = 104 * 1024;  // largest LDS window
 // The dynamic LDS of the tiled push: the window (96 B per window cell)
 // followed by the staged interpolation coefficients (72 B per cell) of the
 // whole window when both fit here, else of the tile's cells only.
@@ -1177,12 +1179,13 @@ __device__ __forceinline__ void tiled_push(
   WindowSink W;
   W.lds = (lds_u64 *)lds_win;
   W.acc = acc;
-  W.wx0 = x0 - 1;
-  W.wy0 = y0 - 1;
-  W.wz0 = z0 - 1;
-  W.wnx = x1 - x0 + 3;
-  W.wny = y1 - y0 + 3;
-  W.wnz = z1 - z0 + 3;
+  const int H = T.halo;
+  W.wx0 = x0 - H;
+  W.wy0 = y0 - H;
+  W.wz0 = z0 - H;
+  W.wnx = x1 - x0 + 1 + 2 * H;
+  W.wny = y1 - y0 + 1 + 2 * H;
+  W.wnz = z1 - z0 + 1 + 2 * H;
   W.sx1 = G.nx + 2;
   W.sy1 = G.ny + 2;
   W.mx = T.mx;
@@ -1234,9 +1237,10 @@ __device__ __forceinline__ void tiled_push(
   }
   for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
   // Interpolation coefficients staged in LDS (after the window) for the
-  // cells of the staging box: the window when the launch made room for it,
-  // else the tile.  A particle in the box reads its 18 coefficients from
-  // LDS, any other from global memory (a dependent global load, which the
+  // cells of the staging box: the tile and a one-cell ring around it (which
+  // stays within the ghost layers) when the launch made room for it, else
+  // the tile.  A particle in the box reads its 18 coefficients from LDS,
+  // any other from global memory (a dependent global load, which the
   // wavefront waits for, so the box should cover the drifted particles).
   const int sw = T.stage_win ? 1 : 0;
   const int sx0 = x0 - sw, sy0 = y0 - sw, sz0 = z0 - sw;
@@ -1545,10 +1549,11 @@ __global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_advance_p_tiled(
 }
 
 // Add the tiles' windows into the accumulator: voxel v receives the
-// window of its own tile and, when it lies on a face of that tile, the
-// halo of the neighbouring tiles across that face (windows do not wrap).
-// One workgroup per tile, one thread per voxel of the tile, so a workgroup
-// reads its own window and the faces of its neighbours' windows; every
+// window of its own tile and, when it lies within the halo (H cells) of a
+// face of that tile, the halo of the neighbouring tiles across that face
+// (windows do not wrap).  One workgroup per tile, one thread per voxel of
+// the tile, so a workgroup reads its own window and the face layers of its
+// neighbours' windows; every
 // cell of acc is written by exactly one thread.
 //
 // The tile's bucket of out-of-window deposits (OowBuckets) is summed first,
@@ -1652,29 +1657,26 @@ __device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
   }
   for (int c = t; c < ncell; c += blockDim.x) {
     const int x = x0 + c % cx, y = y0 + (c / cx) % cy, z = z0 + c / (cx * cy);
-    // tiles whose window holds v, per axis: own, and the neighbours across
-    // the tile faces that v lies on (both, for a tile one cell wide)
-    int ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0}, oz[3] = {0, 0, 0};
-    int nox = 1, noy = 1, noz = 1;
-    if (x == x0 && bx > 0) ox[nox++] = -1;
-    if (x == x0 + cx - 1 && bx + 1 < T.ntx) ox[nox++] = 1;
-    if (y == y0 && by > 0) oy[noy++] = -1;
-    if (y == y0 + cy - 1 && by + 1 < T.nty) oy[noy++] = 1;
-    if (z == z0 && bz > 0) oz[noz++] = -1;
-    if (z == z0 + cz - 1 && bz + 1 < T.ntz) oz[noz++] = 1;
+    // tiles whose window holds v, per axis: those holding a cell within
+    // the halo of v, [tile(v - H), tile(v + H)] clamped to the grid
+    const int H = T.halo;
+    const int tx0 = (max(x - H, 1) - 1) / T.tx, tx1 = (min(x + H, G.nx) - 1) / T.tx;
+    const int ty0 = (max(y - H, 1) - 1) / T.ty, ty1 = (min(y + H, G.ny) - 1) / T.ty;
+    const int tz0 = (max(z - H, 1) - 1) / T.tz, tz1 = (min(z + H, G.nz) - 1) / T.tz;
     float s[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) s[k] = 0.0f;
-    for (int a = 0; a < nox; a++)
-      for (int bb = 0; bb < noy; bb++)
-        for (int cc = 0; cc < noz; cc++) {
-          const int tx = bx + ox[a], ty = by + oy[bb], tz = bz + oz[cc];
+    for (int tz = tz0; tz <= tz1; tz++)
+      for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
           const int wx0 = tx * T.tx, wy0 = ty * T.ty, wz0 = tz * T.tz;
-          const int wnx = min(T.tx, G.nx - wx0) + 2;
-          const int wny = min(T.ty, G.ny - wy0) + 2;
-          const int wnz = min(T.tz, G.nz - wz0) + 2;
+          const int wnx = min(T.tx, G.nx - wx0) + 2 * H;
+          const int wny = min(T.ty, G.ny - wy0) + 2 * H;
+          const int wnz = min(T.tz, G.nz - wz0) + 2 * H;
           const int nc = wnx * wny * wnz;
-          const int pos = (x - wx0) + wnx * ((y - wy0) + wny * (z - wz0));
+          // the window of tile (tx, ty, tz) starts at cell 1 + wx0 - H
+          const int pos = (x - 1 - wx0 + H) +
+                          wnx * ((y - 1 - wy0 + H) + wny * (z - 1 - wz0 + H));
           const float *w =
               wbuf + (int64_t)(tx + T.ntx * (ty + T.nty * tz)) * wstride + pos;
 #pragma unroll
@@ -2072,7 +2074,7 @@ int pow2_at_least(int64_t n) {
   return (int)p;
 }
 
-TileGeom g_tile = {8, 8, 8, 0, 0, 0, 0, 0};
+TileGeom g_tile = {8, 8, 8, 0, 0, 0, 0, 0, 0, 1};
 int g_group = 16;  // lanes sharing one run of particles in the tiled push
 int g_bin_mode = 0;  // tile lists: 0 off, 1 when drift warrants, 2 always
 
@@ -2127,19 +2129,31 @@ unsigned div_magic(unsigned d) {
   return (unsigned)((((uint64_t)1 << 32) + d - 1) / d);
 }
 
+// Cells of the LDS window of a tx x ty x tz tile with a halo of h cells.
+size_t tile_win_cells(int tx, int ty, int tz, int h) {
+  return (size_t)(tx + 2 * h) * (ty + 2 * h) * (tz + 2 * h);
+}
+
+// The tile fits the tiled push: rows and cells within the kernel's tables,
+// and its window (96 B per cell) plus at least the tile's own staged
+// coefficients (72 B per cell) within the dynamic LDS.
+bool tile_fits(int tx, int ty, int tz, int h) {
+  return tx >= 1 && ty >= 1 && tz >= 1 && (h == 1 || h == 2) &&
+         ty * tz <= TILE_MAXROWS && tx * ty * tz <= TILE_MAXCELLS &&
+         96 * tile_win_cells(tx, ty, tz, h) + 72 * (size_t)tx * ty * tz <=
+             (size_t)TILE_DYN_BYTES;
+}
+
 }  // namespace
 
-// Tile shape of the tiled push (set by the tuner; the window of a tile is
-// (tx+2)(ty+2)(tz+2) cells of 48 B in LDS).
-int push_set_tile(int tx, int ty, int tz) {
-  if (tx < 1 || ty < 1 || tz < 1 || ty * tz > TILE_MAXROWS ||
-      tx * ty * tz > TILE_MAXCELLS)
-    return VPIC_EINVAL;
-  if ((size_t)(tx + 2) * (ty + 2) * (tz + 2) * 96 > TILE_WIN_BYTES)
-    return VPIC_EINVAL;
+// Tile shape and window halo of the tiled push (the window of a tile is
+// (tx+2h)(ty+2h)(tz+2h) cells of 96 B in LDS).
+int push_set_tile(int tx, int ty, int tz, int halo) {
+  if (!tile_fits(tx, ty, tz, halo)) return VPIC_EINVAL;
   g_tile.tx = tx;
   g_tile.ty = ty;
   g_tile.tz = tz;
+  g_tile.halo = halo;
   return VPIC_OK;
 }
 
@@ -2333,12 +2347,13 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
       T.mx = magic_ok ? div_magic(sx1) : 0;
       T.my = magic_ok ? div_magic(sy1) : 0;
-      const size_t wcells = (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2);
+      const size_t wcells = tile_win_cells(T.tx, T.ty, T.tz, T.halo);
       const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
-      T.stage_win = win_bytes + 72 * wcells <= (size_t)TILE_DYN_BYTES;
+      const size_t scells = tile_win_cells(T.tx, T.ty, T.tz, 1);
+      T.stage_win = win_bytes + 72 * scells <= (size_t)TILE_DYN_BYTES;
       const size_t lds =
           win_bytes +
-          72 * (T.stage_win ? wcells : (size_t)T.tx * T.ty * T.tz);
+          72 * (T.stage_win ? scells : (size_t)T.tx * T.ty * T.tz);
       const int nblk = T.ntx * T.nty * T.ntz;
       const int wstride = (int)(12 * wcells);
       if ((rc = scratch_reserve(S.wbuf, sizeof(float) * (size_t)wstride *
@@ -2534,8 +2549,8 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
     T.tx = tx < g.nx ? tx : g.nx;
     T.ty = ty < g.ny ? ty : g.ny;
     T.tz = tz < g.nz ? tz : g.nz;
-    if (T.ty * T.tz > TILE_MAXROWS || T.tx * T.ty * T.tz > TILE_MAXCELLS ||
-        (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2) * 96 > TILE_WIN_BYTES)
+    T.halo = 1;
+    if (!tile_fits(T.tx, T.ty, T.tz, T.halo))
       return vpic_set_error(VPIC_EINVAL, "ensemble tiles: tile %dx%dx%d of "
                             "box %d does not fit the LDS window",
                             T.tx, T.ty, T.tz, b);
@@ -2548,11 +2563,12 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
                             b);
     T.mx = div_magic(sx1);
     T.my = div_magic(sy1);
-    const size_t wcells = (size_t)(T.tx + 2) * (T.ty + 2) * (T.tz + 2);
+    const size_t wcells = tile_win_cells(T.tx, T.ty, T.tz, T.halo);
     const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
-    T.stage_win = win_bytes + 72 * wcells <= (size_t)TILE_DYN_BYTES;
+    const size_t scells = tile_win_cells(T.tx, T.ty, T.tz, 1);
+    T.stage_win = win_bytes + 72 * scells <= (size_t)TILE_DYN_BYTES;
     const size_t l = win_bytes +
-                     72 * (T.stage_win ? wcells : (size_t)T.tx * T.ty * T.tz);
+                     72 * (T.stage_win ? scells : (size_t)T.tx * T.ty * T.tz);
     if (l > lds) lds = l;
     const int ntiles = T.ntx * T.nty * T.ntz;
     int64_t cap = (int64_t)np[b] / ntiles / 32;
