@@ -461,6 +461,10 @@ def main():
     assert bad == 0
 
     value = npart * args.steps / el
+    tile = (ctypes.c_int32 * 4)()
+    _lib.call("vpic_hip_push_get_tile", tile)
+    stale = ctypes.c_int64()
+    _lib.call("vpic_hip_sort_keys_stale", ctypes.byref(stale))
     avg_launch_s = push_ms.value / 1e3 / max(launches.value, 1)
     bytes_per_launch = BYTES_PER_PARTICLE * push_parts.value / max(
         launches.value, 1)
@@ -491,6 +495,9 @@ def main():
         "phases_ms_per_step": {k: v / args.steps for k, v in phases.items()},
         "push_kernel": {
             "name": "k_advance_p_tiled",
+            "tile": list(tile[:3]),
+            "window_halo": tile[3],
+            "sort_keys_stale": stale.value,
             "launches": launches.value,
             "avg_launch_ms": avg_launch_s * 1e3,
             "particle_steps_per_s": push_parts.value / (push_ms.value / 1e3),

@@ -205,6 +205,19 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
                              const vpic_grid_t *g, const int32_t *partition,
                              void *stream);
 
+/* vpic_hip_advance_p_async that also writes keys[k] = p[k].i as the push
+ * leaves it (k < np; -1 for a mover returned to the caller, whose index
+ * the caller rewrites), for a vpic_hip_sort_p_keys of these particles at
+ * the start of the next step (advance.cc:25-29 sorts there): the sort
+ * then reads 4 B per particle instead of the 32-B records.  No reference
+ * counterpart (the reference's push and sort share no state). */
+int vpic_hip_advance_p_keys(vpic_particle_t *p, int32_t np, float q, float m,
+                            vpic_particle_mover_t *pm, int32_t max_nm,
+                            int32_t *dev_nm, vpic_accumulator_t *a,
+                            const vpic_interpolator_t *f,
+                            const vpic_grid_t *g, const int32_t *partition,
+                            int32_t *keys, void *stream);
+
 /* vpic_hip_advance_p_async without a partition (the flat push), for
  * callers that cannot allocate device memory: code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc.  No
@@ -229,7 +242,7 @@ int vpic_hip_advance_p_noalloc(vpic_particle_t *p, int32_t np, float q,
                                const vpic_grid_t *g, void *stream);
 
 /* Tile shape (cells) of the partition-driven push; its LDS window holds
- * (tx+2)(ty+2)(tz+2) accumulators (a halo of one cell).  Default 8x8x8.
+ * (tx+2)(ty+2)(tz+2) accumulators (a halo of one cell).  Default 7x8x7.
  * No reference counterpart (the reference's pipelines split particles,
  * not cells). */
 int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
@@ -241,6 +254,10 @@ int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
  * beyond the order of the current sums.  No reference counterpart. */
 int vpic_hip_push_tile_window(int32_t tx, int32_t ty, int32_t tz,
                               int32_t halo);
+
+/* The tile shape and halo in use: out[0..3] = tx, ty, tz, halo.  No
+ * reference counterpart. */
+int vpic_hip_push_get_tile(int32_t *out);
 
 /* Wavefronts whose 64 particles fall in at least `runs` runs of equal
  * voxels deposit lane by lane instead of summing runs first (default 65,
@@ -283,6 +300,21 @@ int vpic_hip_move_p(vpic_particle_t *p, vpic_particle_mover_t *pm,
 int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int32_t np,
                     int32_t *partition, const vpic_grid_t *g, int in_place,
                     void *stream);
+
+/* vpic_hip_sort_p from the caller's copy of the voxel keys (keys[k] =
+ * p[k].i, k < np, as vpic_hip_advance_p_keys writes them).  The result is
+ * vpic_hip_sort_p's, whatever the keys hold: the sort compares every
+ * particle's voxel index with its key, and when any differs (or a key lies
+ * outside [0,nv)) it sorts again from the particles, leaving p as it was
+ * until then.  Synchronises the stream once (twice on a fallback).  No
+ * reference counterpart. */
+int vpic_hip_sort_p_keys(vpic_particle_t *p, vpic_particle_t *aux,
+                         int32_t np, int32_t *partition, const int32_t *keys,
+                         const vpic_grid_t *g, int in_place, void *stream);
+
+/* Number of vpic_hip_sort_p_keys calls in this process whose keys were not
+ * the particles' own (each then sorted from the particles). */
+int vpic_hip_sort_keys_stale(int64_t *count);
 
 /* The Takizuka-Abe binary collision operator: replaces
  * apply_takizuka_abe_pipeline (src/collision/pipeline/

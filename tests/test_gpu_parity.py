@@ -381,6 +381,66 @@ def test_sort_p_bitexact(dev, orc, mode):
     assert H.same_bits(dpart.download(), part_o)
 
 
+def _stale_count():
+    c = ctypes.c_int64()
+    _lib.call("vpic_hip_sort_keys_stale", ctypes.byref(c))
+    return c.value
+
+
+@pytest.mark.parametrize("flat", [False, True], ids=["tiled", "flat"])
+def test_push_keys_then_sort_p_keys(dev, orc, flat):
+    """The keys the push leaves (vpic_hip_advance_p_keys, tiled and flat,
+    with crossers) are the particles' voxel indices, and the sort from them
+    is vpic_hip_sort_p's, bit for bit; keys that are not the particles' own
+    (one changed, one outside the grid) give the same sort through the
+    fallback, which is counted."""
+    g = _box(7, 6, 5)
+    p, fi = _inputs(orc, g, 9, 0.3, 44, shuffle=True)
+    hg = host_grid(g)
+    part = orc.sort_p(p, g)
+    for _ in range(2):  # drift off the sort-time cells
+        orc.advance_p(p, -1.0, 1.0, H.acc_array(g), fi, hg)
+    po, ao = p.copy(), H.acc_array(g)
+    orc.advance_p(po, -1.0, 1.0, ao, fi, hg)
+    n = len(p)
+    dp = dev.DeviceArray.from_host(p)
+    dfi = dev.DeviceArray.from_host(fi)
+    da = dev.DeviceArray.from_host(H.acc_array(g))
+    dpm = dev.movers(n)
+    dnm = dev.DeviceArray((4,), np.int32)
+    dpart = dev.DeviceArray.from_host(part.astype(np.int32))
+    dkeys = dev.DeviceArray((n,), np.int32)
+    G = ctypes.byref(g)
+    say("advance_p_keys", "flat" if flat else "tiled", "np =", n)
+    _lib.call("vpic_hip_advance_p_keys", dp.ptr, n, -1.0, 1.0, dpm.ptr, n,
+              dnm.ptr, da.ptr, dfi.ptr, G, None if flat else dpart.ptr,
+              dkeys.ptr, None)
+    pg = dp.download()
+    assert H.same_bits(pg, po)
+    assert H.acc_rel_err(da.download(), ao) <= ACC_TOL
+    assert np.array_equal(dkeys.download(), po["i"])
+    assert np.any(po["i"] != p["i"])  # crossers exercised
+    ps = po.copy()
+    part_o = orc.sort_p(ps, g)
+    aux = dev.particles(n)
+    for case in ("keys", "one changed", "out of range"):
+        k = po["i"].copy()
+        if case == "one changed":
+            k[n // 2] = (k[n // 2] + 1) % g.nv
+        elif case == "out of range":
+            k[7] = g.nv + 5
+        dk = dev.DeviceArray.from_host(k.astype(np.int32))
+        dq = dev.DeviceArray.from_host(po)
+        dpt = dev.DeviceArray((g.nv + 1,), np.int32)
+        before = _stale_count()
+        say("sort_p_keys", case)
+        _lib.call("vpic_hip_sort_p_keys", dq.ptr, aux.ptr, n, dpt.ptr,
+                  dk.ptr, G, 1, None)
+        assert H.same_bits(dq.download(), ps), case
+        assert H.same_bits(dpt.download(), part_o), case
+        assert _stale_count() - before == (0 if case == "keys" else 1), case
+
+
 def test_energy_p_and_rho(dev, orc):
     g = _box(6, 5, 4)
     p, fi = _inputs(orc, g, 7, 0.1, 12)
@@ -443,13 +503,15 @@ def test_loader_statistics(dev):
 # ---------------------------------------------------------------------------
 # The partition-driven (tiled, LDS-accumulating) push
 # ---------------------------------------------------------------------------
-TILES = [(8, 8, 8), (8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
+TILES = [(7, 8, 7), (8, 8, 8), (8, 8, 4), (1, 1, 1), (16, 4, 4), (3, 2, 5)]
 
 
 @pytest.fixture
 def tile_reset():
+    was = (ctypes.c_int32 * 4)()
+    _lib.call("vpic_hip_push_get_tile", was)
     yield
-    _lib.call("vpic_hip_push_tile", 8, 8, 8)
+    _lib.call("vpic_hip_push_tile_window", *was)
     _lib.call("vpic_hip_push_group", 16)
 
 

@@ -32,13 +32,19 @@ _SIGS = {
                                  _vp, _pg, _vp, _vp],
     "vpic_hip_advance_p_noalloc": [_vp, _i32, _f32, _f32, _vp, _i32, _vp,
                                    _vp, _vp, _pg, _vp],
+    "vpic_hip_advance_p_keys": [_vp, _i32, _f32, _f32, _vp, _i32, _vp, _vp,
+                                _vp, _pg, _vp, _vp, _vp],
     "vpic_hip_push_tile": [_i32, _i32, _i32],
     "vpic_hip_push_tile_window": [_i32, _i32, _i32, _i32],
+    "vpic_hip_push_get_tile": [_vp],
     "vpic_hip_push_direct_runs": [_i32],
     "vpic_hip_push_group": [_i32],
     "vpic_hip_push_bin_mode": [_i32],
     "vpic_hip_move_p": [_vp, _vp, _i32, _vp, _pg, _f32, _vp, _vp],
     "vpic_hip_sort_p": [_vp, _vp, _i32, _vp, _pg, ctypes.c_int, _vp],
+    "vpic_hip_sort_p_keys": [_vp, _vp, _i32, _vp, _vp, _pg, ctypes.c_int,
+                             _vp],
+    "vpic_hip_sort_keys_stale": [_vp],
     "vpic_hip_takizuka_abe": [_vp, _vp, _i32, _f32, _f32, _vp, _vp, _i32,
                               _f32, _f32, ctypes.c_double, _i32,
                               ctypes.c_uint64, ctypes.c_uint64, _pg, _vp],

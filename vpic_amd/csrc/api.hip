@@ -205,7 +205,21 @@ extern "C" int vpic_hip_advance_p_async(
   if (rc) return rc;
   CHECK(dev_nm, "dev_nm is NULL");
   return launch_advance_p(p, np, q, m, pm, max_nm, dev_nm, a, f, g, partition,
-                          S(stream));
+                          nullptr, S(stream));
+}
+
+extern "C" int vpic_hip_advance_p_keys(
+    vpic_particle_t *p, int32_t np, float q, float m,
+    vpic_particle_mover_t *pm, int32_t max_nm, int32_t *dev_nm,
+    vpic_accumulator_t *a, const vpic_interpolator_t *f,
+    const vpic_grid_t *g, const int32_t *partition, int32_t *keys,
+    void *stream) {
+  int rc = check_push_args(p, np, pm, max_nm, a, f, g);
+  if (rc) return rc;
+  CHECK(dev_nm, "dev_nm is NULL");
+  CHECK(keys || np == 0, "keys is NULL");
+  return launch_advance_p(p, np, q, m, pm, max_nm, dev_nm, a, f, g, partition,
+                          keys, S(stream));
 }
 
 extern "C" int vpic_hip_advance_p_noalloc(
@@ -246,6 +260,14 @@ extern "C" int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz) {
   return VPIC_OK;
 }
 
+extern "C" int vpic_hip_push_get_tile(int32_t *out) {
+  CHECK(out, "out is NULL");
+  int t[4];
+  push_get_tile(t);
+  for (int k = 0; k < 4; k++) out[k] = t[k];
+  return VPIC_OK;
+}
+
 extern "C" int vpic_hip_push_tile_window(int32_t tx, int32_t ty, int32_t tz,
                                          int32_t halo) {
   CHECK(halo == 1 || halo == 2, "push window halo %d is not 1 or 2", halo);
@@ -270,7 +292,7 @@ extern "C" int vpic_hip_advance_p(vpic_particle_t *p, int32_t np, float q,
   DeviceScratch &ns = nm_scratch();
   if ((rc = scratch_reserve(ns, 64))) return rc;
   rc = launch_advance_p(p, np, q, m, pm, max_nm, (int *)ns.ptr, a, f, g,
-                        partition, S(stream));
+                        partition, nullptr, S(stream));
   if (rc) return rc;
   int32_t out[3];
   VPIC_HIP(hipMemcpyAsync(out, ns.ptr, sizeof(out), hipMemcpyDeviceToHost,
@@ -390,6 +412,26 @@ extern "C" int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux,
   CHECK(np == 0 || (p && aux && p != aux), "particle/aux arrays invalid");
   CHECK(aligned16(p) && aligned16(aux), "arrays must be 16-byte aligned");
   return launch_sort_p(p, aux, np, partition, g, in_place, S(stream));
+}
+
+extern "C" int vpic_hip_sort_p_keys(vpic_particle_t *p, vpic_particle_t *aux,
+                                    int32_t np, int32_t *partition,
+                                    const int32_t *keys, const vpic_grid_t *g,
+                                    int in_place, void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(np >= 0, "np < 0");
+  CHECK(partition, "partition is NULL");
+  CHECK(np == 0 || (p && aux && p != aux), "particle/aux arrays invalid");
+  CHECK(aligned16(p) && aligned16(aux), "arrays must be 16-byte aligned");
+  return launch_sort_p_keys(p, aux, np, partition, keys, g, in_place,
+                            S(stream));
+}
+
+extern "C" int vpic_hip_sort_keys_stale(int64_t *count) {
+  CHECK(count, "count is NULL");
+  *count = sort_keys_stale_count();
+  return VPIC_OK;
 }
 
 extern "C" int vpic_hip_energy_p(const vpic_particle_t *p, int32_t np,

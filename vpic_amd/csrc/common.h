@@ -216,10 +216,12 @@ struct DeviceScratch {
 int scratch_reserve(DeviceScratch &s, size_t bytes);
 
 // Launch-side kernels entry points implemented in the .hip files.
+// keys: when not null, keys[k] = p[k].i as the push leaves it, for a
+// following launch_sort_p_keys (-1 for a mover returned to the caller).
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
-                     const vpic_grid_t *g, const int *partition,
+                     const vpic_grid_t *g, const int *partition, int *keys,
                      hipStream_t s);
 // The flat push on the library's fixed per-device scratch (no allocation):
 // vpic_hip_advance_p_noalloc.  push_noalloc_setup looks up that scratch's
@@ -231,6 +233,7 @@ int launch_advance_p_noalloc(vpic_particle_t *p, int np, float q, float m,
                              const vpic_grid_t *g, hipStream_t s);
 int push_noalloc_setup();
 int push_set_tile(int tx, int ty, int tz, int halo);
+void push_get_tile(int out[4]);
 int push_set_direct_runs(int runs);
 int push_set_group(int gs);
 int push_set_bin_mode(int mode);
@@ -264,6 +267,13 @@ int launch_inject_p(vpic_particle_t *p, int *np, int max_np,
 int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
                   int *partition, const vpic_grid_t *g, int in_place,
                   hipStream_t s);
+// sort_p from the caller's copy of the voxel keys, checked against the
+// particles (any difference: sorted from the particles instead, and
+// counted in sort_keys_stale_count).
+int launch_sort_p_keys(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                       int *partition, const int *keys, const vpic_grid_t *g,
+                       int in_place, hipStream_t s);
+long long sort_keys_stale_count();
 int launch_energy_p(const vpic_particle_t *p, int np, float q, float m,
                     const vpic_interpolator_t *f, const vpic_grid_t *g,
                     double *host_en, hipStream_t s);

@@ -293,7 +293,8 @@ struct WindowSink {
 template <class Sink>
 __device__ int dev_move_p(vpic_particle_t *__restrict__ p0,
                           vpic_particle_mover_t &mv, const Sink &sink,
-                          const GridDev &G, float qsp) {
+                          const GridDev &G, float qsp,
+                          int *__restrict__ vkey = nullptr) {
   float4 *pp = reinterpret_cast<float4 *>(p0 + mv.i);
   float4 A = pp[0], B = pp[1];
   float pd[3] = {A.x, A.y, A.z};
@@ -355,6 +356,9 @@ __device__ int dev_move_p(vpic_particle_t *__restrict__ p0,
   }
   pp[0] = make_float4(pd[0], pd[1], pd[2], __int_as_float(pi));
   pp[1] = make_float4(pu[0], pu[1], pu[2], B.w);
+  // the voxel a sort would key this particle by (-1 for a returned mover,
+  // whose index the caller rewrites)
+  if (vkey) *vkey = ret ? -1 : pi;
   mv.dispx = md[0];
   mv.dispy = md[1];
   mv.dispz = md[2];
@@ -886,7 +890,7 @@ __global__ __launch_bounds__(256) void k_advance_p(
     const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
     vpic_particle_mover_t *__restrict__ xlist, int xcap,
     vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt,
-    PushCoef C, GridDev G) {
+    PushCoef C, GridDev G, int *__restrict__ vkeys) {
   const int n0 = start ? min(max(*start, 0), np) : 0;
   const int lane = lane_id();
   const GlobalSink sink{acc};
@@ -902,6 +906,8 @@ __global__ __launch_bounds__(256) void k_advance_p(
     PushResult R;
     push_compute(A, B, valid, p, n, f0, C, G.nv, cnt + 3, R);
     wave_deposit(R.active ? R.voxel : -1, R.j, sink);
+    // a particle that stays in its cell keeps its voxel index
+    if (vkeys && valid && !R.cross) vkeys[n] = __float_as_int(A.w);
 
     // Cell crossers: compact into the crosser list (ballot + one atomic per
     // wavefront); the rare overflow of the list is moved right here.
@@ -920,7 +926,8 @@ __global__ __launch_bounds__(256) void k_advance_p(
         mv.i = n;
         if (slot < xcap) {
           xlist[slot] = mv;
-        } else if (dev_move_p(p, mv, sink, G, C.qsp)) {
+        } else if (dev_move_p(p, mv, sink, G, C.qsp,
+                              vkeys ? vkeys + n : nullptr)) {
           push_returned_mover(p, mv, rlist, xcap, cnt);
         }
       }
@@ -933,13 +940,14 @@ __global__ __launch_bounds__(256) void k_advance_p(
 __global__ __launch_bounds__(256) void k_move_p_list(
     vpic_particle_t *__restrict__ p, const vpic_particle_mover_t *xlist,
     int xcap, vpic_particle_mover_t *__restrict__ rlist,
-    float *__restrict__ acc, int *__restrict__ cnt, float qsp, GridDev G) {
+    float *__restrict__ acc, int *__restrict__ cnt, float qsp, GridDev G,
+    int *__restrict__ vkeys) {
   const int n = min(cnt[0], xcap);
   const GlobalSink sink{acc};
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += gridDim.x * blockDim.x) {
     vpic_particle_mover_t mv = xlist[k];
-    if (dev_move_p(p, mv, sink, G, qsp))
+    if (dev_move_p(p, mv, sink, G, qsp, vkeys ? vkeys + mv.i : nullptr))
       push_returned_mover(p, mv, rlist, xcap, cnt);
   }
 }
@@ -1072,12 +1080,15 @@ struct BinArgs {
   unsigned char *key; // per particle offset code (written); null: no binning
   int *counts;        // per tile 27 code counts (written)
   int *oow;           // out-of-window deposits (accumulated)
+  int *vkeys;         // per particle voxel index after the push, for a
+                      // following sort_p (written); null: not asked for
 };
 
 struct BinSink {
   unsigned char *key;  // null: no binning
   int *cnt;            // LDS code counts
   TileBox box;
+  int *vkeys;          // as BinArgs::vkeys
 };
 
 __device__ __forceinline__ void drain_wave_queue(
@@ -1096,6 +1107,7 @@ __device__ __forceinline__ void drain_wave_queue(
     int vout;
     const int ret = dev_move_p_reg(p, A, B, mv, W, G, VD, qsp, vout);
     if (ret) push_returned_mover(p, mv, rlist, rcap, cnt_g);
+    if (BS.vkeys) BS.vkeys[n] = ret ? -1 : vout;
     if (BS.key) {
       int o = 13;
       if (!ret) {
@@ -1211,6 +1223,7 @@ __device__ __forceinline__ void tiled_push(
   BinSink BS;
   BS.key = keyed ? Bn.key : nullptr;
   BS.cnt = bin_cnt;
+  BS.vkeys = Bn.vkeys;
   TileBox &TB = BS.box;
   TB.x0 = x0;
   TB.x1 = x1;
@@ -1439,6 +1452,9 @@ __device__ __forceinline__ void tiled_push(
                    valid ? reinterpret_cast<float4 *>(p + n)
                          : reinterpret_cast<float4 *>(trash + t),
                    R);
+    // a particle that stays in its cell keeps its voxel index (a crosser's
+    // is written when it is moved)
+    if (BS.vkeys && valid && !R.cross) BS.vkeys[n] = iraw;
     if (keyed) {
       // the tile holding the particle now (a crosser's is set when it is
       // moved)
@@ -1718,7 +1734,8 @@ __global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_ens_push_tiled(
     int nitems, int *__restrict__ queue, vpic_particle_t *__restrict__ trash,
     vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt) {
   __shared__ int s_item;
-  const BinArgs Bn = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const BinArgs Bn = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr};
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
     __syncthreads();
@@ -2074,7 +2091,11 @@ int pow2_at_least(int64_t n) {
   return (int)p;
 }
 
-TileGeom g_tile = {8, 8, 8, 0, 0, 0, 0, 0, 0, 1};
+// 7x8x7 cells (9x10x9 window cells and their coefficients in LDS): the
+// fastest of the shapes timed on the 256x128x128 thermal plasma over whole
+// sort intervals (8x8x8, whose coefficients fit for the tile only, took 6%
+// longer per push)
+TileGeom g_tile = {7, 8, 7, 0, 0, 0, 0, 0, 0, 1};
 int g_group = 16;  // lanes sharing one run of particles in the tiled push
 int g_bin_mode = 0;  // tile lists: 0 off, 1 when drift warrants, 2 always
 
@@ -2157,6 +2178,13 @@ int push_set_tile(int tx, int ty, int tz, int halo) {
   return VPIC_OK;
 }
 
+void push_get_tile(int out[4]) {
+  out[0] = g_tile.tx;
+  out[1] = g_tile.ty;
+  out[2] = g_tile.tz;
+  out[3] = g_tile.halo;
+}
+
 int push_set_group(int gs) {
   if (gs < 1 || gs > 64 || (gs & (gs - 1))) return VPIC_EINVAL;
   g_group = gs;
@@ -2226,12 +2254,13 @@ int launch_flat_push(vpic_particle_t *p, int np, float q, const PushCoef &C,
     const int blocks = (int)(want < 16384 ? want : 16384);
     hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p,
                        (const int *)nullptr, np, f, (float *)a, xlist, xcap,
-                       rlist, cnt, C, G);
+                       rlist, cnt, C, G, (int *)nullptr);
     VPIC_LAUNCHED("k_advance_p");
     int mblocks = (xcap + 255) / 256;
     if (mblocks > 8192) mblocks = 8192;
     hipLaunchKernelGGL(k_move_p_list, dim3(mblocks), dim3(256), 0, s, p,
-                       xlist, xcap, rlist, (float *)a, cnt, q, G);
+                       xlist, xcap, rlist, (float *)a, cnt, q, G,
+                       (int *)nullptr);
     VPIC_LAUNCHED("k_move_p_list");
   }
   hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
@@ -2312,7 +2341,7 @@ int launch_advance_p_noalloc(vpic_particle_t *p, int np, float q, float m,
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_particle_mover_t *pm, int max_nm, int *dev_nm,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
-                     const vpic_grid_t *g, const int *partition,
+                     const vpic_grid_t *g, const int *partition, int *keys,
                      hipStream_t s) {
   PushScratch &S = push_scratch();
   const int xcap = crosser_capacity(np);
@@ -2388,7 +2417,8 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         attr_set[ak] = true;
       }
       // Tile lists from the codes of the last tiled push of this array.
-      BinArgs BA = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+      BinArgs BA = {nullptr, nullptr, nullptr, nullptr,
+                    nullptr, nullptr, keys};
       BinState *BSt = nullptr;
       if (g_bin_mode > 0) {
         BinState &B = bin_state(S, p);
@@ -2479,14 +2509,14 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       if (timed && !partition) push_timing_record(s, np, true);
       hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p,
                          flat_start, np, f, (float *)a, xlist, xcap, rlist,
-                         cnt, C, G);
+                         cnt, C, G, keys);
       VPIC_LAUNCHED("k_advance_p");
       if (timed && !partition) push_timing_record(s, np, false);
     }
     int mblocks = (xcap + 255) / 256;
     if (mblocks > 8192) mblocks = 8192;
     hipLaunchKernelGGL(k_move_p_list, dim3(mblocks), dim3(256), 0, s, p,
-                       xlist, xcap, rlist, (float *)a, cnt, q, G);
+                       xlist, xcap, rlist, (float *)a, cnt, q, G, keys);
     VPIC_LAUNCHED("k_move_p_list");
   }
   hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,

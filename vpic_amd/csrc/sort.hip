@@ -26,6 +26,8 @@
 // counting sort over packed arrays, no order within a voxel.
 #include "common.h"
 
+#include <atomic>
+
 namespace {
 
 __device__ __forceinline__ void run_info(int key, int lane, int &seg_start,
@@ -272,9 +274,11 @@ __device__ __forceinline__ int axis_digit(int v, int axis, const SortDig &D) {
   return (int)z;
 }
 
-// Per block, the count of each digit: hist[d * nblocks + block].  The first
-// pass reads the voxel indices from the particles and keeps them in keys.
-template <bool FIRST>
+// Per block, the count of each digit: hist[d * nblocks + block].  SRC 0:
+// the voxel indices read from the particles (checked, kept in keys_out);
+// 1: keys_in; 2: keys_in as given by the caller (checked like SRC 0, an
+// index outside [0, nv) counted in nbad and taken as 0).
+template <int SRC>
 __global__ __launch_bounds__(64) void k_rs_hist(
     const vpic_particle_t *__restrict__ p, const int *__restrict__ keys_in,
     int *__restrict__ keys_out, int np, SortDig D, int axis, int nbins,
@@ -286,25 +290,34 @@ __global__ __launch_bounds__(64) void k_rs_hist(
   __syncthreads();
   const int base = b * RS_BLK, end = min(base + RS_BLK, np);
   int bad = 0;
-  constexpr int U = 4;  // rounds whose loads are in flight together
-  for (int i0 = base; i0 < end; i0 += 64 * U) {
-    int kk[U];
+  // U rounds of 64 keys per batch; the next batch is loaded while this one
+  // is counted
+  constexpr int U = 4;
+  int kk[U];
+  // (indices clamped into the block, so every load is issued and the
+  // wait for a batch does not wait for the next)
+  auto fetch = [&](int i0, int (&k)[U]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int i = i0 + u * 64 + lane;
-      kk[u] = i < end ? (FIRST ? p[i].i : keys_in[i]) : 0;
+      const int i = min(i0 + u * 64 + lane, end - 1);
+      k[u] = SRC == 0 ? p[i].i : keys_in[i];
     }
+  };
+  fetch(base, kk);
+  for (int i0 = base; i0 < end; i0 += 64 * U) {
+    int kn[U];
+    fetch(i0 + 64 * U, kn);
 #pragma unroll
     for (int u = 0; u < U; u++) {
     const int i = i0 + u * 64 + lane;
     const bool valid = i < end;
     int key = kk[u];
-    if (valid && FIRST) {
+    if (valid && SRC != 1) {
       if ((unsigned)key >= (unsigned)D.nv) {
         bad++;  // never used as an index; the sort reports it
         key = 0;
       }
-      keys_out[i] = key;
+      if (SRC == 0) keys_out[i] = key;
     }
     const int d = valid ? axis_digit(key, axis, D) : -1;
     // runs of equal digits over consecutive lanes: the last lane of a run
@@ -318,8 +331,10 @@ __global__ __launch_bounds__(64) void k_rs_hist(
     const int h0 = 63 - __clzll(heads & upto);
     if (tail && d >= 0) atomicAdd(&h[d], lane - h0 + 1);
     }
+#pragma unroll
+    for (int u = 0; u < U; u++) kk[u] = kn[u];
   }
-  if (FIRST && bad) atomicAdd(nbad, bad);
+  if (SRC != 1 && bad) atomicAdd(nbad, bad);
   __syncthreads();
   for (int d = lane; d < nbins; d += 64) hist[(int64_t)d * nblocks + b] = h[d];
 }
@@ -338,20 +353,32 @@ __global__ __launch_bounds__(64) void k_rs_scatter(
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
   const int base = b * RS_BLK, end = min(base + RS_BLK, np);
-  constexpr int U = 4;  // rounds whose loads are in flight together
-  for (int i0 = base; i0 < end; i0 += 64 * U) {
-    int kk[U], ii[U];
+  // U rounds of 64 elements per batch; the next batch is loaded while this
+  // one is placed
+  constexpr int U = 4;
+  int kk[U], ii[U];
+  // (indices clamped into the block, so every load is issued and the
+  // wait for a batch does not wait for the next)
+  auto fetch = [&](int i0, int (&k)[U], int (&x)[U]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int i = i0 + u * 64 + lane;
-      kk[u] = i < end ? keys_in[i] : 0;
-      ii[u] = FIRST ? i : (i < end ? idx_in[i] : 0);
+      const int i = min(i0 + u * 64 + lane, end - 1);
+      k[u] = keys_in[i];
+      x[u] = FIRST ? i : idx_in[i];
     }
+  };
+  fetch(base, kk, ii);
+  for (int i0 = base; i0 < end; i0 += 64 * U) {
+    int kn[U], xn[U];
+    fetch(i0 + 64 * U, kn, xn);
 #pragma unroll
     for (int u = 0; u < U; u++) {
     const int i = i0 + u * 64 + lane;
     const bool valid = i < end;
-    const int key = kk[u];
+    // the first pass may read a caller's keys: one outside [0, nv) is
+    // counted by the histogram pass and sorted as 0 here
+    const int key =
+        FIRST ? ((unsigned)kk[u] < (unsigned)D.nv ? kk[u] : 0) : kk[u];
     const int idx = ii[u];
     const int d = valid ? axis_digit(key, axis, D) : 0;
     // lanes holding the same digit
@@ -374,37 +401,63 @@ __global__ __launch_bounds__(64) void k_rs_scatter(
       idx_out[p0 + rank] = idx;
     }
     }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      kk[u] = kn[u];
+      ii[u] = xn[u];
+    }
   }
 }
 
-// out[j] = p[perm[j]] for the 2 x 256 slots of a block, each particle
+// out[j] = p[perm[j]] for the 128 * GU slots of a block, each particle
 // moved by a pair of lanes (16 B each, so every load and store instruction
-// covers consecutive 16-B pieces; four in flight per lane), and
+// covers consecutive 16-B pieces), GU particles in flight per lane, and
 // partition[v] = first slot whose key is >= v (partition[nv] = np), from
-// the sorted keys (written by the even lane of the pair).
+// the sorted keys (written by the even lane of the pair).  With `stale`
+// given, every particle's voxel index is compared with the key it was
+// sorted by, and the differences are counted there.  np >= 1.
+constexpr int GATHER_GU = 4;
+typedef float gather_f4 __attribute__((ext_vector_type(4)));
+
+template <int GU>
 __global__ __launch_bounds__(256) void k_gather_part(
     const vpic_particle_t *__restrict__ src, const int *__restrict__ perm,
     const int *__restrict__ keys, int np, int nv,
-    vpic_particle_t *__restrict__ dst, int *__restrict__ part) {
+    vpic_particle_t *__restrict__ dst, int *__restrict__ part,
+    int *__restrict__ stale) {
   const int half = threadIdx.x & 1;
-  const int64_t j0 = 512 * (int64_t)blockIdx.x + (threadIdx.x >> 1);
-  float4 a[4];
-  int kc[4], kp[4];
+  const int64_t j0 = (int64_t)blockIdx.x * (128 * GU) + (threadIdx.x >> 1);
+  // indices clamped into the arrays, so that every load is issued at once
+  int q[GU], kc[GU], kp[GU];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
+  for (int u = 0; u < GU; u++) {
     const int64_t j = j0 + 128 * u;
-    const int q = j < np ? perm[j] : 0;
-    kc[u] = j < np ? keys[j] : nv;
-    kp[u] = j == 0 ? -1 : (j <= np ? keys[j - 1] : nv);
-    a[u] = reinterpret_cast<const float4 *>(src + q)[half];
+    const int jc = (int)(j < np ? j : np - 1);
+    const int jp = (int)(j - 1 < np ? (j > 0 ? j - 1 : 0) : np - 1);
+    q[u] = perm[jc];
+    kc[u] = keys[jc];
+    kp[u] = keys[jp];
   }
+  gather_f4 a[GU];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
+  for (int u = 0; u < GU; u++)
+    a[u] = reinterpret_cast<const gather_f4 *>(src + q[u])[half];
+  // every particle load issued before the first store waits for one (the
+  // compiler would otherwise move each load down to its store)
+#pragma unroll
+  for (int u = 0; u < GU; u++) asm volatile("" : "+v"(a[u]));
+  int nstale = 0;
+#pragma unroll
+  for (int u = 0; u < GU; u++) {
     const int64_t j = j0 + 128 * u;
-    if (j < np) reinterpret_cast<float4 *>(dst + j)[half] = a[u];
+    const int kcu = j < np ? kc[u] : nv;
+    const int kpu = j == 0 ? -1 : (j <= np ? kp[u] : nv);
+    if (j < np) reinterpret_cast<gather_f4 *>(dst + j)[half] = a[u];
+    if (stale && !half && j < np) nstale += __float_as_int(a[u][3]) != kcu;
     if (!half && j <= np)
-      for (int v = kp[u] + 1; v <= kc[u] && v <= nv; v++) part[v] = (int)j;
+      for (int v = kpu + 1; v <= kcu && v <= nv; v++) part[v] = (int)j;
   }
+  if (nstale) atomicAdd(stale, nstale);
 }
 
 struct SortScratch {
@@ -444,10 +497,21 @@ int nbits_for(int nbins) {
   return b;
 }
 
-// The axis-pass sort (grids up to RS_MAXB - 2 cells per axis).
+// sort_p_axes' answer when the caller's keys were not the particles' own
+// (nothing of p was changed; the caller sorts again without them).
+constexpr int SORT_KEYS_STALE = 1;
+
+std::atomic<long long> g_sort_keys_stale{0};
+void note_sort_keys_stale() { g_sort_keys_stale.fetch_add(1); }
+
+// The axis-pass sort (grids up to RS_MAXB - 2 cells per axis).  With
+// `hint` (the caller's copy of p[k].i, k < np) the first pass reads the
+// keys from it instead of from the particles, and the gather checks every
+// particle's voxel index against the key it was sorted by; a difference,
+// or a key outside [0, nv), returns SORT_KEYS_STALE before p is touched.
 int sort_p_axes(vpic_particle_t *p, vpic_particle_t *aux, int np,
                 int *partition, const vpic_grid_t *g, int in_place,
-                hipStream_t s) {
+                const int *hint, hipStream_t s) {
   SortScratch &S = sort_scratch();
   const int nv = g->nv;
   const int nblocks = blocks_for(np, RS_BLK);
@@ -473,33 +537,44 @@ int sort_p_axes(vpic_particle_t *p, vpic_particle_t *aux, int np,
   D.sx1 = (unsigned)(g->nx + 2);
   D.sy1 = (unsigned)(g->ny + 2);
   D.nv = nv;
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, nbad, (int64_t)1, 0);
+  // nbad[0]: keys outside [0, nv); nbad[1]: particles whose voxel index is
+  // not the caller's key
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, nbad, (int64_t)2, 0);
   VPIC_LAUNCHED("k_fill_int");
-  // pass x: keys from the particles into kb, then (kb, index) -> (ka, ia)
-  hipLaunchKernelGGL(k_rs_hist<true>, dim3(nblocks), dim3(64), 0, s, p,
-                     (const int *)nullptr, kb, np, D, 0, bins[0], hist,
-                     nblocks, nbad);
+  // pass x: keys from the particles into kb (or the caller's keys), then
+  // (keys, index) -> (ka, ia)
+  if (hint) {
+    hipLaunchKernelGGL(k_rs_hist<2>, dim3(nblocks), dim3(64), 0, s,
+                       (const vpic_particle_t *)nullptr, hint, (int *)nullptr,
+                       np, D, 0, bins[0], hist, nblocks, nbad);
+  } else {
+    hipLaunchKernelGGL(k_rs_hist<0>, dim3(nblocks), dim3(64), 0, s, p,
+                       (const int *)nullptr, kb, np, D, 0, bins[0], hist,
+                       nblocks, nbad);
+    VPIC_LAUNCHED("k_rs_hist");
+    int h_nbad = 0;
+    VPIC_HIP(hipMemcpyAsync(&h_nbad, nbad, sizeof(int), hipMemcpyDeviceToHost,
+                            s));
+    VPIC_HIP(hipStreamSynchronize(s));
+    if (h_nbad > 0)
+      return vpic_set_error(VPIC_EINVAL,
+                            "sort_p: %d particles have a voxel index outside "
+                            "[0,nv); particles left unsorted", h_nbad);
+  }
   VPIC_LAUNCHED("k_rs_hist");
-  int h_nbad = 0;
-  VPIC_HIP(hipMemcpyAsync(&h_nbad, nbad, sizeof(int), hipMemcpyDeviceToHost,
-                          s));
-  VPIC_HIP(hipStreamSynchronize(s));
-  if (h_nbad > 0)
-    return vpic_set_error(VPIC_EINVAL,
-                          "sort_p: %d particles have a voxel index outside "
-                          "[0,nv); particles left unsorted", h_nbad);
   if ((rc = device_exclusive_scan(hist, hist, (int64_t)bins[0] * nblocks,
                                   sums, s)))
     return rc;
   hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblocks), dim3(64), 0, s,
-                     (const int *)kb, (const int *)nullptr, np, D, 0, bins[0],
-                     nbits_for(bins[0]), (const int *)hist, nblocks, ka, ia);
+                     hint ? hint : (const int *)kb, (const int *)nullptr, np,
+                     D, 0, bins[0], nbits_for(bins[0]), (const int *)hist,
+                     nblocks, ka, ia);
   VPIC_LAUNCHED("k_rs_scatter");
   // passes y and z: (ka, ia) -> (kb, ib) -> (ka, ia)
   for (int axis = 1; axis < 3; axis++) {
     const int *kin = axis == 1 ? ka : kb, *iin = axis == 1 ? ia : ib;
     int *kout = axis == 1 ? kb : ka, *iout = axis == 1 ? ib : ia;
-    hipLaunchKernelGGL(k_rs_hist<false>, dim3(nblocks), dim3(64), 0, s,
+    hipLaunchKernelGGL(k_rs_hist<1>, dim3(nblocks), dim3(64), 0, s,
                        (const vpic_particle_t *)nullptr, kin, (int *)nullptr,
                        np, D, axis, bins[axis], hist, nblocks, nbad);
     VPIC_LAUNCHED("k_rs_hist");
@@ -511,10 +586,17 @@ int sort_p_axes(vpic_particle_t *p, vpic_particle_t *aux, int np,
                        (const int *)hist, nblocks, kout, iout);
     VPIC_LAUNCHED("k_rs_scatter");
   }
-  hipLaunchKernelGGL(k_gather_part, dim3(blocks_for((int64_t)np + 1, 512)),
+  hipLaunchKernelGGL(k_gather_part<GATHER_GU>,
+                     dim3(blocks_for((int64_t)np + 1, 128 * GATHER_GU)),
                      dim3(256), 0, s, p, (const int *)ia, (const int *)ka, np,
-                     nv, aux, partition);
+                     nv, aux, partition, hint ? nbad + 1 : (int *)nullptr);
   VPIC_LAUNCHED("k_gather_part");
+  if (hint) {
+    int h[2] = {0, 0};
+    VPIC_HIP(hipMemcpyAsync(h, nbad, sizeof(h), hipMemcpyDeviceToHost, s));
+    VPIC_HIP(hipStreamSynchronize(s));
+    if (h[0] > 0 || h[1] > 0) return SORT_KEYS_STALE;
+  }
   if (in_place)
     VPIC_HIP(hipMemcpyAsync(p, aux, sizeof(vpic_particle_t) * (size_t)np,
                             hipMemcpyDeviceToDevice, s));
@@ -747,7 +829,27 @@ int launch_sort_p(vpic_particle_t *p, vpic_particle_t *aux, int np,
   push_invalidate_bins(aux);
   if (np > 0 && g->nx + 2 <= RS_MAXB && g->ny + 2 <= RS_MAXB &&
       g->nz + 2 <= RS_MAXB)
-    return sort_p_axes(p, aux, np, partition, g, in_place, s);
+    return sort_p_axes(p, aux, np, partition, g, in_place, nullptr, s);
   return sort_p_counting(p, aux, np, partition, g, in_place, s);
 }
+
+// sort_p with the caller's copy of the voxel keys (keys[k] = p[k].i, e.g.
+// written by the push before): the same result, from one pass fewer over
+// the particle records when the keys are right, and from the particles
+// when any is not.
+int launch_sort_p_keys(vpic_particle_t *p, vpic_particle_t *aux, int np,
+                       int *partition, const int *keys, const vpic_grid_t *g,
+                       int in_place, hipStream_t s) {
+  if (!keys || !(np > 0 && g->nx + 2 <= RS_MAXB && g->ny + 2 <= RS_MAXB &&
+                 g->nz + 2 <= RS_MAXB))
+    return launch_sort_p(p, aux, np, partition, g, in_place, s);
+  push_invalidate_bins(p);
+  push_invalidate_bins(aux);
+  const int rc = sort_p_axes(p, aux, np, partition, g, in_place, keys, s);
+  if (rc != SORT_KEYS_STALE) return rc;
+  note_sort_keys_stale();
+  return sort_p_axes(p, aux, np, partition, g, in_place, nullptr, s);
+}
+
+long long sort_keys_stale_count() { return g_sort_keys_stale.load(); }
 

@@ -44,6 +44,10 @@ class Species:
         # buffer per sort); allocated by the first sort, then p and aux
         # swap roles at every sort, so both always hold max_np records
         self.aux = None
+        # voxel keys written by the push of the step before a sort
+        # (allocated by the first such push) and that step
+        self.keys = None
+        self.keys_step = -1
 
 
 class Simulation:
@@ -85,8 +89,16 @@ class Simulation:
             raise _lib.VpicError(
                 "sort_p: species %s has %d particles, its buffers hold %d "
                 "and %d" % (sp.name, sp.np, sp.p.shape[0], sp.aux.shape[0]))
-        _lib.call("vpic_hip_sort_p", sp.p.ptr, sp.aux.ptr, sp.np,
-                  sp.partition.ptr, self._G(), 0, self.stream)
+        if sp.keys is not None and sp.keys_step == self.step - 1:
+            # the keys the last push wrote (checked against the particles
+            # by the sort, which falls back to reading them on a mismatch)
+            _lib.call("vpic_hip_sort_p_keys", sp.p.ptr, sp.aux.ptr, sp.np,
+                      sp.partition.ptr, sp.keys.ptr, self._G(), 0,
+                      self.stream)
+        else:
+            _lib.call("vpic_hip_sort_p", sp.p.ptr, sp.aux.ptr, sp.np,
+                      sp.partition.ptr, self._G(), 0, self.stream)
+        sp.keys_step = -1
         sp.p, sp.aux = sp.aux, sp.p
         sp.last_sorted = self.step
 
@@ -120,9 +132,22 @@ class Simulation:
     def advance_p(self, sp):
         # after a sort the partition drives the tiled (LDS) push
         part = sp.partition.ptr if sp.last_sorted >= 0 else None
-        _lib.call("vpic_hip_advance_p_async", sp.p.ptr, sp.np, sp.q, sp.m,
-                  sp.pm.ptr, sp.max_nm, sp.dev_nm.ptr, self.acc.ptr,
-                  self.interp.ptr, self._G(), part, self.stream)
+        if sp.sort_interval > 0 and (self.step + 1) % sp.sort_interval == 0:
+            # the next step starts with this species' sort_p
+            # (advance.cc:25-29): the push leaves it the voxel keys
+            if sp.keys is None or sp.keys.shape[0] < sp.max_np:
+                sp.keys = device.DeviceArray((max(sp.max_np, 1),), np.int32)
+            _lib.call("vpic_hip_advance_p_keys", sp.p.ptr, sp.np, sp.q,
+                      sp.m, sp.pm.ptr, sp.max_nm, sp.dev_nm.ptr,
+                      self.acc.ptr, self.interp.ptr, self._G(), part,
+                      sp.keys.ptr, self.stream)
+            sp.keys_step = self.step
+        else:
+            _lib.call("vpic_hip_advance_p_async", sp.p.ptr, sp.np, sp.q,
+                      sp.m, sp.pm.ptr, sp.max_nm, sp.dev_nm.ptr,
+                      self.acc.ptr, self.interp.ptr, self._G(), part,
+                      self.stream)
+            sp.keys_step = -1
 
     def load_interpolator_array(self):
         _lib.call("vpic_hip_load_interpolator_array", self.interp.ptr,
@@ -243,6 +268,7 @@ class Simulation:
             sp.nm = nm
             if nm == 0:
                 continue
+            sp.keys_step = -1  # particles are removed and moved below
             npc = ctypes.c_int32(sp.np)
             counts = (ctypes.c_int32 * 2)()
             _lib.call("vpic_hip_boundary_p", sp.p.ptr, ctypes.byref(npc),
