@@ -249,9 +249,12 @@ int vpic_hip_push_tile(int32_t tx, int32_t ty, int32_t tz);
 
 /* The same with the window's halo: 1 or 2 cells, (tx+2h)(ty+2h)(tz+2h)
  * accumulators of 96 B, which with 72 B per tile cell must fit 140 KiB
- * of LDS.  A wider halo keeps the deposits of particles that drifted
- * further since the last sort_p in LDS.  Results do not depend on it
- * beyond the order of the current sums.  No reference counterpart. */
+ * of LDS.  Along an axis where the grid is one cell thick the tile is one
+ * cell and the window has no halo (a 2D run's tiles can be 32x16x1); a
+ * push on a grid where the window does not fit is an error.  A wider halo
+ * keeps the deposits of particles that drifted further since the last
+ * sort_p in LDS.  Results do not depend on it beyond the order of the
+ * current sums.  No reference counterpart. */
 int vpic_hip_push_tile_window(int32_t tx, int32_t ty, int32_t tz,
                               int32_t halo);
 

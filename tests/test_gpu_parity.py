@@ -595,6 +595,35 @@ def test_tiled_push_halo2_parity(dev, orc, win, tile_reset):
         assert H.acc_rel_err(ag, ao) <= ACC_TOL
 
 
+# grids one cell thick along an axis (2D runs): the tile is clamped to the
+# grid and its window has no halo along that axis
+THIN = [((9, 7, 1), (8, 8, 8, 1)), ((9, 1, 6), (4, 8, 4, 2)),
+        ((1, 6, 5), (8, 3, 2, 1)), ((10, 9, 1), (32, 16, 1, 1))]
+
+
+@pytest.mark.parametrize("n,win", THIN,
+                         ids=["%dx%dx%d" % t[0] for t in THIN])
+def test_tiled_push_thin_grid(dev, orc, n, win, tile_reset):
+    """A 2D grid (one cell along x, y or z): every deposit stays in the
+    plane, with particles drifted off their sort-time cells."""
+    _lib.call("vpic_hip_push_tile_window", *win)
+    g = _box(*n)
+    for (q, m, uth, steps) in ((-1.0, 1.0, 0.3, 3), (-1.0, 1.0, 0.8, 4)):
+        p, fi = _inputs(orc, g, 13, uth, 35, shuffle=True)
+        hg = host_grid(g)
+        part = orc.sort_p(p, g)
+        for _ in range(steps):
+            orc.advance_p(p, q, m, H.acc_array(g), fi, hg)
+        po, ao = p.copy(), H.acc_array(g)
+        orc.advance_p(po, q, m, ao, fi, hg)
+        say("tiled advance_p thin grid", n, "window", win, "uth", uth)
+        pg, ag, nmg, igg, _ = _gpu_advance_p(dev, p, q, m, fi, g,
+                                             partition=part)
+        assert H.same_bits(pg, po)
+        assert (nmg, igg) == (0, 0)
+        assert H.acc_rel_err(ag, ao) <= ACC_TOL
+
+
 def test_tiled_push_multistep_device_sort(dev, orc):
     """sort_p and several tiled pushes entirely on the device (the partition
     goes stale as the particles drift), each step against the oracle."""

@@ -579,8 +579,11 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
       {64, 1024, 128, 1024, 16},  {256, 32768, 256, 16384, 1},
       {128, 4096, 128, 2048, 8},  {256, 8192, 512, 8192, 2},
   };
-  static const TileShape tiles[] = {{16, 16, 1}, {32, 8, 1}, {16, 8, 1},
-                                    {8, 8, 1},   {16, 8, 4}, {8, 8, 8}};
+  // (a box one cell thick along an axis has no window halo along it, so
+  // the flat shapes' windows are the tile and a one-cell ring in-plane)
+  static const TileShape tiles[] = {{32, 16, 1}, {64, 8, 1}, {16, 16, 1},
+                                    {32, 8, 1},  {16, 8, 1}, {8, 8, 1},
+                                    {16, 8, 4},  {7, 8, 7}};
   const int ncand = sizeof(cands) / sizeof(cands[0]);
   const int ntile = sizeof(tiles) / sizeof(tiles[0]);
   void *saveP = nullptr, *saveF = nullptr, *saveFI = nullptr;

@@ -962,6 +962,9 @@ struct TileGeom {
   int stage_win;      // coefficients staged for the tile and a one-cell
                       // ring (1) or the tile only (0)
   int halo;           // window cells beyond the tile on every side (1 or 2)
+  int hx, hy, hz;     // the halo per axis: `halo`, or 0 along an axis of
+                      // one cell (no particle leaves it, so no deposit
+                      // lands beside it)
 };
 
 #ifndef VPIC_TILE_BS
@@ -1191,13 +1194,12 @@ __device__ __forceinline__ void tiled_push(
   WindowSink W;
   W.lds = (lds_u64 *)lds_win;
   W.acc = acc;
-  const int H = T.halo;
-  W.wx0 = x0 - H;
-  W.wy0 = y0 - H;
-  W.wz0 = z0 - H;
-  W.wnx = x1 - x0 + 1 + 2 * H;
-  W.wny = y1 - y0 + 1 + 2 * H;
-  W.wnz = z1 - z0 + 1 + 2 * H;
+  W.wx0 = x0 - T.hx;
+  W.wy0 = y0 - T.hy;
+  W.wz0 = z0 - T.hz;
+  W.wnx = x1 - x0 + 1 + 2 * T.hx;
+  W.wny = y1 - y0 + 1 + 2 * T.hy;
+  W.wnz = z1 - z0 + 1 + 2 * T.hz;
   W.sx1 = G.nx + 2;
   W.sy1 = G.ny + 2;
   W.mx = T.mx;
@@ -1255,10 +1257,12 @@ __device__ __forceinline__ void tiled_push(
   // the tile.  A particle in the box reads its 18 coefficients from LDS,
   // any other from global memory (a dependent global load, which the
   // wavefront waits for, so the box should cover the drifted particles).
-  const int sw = T.stage_win ? 1 : 0;
-  const int sx0 = x0 - sw, sy0 = y0 - sw, sz0 = z0 - sw;
-  const int snx = x1 - x0 + 1 + 2 * sw, sny = y1 - y0 + 1 + 2 * sw,
-            snz = z1 - z0 + 1 + 2 * sw;
+  const int swx = T.stage_win ? min(T.hx, 1) : 0,
+            swy = T.stage_win ? min(T.hy, 1) : 0,
+            swz = T.stage_win ? min(T.hz, 1) : 0;
+  const int sx0 = x0 - swx, sy0 = y0 - swy, sz0 = z0 - swz;
+  const int snx = x1 - x0 + 1 + 2 * swx, sny = y1 - y0 + 1 + 2 * swy,
+            snz = z1 - z0 + 1 + 2 * swz;
   const int sncell = snx * sny * snz;
   float4 *ic_e0 = reinterpret_cast<float4 *>(lds_win + 12 * ncell);
   float4 *ic_e1 = ic_e0 + sncell;
@@ -1674,11 +1678,11 @@ __device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
   for (int c = t; c < ncell; c += blockDim.x) {
     const int x = x0 + c % cx, y = y0 + (c / cx) % cy, z = z0 + c / (cx * cy);
     // tiles whose window holds v, per axis: those holding a cell within
-    // the halo of v, [tile(v - H), tile(v + H)] clamped to the grid
-    const int H = T.halo;
-    const int tx0 = (max(x - H, 1) - 1) / T.tx, tx1 = (min(x + H, G.nx) - 1) / T.tx;
-    const int ty0 = (max(y - H, 1) - 1) / T.ty, ty1 = (min(y + H, G.ny) - 1) / T.ty;
-    const int tz0 = (max(z - H, 1) - 1) / T.tz, tz1 = (min(z + H, G.nz) - 1) / T.tz;
+    // the halo of v, [tile(v - h), tile(v + h)] clamped to the grid
+    const int hx = T.hx, hy = T.hy, hz = T.hz;
+    const int tx0 = (max(x - hx, 1) - 1) / T.tx, tx1 = (min(x + hx, G.nx) - 1) / T.tx;
+    const int ty0 = (max(y - hy, 1) - 1) / T.ty, ty1 = (min(y + hy, G.ny) - 1) / T.ty;
+    const int tz0 = (max(z - hz, 1) - 1) / T.tz, tz1 = (min(z + hz, G.nz) - 1) / T.tz;
     float s[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) s[k] = 0.0f;
@@ -1686,13 +1690,13 @@ __device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
       for (int ty = ty0; ty <= ty1; ty++)
         for (int tx = tx0; tx <= tx1; tx++) {
           const int wx0 = tx * T.tx, wy0 = ty * T.ty, wz0 = tz * T.tz;
-          const int wnx = min(T.tx, G.nx - wx0) + 2 * H;
-          const int wny = min(T.ty, G.ny - wy0) + 2 * H;
-          const int wnz = min(T.tz, G.nz - wz0) + 2 * H;
+          const int wnx = min(T.tx, G.nx - wx0) + 2 * hx;
+          const int wny = min(T.ty, G.ny - wy0) + 2 * hy;
+          const int wnz = min(T.tz, G.nz - wz0) + 2 * hz;
           const int nc = wnx * wny * wnz;
-          // the window of tile (tx, ty, tz) starts at cell 1 + wx0 - H
-          const int pos = (x - 1 - wx0 + H) +
-                          wnx * ((y - 1 - wy0 + H) + wny * (z - 1 - wz0 + H));
+          // the window of tile (tx, ty, tz) starts at cell 1 + wx0 - hx
+          const int pos = (x - 1 - wx0 + hx) +
+                          wnx * ((y - 1 - wy0 + hy) + wny * (z - 1 - wz0 + hz));
           const float *w =
               wbuf + (int64_t)(tx + T.ntx * (ty + T.nty * tz)) * wstride + pos;
 #pragma unroll
@@ -2095,7 +2099,7 @@ int pow2_at_least(int64_t n) {
 // fastest of the shapes timed on the 256x128x128 thermal plasma over whole
 // sort intervals (8x8x8, whose coefficients fit for the tile only, took 6%
 // longer per push)
-TileGeom g_tile = {7, 8, 7, 0, 0, 0, 0, 0, 0, 1};
+TileGeom g_tile = {7, 8, 7, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1};
 int g_group = 16;  // lanes sharing one run of particles in the tiled push
 int g_bin_mode = 0;  // tile lists: 0 off, 1 when drift warrants, 2 always
 
@@ -2155,14 +2159,35 @@ size_t tile_win_cells(int tx, int ty, int tz, int h) {
   return (size_t)(tx + 2 * h) * (ty + 2 * h) * (tz + 2 * h);
 }
 
-// The tile fits the tiled push: rows and cells within the kernel's tables,
-// and its window (96 B per cell) plus at least the tile's own staged
-// coefficients (72 B per cell) within the dynamic LDS.
+// The per-axis halos of T on grid g (0 along an axis of one cell) and the
+// cells of its window and of its coefficient staging box (the tile and a
+// ring of min(halo, 1) cells).
+void tile_axis_halos(TileGeom &T, const vpic_grid_t *g) {
+  T.hx = g->nx == 1 ? 0 : T.halo;
+  T.hy = g->ny == 1 ? 0 : T.halo;
+  T.hz = g->nz == 1 ? 0 : T.halo;
+}
+size_t tile_window(const TileGeom &T) {
+  return (size_t)(T.tx + 2 * T.hx) * (T.ty + 2 * T.hy) * (T.tz + 2 * T.hz);
+}
+size_t tile_ring(const TileGeom &T) {
+  const int rx = T.hx < 1 ? T.hx : 1, ry = T.hy < 1 ? T.hy : 1,
+            rz = T.hz < 1 ? T.hz : 1;
+  return (size_t)(T.tx + 2 * rx) * (T.ty + 2 * ry) * (T.tz + 2 * rz);
+}
+
+// The tile can fit the tiled push: rows and cells within the kernel's
+// tables, and its window (96 B per cell) plus at least the tile's own
+// staged coefficients (72 B per cell) within the dynamic LDS -- counting
+// no halo along an axis of one tile cell, as on a grid one cell thick
+// there (a launch on a thicker grid checks the window it needs).
 bool tile_fits(int tx, int ty, int tz, int h) {
-  return tx >= 1 && ty >= 1 && tz >= 1 && (h == 1 || h == 2) &&
-         ty * tz <= TILE_MAXROWS && tx * ty * tz <= TILE_MAXCELLS &&
-         96 * tile_win_cells(tx, ty, tz, h) + 72 * (size_t)tx * ty * tz <=
-             (size_t)TILE_DYN_BYTES;
+  if (!(tx >= 1 && ty >= 1 && tz >= 1 && (h == 1 || h == 2) &&
+        ty * tz <= TILE_MAXROWS && tx * ty * tz <= TILE_MAXCELLS))
+    return false;
+  const size_t w = (size_t)(tx + (tx > 1 ? 2 * h : 0)) *
+                   (ty + (ty > 1 ? 2 * h : 0)) * (tz + (tz > 1 ? 2 * h : 0));
+  return 96 * w + 72 * (size_t)tx * ty * tz <= (size_t)TILE_DYN_BYTES;
 }
 
 }  // namespace
@@ -2369,6 +2394,11 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
     const int *flat_start = nullptr;
     if (partition) {
       TileGeom T = g_tile;
+      // a tile no larger than the grid (a 2D grid's tiles are one cell
+      // thick; the window then has no halo along that axis either)
+      T.tx = T.tx < g->nx ? T.tx : g->nx;
+      T.ty = T.ty < g->ny ? T.ty : g->ny;
+      T.tz = T.tz < g->nz ? T.tz : g->nz;
       T.ntx = (g->nx + T.tx - 1) / T.tx;
       T.nty = (g->ny + T.ty - 1) / T.ty;
       T.ntz = (g->nz + T.tz - 1) / T.tz;
@@ -2376,9 +2406,16 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
       T.mx = magic_ok ? div_magic(sx1) : 0;
       T.my = magic_ok ? div_magic(sy1) : 0;
-      const size_t wcells = tile_win_cells(T.tx, T.ty, T.tz, T.halo);
+      tile_axis_halos(T, g);
+      const size_t wcells = tile_window(T);
       const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
-      const size_t scells = tile_win_cells(T.tx, T.ty, T.tz, 1);
+      const size_t scells = tile_ring(T);
+      if (win_bytes + 72 * (size_t)T.tx * T.ty * T.tz > (size_t)TILE_DYN_BYTES)
+        return vpic_set_error(VPIC_EINVAL,
+                              "advance_p: push tile %dx%dx%d with halo %d "
+                              "needs %zu window cells on this grid, more "
+                              "than the LDS holds", T.tx, T.ty, T.tz, T.halo,
+                              wcells);
       T.stage_win = win_bytes + 72 * scells <= (size_t)TILE_DYN_BYTES;
       const size_t lds =
           win_bytes +
@@ -2580,7 +2617,10 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
     T.ty = ty < g.ny ? ty : g.ny;
     T.tz = tz < g.nz ? tz : g.nz;
     T.halo = 1;
-    if (!tile_fits(T.tx, T.ty, T.tz, T.halo))
+    tile_axis_halos(T, &g);
+    if (T.ty * T.tz > TILE_MAXROWS || T.tx * T.ty * T.tz > TILE_MAXCELLS ||
+        96 * tile_window(T) + 72 * (size_t)T.tx * T.ty * T.tz >
+            (size_t)TILE_DYN_BYTES)
       return vpic_set_error(VPIC_EINVAL, "ensemble tiles: tile %dx%dx%d of "
                             "box %d does not fit the LDS window",
                             T.tx, T.ty, T.tz, b);
@@ -2593,9 +2633,9 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
                             b);
     T.mx = div_magic(sx1);
     T.my = div_magic(sy1);
-    const size_t wcells = tile_win_cells(T.tx, T.ty, T.tz, T.halo);
+    const size_t wcells = tile_window(T);
     const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
-    const size_t scells = tile_win_cells(T.tx, T.ty, T.tz, 1);
+    const size_t scells = tile_ring(T);
     T.stage_win = win_bytes + 72 * scells <= (size_t)TILE_DYN_BYTES;
     const size_t l = win_bytes +
                      72 * (T.stage_win ? scells : (size_t)T.tx * T.ty * T.tz);
