@@ -207,9 +207,10 @@ int vpic_hip_advance_p_async(vpic_particle_t *p, int32_t np, float q,
 
 /* vpic_hip_advance_p_async that also writes keys[k] = p[k].i as the push
  * leaves it (k < np; -1 for a mover returned to the caller, whose index
- * the caller rewrites), for a vpic_hip_sort_p_keys of these particles at
- * the start of the next step (advance.cc:25-29 sorts there): the sort
- * then reads 4 B per particle instead of the 32-B records.  No reference
+ * the caller rewrites; keys NULL: no keys), for a vpic_hip_sort_p_keys of
+ * these particles at the start of the next step (advance.cc:25-29 sorts
+ * there): the sort then reads 4 B per particle instead of the 32-B
+ * records.  No reference
  * counterpart (the reference's push and sort share no state). */
 int vpic_hip_advance_p_keys(vpic_particle_t *p, int32_t np, float q, float m,
                             vpic_particle_mover_t *pm, int32_t max_nm,
@@ -217,6 +218,31 @@ int vpic_hip_advance_p_keys(vpic_particle_t *p, int32_t np, float q, float m,
                             const vpic_interpolator_t *f,
                             const vpic_grid_t *g, const int32_t *partition,
                             int32_t *keys, void *stream);
+
+/* One species' arguments of vpic_hip_advance_p_species (those of
+ * vpic_hip_advance_p_keys; keys may be NULL). */
+typedef struct vpic_push_species {
+  vpic_particle_t *p;
+  int32_t np;
+  float q, m;
+  vpic_particle_mover_t *pm;
+  int32_t max_nm;
+  int32_t *dev_nm;
+  const int32_t *partition;
+  int32_t *keys;
+} vpic_push_species_t;
+
+/* advance_p of nsp species (1 or 2) into one accumulator: the same as
+ * vpic_hip_advance_p_keys for each in turn (advance.cc:56-62 loops
+ * advance_p over the species list), except for the summation order of
+ * the currents.  When every species has a partition, one tiled launch
+ * pushes them all, tile by tile, into the same LDS windows with the same
+ * staged coefficients, and one merge adds the windows.  No reference
+ * counterpart. */
+int vpic_hip_advance_p_species(int32_t nsp, const vpic_push_species_t *sp,
+                               vpic_accumulator_t *a,
+                               const vpic_interpolator_t *f,
+                               const vpic_grid_t *g, void *stream);
 
 /* vpic_hip_advance_p_async without a partition (the flat push), for
  * callers that cannot allocate device memory: code under HIP graph

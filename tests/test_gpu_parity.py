@@ -441,6 +441,54 @@ def test_push_keys_then_sort_p_keys(dev, orc, flat):
         assert _stale_count() - before == (0 if case == "keys" else 1), case
 
 
+@pytest.mark.parametrize("case", ["both sorted", "one unsorted"])
+def test_advance_p_species_pair(dev, orc, case, tile_reset):
+    """Electrons and ions in one vpic_hip_advance_p_species call (one tiled
+    launch through the same windows when both have partitions, one call
+    each otherwise): each species bit-exact, the shared accumulator
+    against the oracle's two advance_p calls, the keys the particles'."""
+    _lib.call("vpic_hip_push_tile", 4, 3, 5)
+    g = _box(9, 7, 6)
+    f = H.random_full_fields(g, 81, amp=0.3)
+    fi = H.interp_array(g)
+    orc.load_interpolator(fi, f, g)
+    hg = host_grid(g)
+    specs = [(-1.0, 1.0, 0.3, 11, 91), (1.0, 25.0, 0.05, 7, 92)]
+    ps, parts = [], []
+    for q, m, uth, ppc, seed in specs:
+        p = H.thermal_particles(g, ppc, uth, seed, shuffle=True)
+        parts.append(orc.sort_p(p, g))
+        for _ in range(3):  # drift off the sort-time cells
+            orc.advance_p(p, q, m, H.acc_array(g), fi, hg)
+        ps.append(p)
+    ao = H.acc_array(g)
+    pos = [p.copy() for p in ps]
+    for (q, m, _, _, _), po in zip(specs, pos):
+        orc.advance_p(po, q, m, ao, fi, hg)
+    dfi = dev.DeviceArray.from_host(fi)
+    da = dev.DeviceArray.from_host(H.acc_array(g))
+    keep, arr = [], (T.PushSpecies * 2)()
+    for k, ((q, m, _, _, _), p, part) in enumerate(zip(specs, ps, parts)):
+        dp = dev.DeviceArray.from_host(p)
+        dpm = dev.movers(len(p))
+        dnm = dev.DeviceArray((4,), np.int32)
+        dkeys = dev.DeviceArray((len(p),), np.int32)
+        dpart = (dev.DeviceArray.from_host(part.astype(np.int32))
+                 if case == "both sorted" or k == 0 else None)
+        keep.append((dp, dpm, dnm, dkeys, dpart))
+        arr[k] = T.PushSpecies(dp.ptr, len(p), q, m, dpm.ptr, len(p),
+                               dnm.ptr, dpart.ptr if dpart else None,
+                               dkeys.ptr)
+    say("advance_p_species", case)
+    _lib.call("vpic_hip_advance_p_species", 2, arr, da.ptr, dfi.ptr,
+              ctypes.byref(g), None)
+    for k, (dp, dpm, dnm, dkeys, dpart) in enumerate(keep):
+        assert H.same_bits(dp.download(), pos[k]), k
+        assert np.array_equal(dkeys.download(), pos[k]["i"]), k
+        assert tuple(dnm.download()[:3]) == (0, 0, 0), k
+    assert H.acc_rel_err(da.download(), ao) <= ACC_TOL
+
+
 def test_energy_p_and_rho(dev, orc):
     g = _box(6, 5, 4)
     p, fi = _inputs(orc, g, 7, 0.1, 12)

@@ -217,7 +217,6 @@ extern "C" int vpic_hip_advance_p_keys(
   int rc = check_push_args(p, np, pm, max_nm, a, f, g);
   if (rc) return rc;
   CHECK(dev_nm, "dev_nm is NULL");
-  CHECK(keys || np == 0, "keys is NULL");
   return launch_advance_p(p, np, q, m, pm, max_nm, dev_nm, a, f, g, partition,
                           keys, S(stream));
 }
@@ -399,6 +398,25 @@ extern "C" int vpic_hip_takizuka_abe(
         "aligned");
   return launch_takizuka_abe(pi, part_i, np_i, qi, mi, pj, part_j, np_j, qj,
                              mj, cvar0, interval, seed, call, g, S(stream));
+}
+
+extern "C" int vpic_hip_advance_p_species(int32_t nsp,
+                                          const vpic_push_species_t *sp,
+                                          vpic_accumulator_t *a,
+                                          const vpic_interpolator_t *f,
+                                          const vpic_grid_t *g, void *stream) {
+  CHECK(sp, "sp is NULL");
+  CHECK(nsp >= 1 && nsp <= 2, "%d species in one advance_p (1 or 2)", nsp);
+  for (int k = 0; k < nsp; k++) {
+    int rc = check_push_args(sp[k].p, sp[k].np, sp[k].pm, sp[k].max_nm, a, f,
+                             g);
+    if (rc) return rc;
+    CHECK(sp[k].dev_nm, "species %d: dev_nm is NULL", k);
+    for (int j = 0; j < k; j++)
+      CHECK(sp[j].p != sp[k].p || sp[k].np == 0,
+            "species %d and %d share a particle array", j, k);
+  }
+  return launch_advance_p_set(nsp, sp, a, f, g, S(stream));
 }
 
 extern "C" int vpic_hip_sort_p(vpic_particle_t *p, vpic_particle_t *aux,

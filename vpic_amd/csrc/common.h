@@ -216,6 +216,11 @@ struct DeviceScratch {
 int scratch_reserve(DeviceScratch &s, size_t bytes);
 
 // Launch-side kernels entry points implemented in the .hip files.
+// advance_p of several species into one accumulator (one tiled launch when
+// all have partitions).
+int launch_advance_p_set(int nsp, const vpic_push_species_t *sp,
+                         vpic_accumulator_t *a, const vpic_interpolator_t *f,
+                         const vpic_grid_t *g, hipStream_t s);
 // keys: when not null, keys[k] = p[k].i as the push leaves it, for a
 // following launch_sort_p_keys (-1 for a mover returned to the caller).
 int launch_advance_p(vpic_particle_t *p, int np, float q, float m,

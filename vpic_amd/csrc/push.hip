@@ -993,6 +993,26 @@ struct EnsTileItem {
   int box, tile;
 };
 
+// One species of a tiled push: its particles, sort-time partition,
+// couplings, returned-mover list and counters, and (when not null) the
+// voxel keys to write for a following sort_p.
+struct TileSpecies {
+  vpic_particle_t *p;
+  const int *part;
+  vpic_particle_mover_t *rlist;
+  int *cnt;
+  int *vkeys;
+  PushCoef C;
+  int np, rcap;
+};
+
+// The species one launch of the tiled push advances into the same windows
+// (NSP of them, fixed per kernel instance).
+constexpr int TILE_MAXSP = 2;
+struct TileSpeciesSet {
+  TileSpecies s[TILE_MAXSP];
+};
+
 constexpr int TILE_BS = VPIC_TILE_BS;     // threads per workgroup
 constexpr int TILE_WAVES = TILE_BS / 64;
 // waves per SIMD the launch bounds ask for (one workgroup per CU)
@@ -1083,15 +1103,14 @@ struct BinArgs {
   unsigned char *key; // per particle offset code (written); null: no binning
   int *counts;        // per tile 27 code counts (written)
   int *oow;           // out-of-window deposits (accumulated)
-  int *vkeys;         // per particle voxel index after the push, for a
-                      // following sort_p (written); null: not asked for
 };
 
 struct BinSink {
   unsigned char *key;  // null: no binning
   int *cnt;            // LDS code counts
   TileBox box;
-  int *vkeys;          // as BinArgs::vkeys
+  int *vkeys;          // per particle voxel index after the push, for a
+                       // following sort_p (written); null: not asked for
 };
 
 __device__ __forceinline__ void drain_wave_queue(
@@ -1149,18 +1168,22 @@ __device__ __forceinline__ void drain_wave_queue(
 // that do not match return at once; compiling each state separately keeps
 // the unused paths' values out of the scalar registers of the particle loop.
 //
+// NSP species (each with its own particles, partition, couplings and
+// mover lists) are pushed in turn by every tile into the same window, so
+// the species of one step share one launch, one staging of the
+// coefficients and one merge.
+//
 // The body is a device function of the tile index, so that a batched
 // launch over many grids (k_ens_push_tiled) runs the same code per (grid,
 // tile) work item.
-template <int GS, int MODE>
+template <int GS, int MODE, int NSP>
 __device__ __forceinline__ void tiled_push(
-    int tile, vpic_particle_t *__restrict__ p, int np,
-    const int *__restrict__ part, const vpic_interpolator_t *__restrict__ f0,
+    int tile, const TileSpecies *SP, const vpic_interpolator_t *__restrict__ f0,
     float *__restrict__ acc, float *__restrict__ wbuf, int wstride,
-    vpic_particle_t *__restrict__ trash,
-    vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
-    const PushCoef &C, const GridDev &G, const TileGeom &T, const BinArgs &Bn,
-    const OowBuckets &OB) {
+    vpic_particle_t *__restrict__ trash, const GridDev &G, const TileGeom &T,
+    const BinArgs &Bn, const OowBuckets &OB) {
+  static_assert(NSP >= 1 && NSP <= TILE_MAXSP, "species per launch");
+  static_assert(NSP == 1 || MODE == 0, "tile lists push one species");
   extern __shared__ unsigned long long lds_win[];  // 12 * ncell
   __shared__ vpic_particle_mover_t wq[TILE_WAVES][WAVE_Q];
   __shared__ int row_start[TILE_MAXROWS];
@@ -1225,7 +1248,7 @@ __device__ __forceinline__ void tiled_push(
   BinSink BS;
   BS.key = keyed ? Bn.key : nullptr;
   BS.cnt = bin_cnt;
-  BS.vkeys = Bn.vkeys;
+  BS.vkeys = nullptr;
   TileBox &TB = BS.box;
   TB.x0 = x0;
   TB.x1 = x1;
@@ -1240,16 +1263,6 @@ __device__ __forceinline__ void tiled_push(
   TB.lz = btz > 0 ? z0 - T.tz : z0;
   TB.hz = btz + 1 < T.ntz ? min(z1 + T.tz, G.nz) : z1;
 
-  // Rows of the tile in the sort-time partition, clamped to [0, np).
-  for (int r = t; r < nrow; r += TILE_BS) {
-    const int y = y0 + r % rows_y, z = z0 + r / rows_y;
-    int s = part[vox(x0, y, z, G.nx, G.ny)];
-    int e = part[vox(x1 + 1, y, z, G.nx, G.ny)];
-    s = min(max(s, 0), np);
-    e = min(max(e, s), np);
-    row_start[r] = s;
-    row_pref[r + 1] = e - s;
-  }
   for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
   // Interpolation coefficients staged in LDS (after the window) for the
   // cells of the staging box: the tile and a one-cell ring around it (which
@@ -1284,264 +1297,304 @@ __device__ __forceinline__ void tiled_push(
   if (t == 0) oow_cnt = 0;
   if (t < 27) oow_dcnt[t] = 0;
   __syncthreads();
-  if (t < 64) {
-    // inclusive prefix of the row lengths by the first wavefront: four rows
-    // per lane (nrow <= TILE_MAXROWS = 256), then the lanes' totals
-    static_assert(TILE_MAXROWS <= 256, "four rows per lane");
-    int v[4], run = 0;
+  // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
+  // the first particle of every row of every species.  One particle's
+  // current is below 4.4|q w|, so its components scale below 2^31 and
+  // convert in FP32; a register sum of many of them may exceed 2^31 and
+  // converts in FP64 (exact below 2^51: sums of up to 2^20 particles; a
+  // larger value takes the float path), as does a particle of larger |q w|.
+  // The resolution, 8|q w|max / 2^31 per deposit, is far below FP32
+  // rounding of the cell sums.
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int r = 4 * t + i;
-      run += r < nrow ? row_pref[r + 1] : 0;
-      v[i] = run;
+  for (int sp = 0; sp < NSP; sp++) {
+    const TileSpecies &S = SP[sp];
+    for (int r = t; r < nrow; r += TILE_BS) {
+      const int y = y0 + r % rows_y, z = z0 + r / rows_y;
+      int a = S.part[vox(x0, y, z, G.nx, G.ny)];
+      int e = S.part[vox(x1 + 1, y, z, G.nx, G.ny)];
+      a = min(max(a, 0), S.np);
+      e = min(max(e, a), S.np);
+      if (a < e) {
+        const float4 *pp = reinterpret_cast<const float4 *>(S.p + a);
+        const float4 A = pp[0], B = pp[1];
+        if ((unsigned)__float_as_int(A.w) < (unsigned)G.nv)
+          atomicMax(&qw_bits, __float_as_uint(fabsf(S.C.qsp * B.w)));
+      }
     }
-    int incl = run;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int o = __shfl_up(incl, d);
-      if (t >= d) incl += o;
-    }
-    const int off = incl - run;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int r = 4 * t + i;
-      if (r < nrow) row_pref[r + 1] = v[i] + off;
-    }
-    if (t == 0) row_pref[0] = 0;
   }
   __syncthreads();
-  // the slots: the partition rows, or this tile's list
-  int total = row_pref[nrow];
-  const int *L = Bn.list;
-  if (listed) {
-    int ls = Bn.lstart[tile], le = Bn.lstart[tile + 1];
-    ls = min(max(ls, 0), np);
-    le = min(max(le, ls), np);
-    L += ls;
-    total = le - ls;
-  }
-
-  // This group's run of slots [g0, g1); lane j of the group takes slots
-  // g0 + j, g0 + j + GS, ...
-  constexpr int NGROUP = TILE_BS / GS;
-  const int per = (total + NGROUP - 1) / NGROUP;  // slots per group
-  const int steps = (per + GS - 1) / GS;          // same for every lane
-  const int grp = t / GS, j = t % GS;
-  const int g0 = min(grp * per, total), g1 = min(g0 + per, total);
-  const vpic_particle_mover_t *qr = wq[t >> 6];
-  vpic_particle_mover_t *q = wq[t >> 6];
-  int qn = 0;  // entries in this wavefront's queue (wavefront-uniform)
-
-  float jacc[12];  // this lane's current, not yet in the window
-#pragma unroll
-  for (int k = 0; k < 12; k++) jacc[k] = 0.0f;
-  int jkey = -1;   // its voxel (-1: nothing held)
-  int jcell = -1;  // jkey's window cell (-1: outside the window)
-  Interp F;        // coefficients of voxel fkey
-  F.e0 = F.e1 = F.e2 = F.b0 = make_float4(0.f, 0.f, 0.f, 0.f);
-  F.b1 = make_float2(0.f, 0.f);
-  int fkey = -1;
-  int fcell = -1;  // fkey's window cell (-1: outside the window)
-  int nbad = 0;  // slots with a voxel index outside the grid
-
-  RowWalk rw;
-  rw.start(row_start, row_pref);
-  int k = g0 + j;  // the slot being pushed
-  // The particle of slot ks (false past the group's run; a list entry
-  // outside [0, np) is skipped).  Asked for increasing ks only.
-  auto particle_of = [&](int ks, int &nn) -> bool {
-    nn = 0;
-    if (ks >= g1) return false;
-    nn = listed ? L[ks] : rw.particle(row_start, row_pref, nrow, ks);
-    const bool ok = (unsigned)nn < (unsigned)np;
-    if (!ok) nn = 0;
-    return ok;
-  };
-  // A lane's particles in flight: the one it pushes and the next
-  // PUSH_DEPTH - 1 loaded ahead; each slot loads the particle PUSH_DEPTH
-  // slots on into the set it frees, so a load is waited for PUSH_DEPTH
-  // slots after it is issued.
-  struct PSlot {
-    float4 A, B;
-    int n;
-    bool ok;
-  };
-  PSlot P0, P1, P2, P3;
-  P0.ok = particle_of(k, P0.n);
-  load_particle(p, P0.n, P0.ok, P0.A, P0.B);
-  if (PUSH_DEPTH >= 2) {
-    P1.ok = particle_of(k + GS, P1.n);
-    load_particle(p, P1.n, P1.ok, P1.A, P1.B);
-  }
-  if (PUSH_DEPTH >= 3) {
-    P2.ok = particle_of(k + 2 * GS, P2.n);
-    load_particle(p, P2.n, P2.ok, P2.A, P2.B);
-  }
-  const float4 A0 = P0.A, B0 = P0.B;
-  const bool cur_ok = P0.ok;
-  PushCoef Cs = C;  // C with the deposit charge scaled by the window's 2^k
   {
-    // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
-    // the first particle of every lane.  One particle's current is below
-    // 4.4|q w|, so its components scale below 2^31 and convert in FP32; a
-    // register sum of many of them may exceed 2^31 and converts in FP64
-    // (exact below 2^51: sums of up to 2^20 particles; a larger value takes
-    // the float path).  The resolution, 8|q w|max / 2^31 per deposit, is far
-    // below FP32 rounding of the cell sums.
-    const bool act0 =
-        cur_ok && (unsigned)__float_as_int(A0.w) < (unsigned)G.nv;
-    const float qw = act0 ? fabsf(C.qsp * B0.w) : 0.0f;
-    atomicMax(&qw_bits, __float_as_uint(qw));
-    __syncthreads();
     const float m = __uint_as_float(qw_bits);
     int e = 0;
     if (m > 0.0f) frexpf(8.0f * m, &e);  // 8|q w|max < 2^e
     W.scale = ldexp(1.0, 31 - e);
     W.fscale = (float)W.scale;
     W.ifscale = (float)(1.0 / W.scale);
-    // The deposits of particles that stay in their cell are formed with the
-    // charge times 2^k (exact while the product is a normal float), so the
-    // register sums need no scaling when they are flushed.
-    Cs.qsp = C.qsp * W.fscale;
-    const float aq = fabsf(Cs.qsp);
-    W.pre = W.fscale > 0.0f && aq <= 3.0e38f && W.ifscale > 0.0f &&
-            (C.qsp == 0.0f || aq >= 1.0e-30f);
-    if (!W.pre) Cs.qsp = C.qsp;
   }
-  int it = 0;
-  auto slot = [&](const PSlot &cur, PSlot &pref) {
-    const float4 A = cur.A, B = cur.B;
-    const int n = cur.n;
-    const bool valid = cur.ok;
-    const int iraw = __float_as_int(A.w);
-    const bool active = valid && (unsigned)iraw < (unsigned)G.nv;
-    nbad += valid && !active;
-    const int ii = active ? iraw : 0;
-    if (active && ii != fkey) {
-      int x, y, z;
-      VD.coords(G, ii, x, y, z);
-      const int wx = x - W.wx0, wy = y - W.wy0, wz = z - W.wz0;
-      fcell = ((unsigned)wx < (unsigned)W.wnx &&
-               (unsigned)wy < (unsigned)W.wny &&
-               (unsigned)wz < (unsigned)W.wnz)
-                  ? wx + W.wnx * (wy + W.wny * wz)
-                  : -1;
-      const int sx = x - sx0, sy = y - sy0, sz = z - sz0;
-      if ((unsigned)sx < (unsigned)snx && (unsigned)sy < (unsigned)sny &&
-          (unsigned)sz < (unsigned)snz) {
-        const int c = sx + snx * (sy + sny * sz);
-        F.e0 = ic_e0[c];
-        F.e1 = ic_e1[c];
-        F.e2 = ic_e2[c];
-        F.b0 = ic_b0[c];
-        F.b1 = ic_b1[c];
-      } else {
-        load_interp(f0, ii, F);
-      }
-      fkey = ii;
-    }
-    // prefetch this lane's next particle (issued after the coefficients,
-    // so waiting for them leaves the prefetch in flight), then the list
-    // entry after it
-    const int kn = k + PUSH_DEPTH * GS;
-    pref.ok = particle_of(kn, pref.n);
-    load_particle(p, pref.n, pref.ok, pref.A, pref.B);
-    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch issued here
 
-    // every lane stores (an idle lane to its own trash slot), so the
-    // number of memory operations per iteration does not depend on the
-    // particle and the loop never waits for its own stores
-    PushResult R;
-    push_compute_u(A, B, active, F, Cs,
-                   valid ? reinterpret_cast<float4 *>(p + n)
-                         : reinterpret_cast<float4 *>(trash + t),
-                   R);
-    // a particle that stays in its cell keeps its voxel index (a crosser's
-    // is written when it is moved)
-    if (BS.vkeys && valid && !R.cross) BS.vkeys[n] = iraw;
-    if (keyed) {
-      // the tile holding the particle now (a crosser's is set when it is
-      // moved)
-      int o = 13;
-      if (active && !R.cross) {
+  // The species in turn, into the same window.
+#pragma unroll
+  for (int sp = 0; sp < NSP; sp++) {
+    vpic_particle_t *__restrict__ p = SP[sp].p;
+    const int np = SP[sp].np;
+    const int *__restrict__ part = SP[sp].part;
+    vpic_particle_mover_t *__restrict__ rlist = SP[sp].rlist;
+    const int rcap = SP[sp].rcap;
+    int *__restrict__ cnt = SP[sp].cnt;
+    const PushCoef C = SP[sp].C;
+    BS.vkeys = SP[sp].vkeys;
+    // all wavefronts are done with the last species' rows
+    if (sp > 0) __syncthreads();
+    // Rows of the tile in the sort-time partition, clamped to [0, np).
+    for (int r = t; r < nrow; r += TILE_BS) {
+      const int y = y0 + r % rows_y, z = z0 + r / rows_y;
+      int s = part[vox(x0, y, z, G.nx, G.ny)];
+      int e = part[vox(x1 + 1, y, z, G.nx, G.ny)];
+      s = min(max(s, 0), np);
+      e = min(max(e, s), np);
+      row_start[r] = s;
+      row_pref[r + 1] = e - s;
+    }
+    __syncthreads();
+    if (t < 64) {
+      // inclusive prefix of the row lengths by the first wavefront: four rows
+      // per lane (nrow <= TILE_MAXROWS = 256), then the lanes' totals
+      static_assert(TILE_MAXROWS <= 256, "four rows per lane");
+      int v[4], run = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int r = 4 * t + i;
+        run += r < nrow ? row_pref[r + 1] : 0;
+        v[i] = run;
+      }
+      int incl = run;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(incl, d);
+        if (t >= d) incl += o;
+      }
+      const int off = incl - run;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int r = 4 * t + i;
+        if (r < nrow) row_pref[r + 1] = v[i] + off;
+      }
+      if (t == 0) row_pref[0] = 0;
+    }
+    __syncthreads();
+    // the slots: the partition rows, or this tile's list
+    int total = row_pref[nrow];
+    const int *L = Bn.list;
+    if (listed) {
+      int ls = Bn.lstart[tile], le = Bn.lstart[tile + 1];
+      ls = min(max(ls, 0), np);
+      le = min(max(le, ls), np);
+      L += ls;
+      total = le - ls;
+    }
+
+    // This group's run of slots [g0, g1); lane j of the group takes slots
+    // g0 + j, g0 + j + GS, ...
+    constexpr int NGROUP = TILE_BS / GS;
+    const int per = (total + NGROUP - 1) / NGROUP;  // slots per group
+    const int steps = (per + GS - 1) / GS;          // same for every lane
+    const int grp = t / GS, j = t % GS;
+    const int g0 = min(grp * per, total), g1 = min(g0 + per, total);
+    const vpic_particle_mover_t *qr = wq[t >> 6];
+    vpic_particle_mover_t *q = wq[t >> 6];
+    int qn = 0;  // entries in this wavefront's queue (wavefront-uniform)
+
+    float jacc[12];  // this lane's current, not yet in the window
+#pragma unroll
+    for (int k = 0; k < 12; k++) jacc[k] = 0.0f;
+    int jkey = -1;   // its voxel (-1: nothing held)
+    int jcell = -1;  // jkey's window cell (-1: outside the window)
+    Interp F;        // coefficients of voxel fkey
+    F.e0 = F.e1 = F.e2 = F.b0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    F.b1 = make_float2(0.f, 0.f);
+    int fkey = -1;
+    int fcell = -1;  // fkey's window cell (-1: outside the window)
+    int nbad = 0;  // slots with a voxel index outside the grid
+
+    RowWalk rw;
+    rw.start(row_start, row_pref);
+    int k = g0 + j;  // the slot being pushed
+    // The particle of slot ks (false past the group's run; a list entry
+    // outside [0, np) is skipped).  Asked for increasing ks only.
+    auto particle_of = [&](int ks, int &nn) -> bool {
+      nn = 0;
+      if (ks >= g1) return false;
+      nn = listed ? L[ks] : rw.particle(row_start, row_pref, nrow, ks);
+      const bool ok = (unsigned)nn < (unsigned)np;
+      if (!ok) nn = 0;
+      return ok;
+    };
+    // A lane's particles in flight: the one it pushes and the next
+    // PUSH_DEPTH - 1 loaded ahead; each slot loads the particle PUSH_DEPTH
+    // slots on into the set it frees, so a load is waited for PUSH_DEPTH
+    // slots after it is issued.
+    struct PSlot {
+      float4 A, B;
+      int n;
+      bool ok;
+    };
+    PSlot P0, P1, P2, P3;
+    P0.ok = particle_of(k, P0.n);
+    load_particle(p, P0.n, P0.ok, P0.A, P0.B);
+    if (PUSH_DEPTH >= 2) {
+      P1.ok = particle_of(k + GS, P1.n);
+      load_particle(p, P1.n, P1.ok, P1.A, P1.B);
+    }
+    if (PUSH_DEPTH >= 3) {
+      P2.ok = particle_of(k + 2 * GS, P2.n);
+      load_particle(p, P2.n, P2.ok, P2.A, P2.B);
+    }
+    PushCoef Cs = C;  // C with the deposit charge scaled by the window's 2^k
+    {
+      // The deposits of particles that stay in their cell are formed with the
+      // charge times 2^k (exact while the product is a normal float), so the
+      // register sums need no scaling when they are flushed.
+      Cs.qsp = C.qsp * W.fscale;
+      const float aq = fabsf(Cs.qsp);
+      W.pre = W.fscale > 0.0f && aq <= 3.0e38f && W.ifscale > 0.0f &&
+              (C.qsp == 0.0f || aq >= 1.0e-30f);
+      if (!W.pre) Cs.qsp = C.qsp;
+    }
+    int it = 0;
+    auto slot = [&](const PSlot &cur, PSlot &pref) {
+      const float4 A = cur.A, B = cur.B;
+      const int n = cur.n;
+      const bool valid = cur.ok;
+      const int iraw = __float_as_int(A.w);
+      const bool active = valid && (unsigned)iraw < (unsigned)G.nv;
+      nbad += valid && !active;
+      const int ii = active ? iraw : 0;
+      if (active && ii != fkey) {
         int x, y, z;
         VD.coords(G, ii, x, y, z);
-        o = TB.code(x, y, z);
+        const int wx = x - W.wx0, wy = y - W.wy0, wz = z - W.wz0;
+        fcell = ((unsigned)wx < (unsigned)W.wnx &&
+                 (unsigned)wy < (unsigned)W.wny &&
+                 (unsigned)wz < (unsigned)W.wnz)
+                    ? wx + W.wnx * (wy + W.wny * wz)
+                    : -1;
+        const int sx = x - sx0, sy = y - sy0, sz = z - sz0;
+        if ((unsigned)sx < (unsigned)snx && (unsigned)sy < (unsigned)sny &&
+            (unsigned)sz < (unsigned)snz) {
+          const int c = sx + snx * (sy + sny * sz);
+          F.e0 = ic_e0[c];
+          F.e1 = ic_e1[c];
+          F.e2 = ic_e2[c];
+          F.b0 = ic_b0[c];
+          F.b1 = ic_b1[c];
+        } else {
+          load_interp(f0, ii, F);
+        }
+        fkey = ii;
       }
-      unsigned char *kp = valid ? Bn.key + n
-                                : reinterpret_cast<unsigned char *>(trash) + t;
-      *kp = (unsigned char)o;
-      const bool counted = valid && !R.cross;
-      const uint64_t m13 = __ballot(counted && o == 13);
-      if (m13 && lane == __ffsll((unsigned long long)m13) - 1)
-        atomicAdd(&bin_cnt[13], __popcll(m13));
-      if (counted && o != 13) atomicAdd(&bin_cnt[o], 1);
-    }
-    if (R.active && !R.cross) {
-      if (ii != jkey) {
-        if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
+      // prefetch this lane's next particle (issued after the coefficients,
+      // so waiting for them leaves the prefetch in flight), then the list
+      // entry after it
+      const int kn = k + PUSH_DEPTH * GS;
+      pref.ok = particle_of(kn, pref.n);
+      load_particle(p, pref.n, pref.ok, pref.A, pref.B);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch issued here
+
+      // every lane stores (an idle lane to its own trash slot), so the
+      // number of memory operations per iteration does not depend on the
+      // particle and the loop never waits for its own stores
+      PushResult R;
+      push_compute_u(A, B, active, F, Cs,
+                     valid ? reinterpret_cast<float4 *>(p + n)
+                           : reinterpret_cast<float4 *>(trash + t),
+                     R);
+      // a particle that stays in its cell keeps its voxel index (a crosser's
+      // is written when it is moved)
+      if (BS.vkeys && valid && !R.cross) BS.vkeys[n] = iraw;
+      if (keyed) {
+        // the tile holding the particle now (a crosser's is set when it is
+        // moved)
+        int o = 13;
+        if (active && !R.cross) {
+          int x, y, z;
+          VD.coords(G, ii, x, y, z);
+          o = TB.code(x, y, z);
+        }
+        unsigned char *kp = valid ? Bn.key + n
+                                  : reinterpret_cast<unsigned char *>(trash) + t;
+        *kp = (unsigned char)o;
+        const bool counted = valid && !R.cross;
+        const uint64_t m13 = __ballot(counted && o == 13);
+        if (m13 && lane == __ffsll((unsigned long long)m13) - 1)
+          atomicAdd(&bin_cnt[13], __popcll(m13));
+        if (counted && o != 13) atomicAdd(&bin_cnt[o], 1);
+      }
+      if (R.active && !R.cross) {
+        if (ii != jkey) {
+          if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
 #pragma unroll
-        for (int c = 0; c < 12; c++) jacc[c] = R.j[c];
-        jkey = ii;
-        jcell = fcell;
-      } else {
+          for (int c = 0; c < 12; c++) jacc[c] = R.j[c];
+          jkey = ii;
+          jcell = fcell;
+        } else {
 #pragma unroll
-        for (int c = 0; c < 12; c++) jacc[c] += R.j[c];
+          for (int c = 0; c < 12; c++) jacc[c] += R.j[c];
+        }
       }
-    }
-    // queue the crossers; drain first if they would not fit
-    const uint64_t cm = __ballot(R.cross);
-    if (cm) {
-      const int c = __popcll(cm);
-      if (qn + c > WAVE_Q) {
-        drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt,
-                         BS);
-        qn = 0;
+      // queue the crossers; drain first if they would not fit
+      const uint64_t cm = __ballot(R.cross);
+      if (cm) {
+        const int c = __popcll(cm);
+        if (qn + c > WAVE_Q) {
+          drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt,
+                           BS);
+          qn = 0;
+        }
+        if (R.cross) {
+          vpic_particle_mover_t mv;
+          mv.dispx = R.dispx;
+          mv.dispy = R.dispy;
+          mv.dispz = R.dispz;
+          mv.i = n;
+          q[qn + __popcll(cm & ((1ull << lane) - 1ull))] = mv;
+        }
+        qn += c;
       }
-      if (R.cross) {
-        vpic_particle_mover_t mv;
-        mv.dispx = R.dispx;
-        mv.dispy = R.dispy;
-        mv.dispz = R.dispz;
-        mv.i = n;
-        q[qn + __popcll(cm & ((1ull << lane) - 1ull))] = mv;
+      k += GS;
+      it++;
+    };
+    // the sets rotate: slot `it` pushes set it % (PUSH_DEPTH + 1) and loads
+    // into set (it + PUSH_DEPTH) % (PUSH_DEPTH + 1)
+    if (PUSH_DEPTH == 1) {
+      while (it + 1 < steps) {
+        slot(P0, P1);
+        slot(P1, P0);
       }
-      qn += c;
+      if (it < steps) slot(P0, P1);
+    } else if (PUSH_DEPTH == 2) {
+      while (it + 2 < steps) {
+        slot(P0, P2);
+        slot(P1, P0);
+        slot(P2, P1);
+      }
+      if (it < steps) slot(P0, P2);
+      if (it < steps) slot(P1, P0);
+    } else {
+      while (it + 3 < steps) {
+        slot(P0, P3);
+        slot(P1, P0);
+        slot(P2, P1);
+        slot(P3, P2);
+      }
+      if (it < steps) slot(P0, P3);
+      if (it < steps) slot(P1, P0);
+      if (it < steps) slot(P2, P1);
     }
-    k += GS;
-    it++;
-  };
-  // the sets rotate: slot `it` pushes set it % (PUSH_DEPTH + 1) and loads
-  // into set (it + PUSH_DEPTH) % (PUSH_DEPTH + 1)
-  if (PUSH_DEPTH == 1) {
-    while (it + 1 < steps) {
-      slot(P0, P1);
-      slot(P1, P0);
-    }
-    if (it < steps) slot(P0, P1);
-  } else if (PUSH_DEPTH == 2) {
-    while (it + 2 < steps) {
-      slot(P0, P2);
-      slot(P1, P0);
-      slot(P2, P1);
-    }
-    if (it < steps) slot(P0, P2);
-    if (it < steps) slot(P1, P0);
-  } else {
-    while (it + 3 < steps) {
-      slot(P0, P3);
-      slot(P1, P0);
-      slot(P2, P1);
-      slot(P3, P2);
-    }
-    if (it < steps) slot(P0, P3);
-    if (it < steps) slot(P1, P0);
-    if (it < steps) slot(P2, P1);
-  }
-  if (nbad) atomicAdd(cnt + 3, nbad);
-  if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
-  if (qn)
-    drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt, BS);
+    if (nbad) atomicAdd(cnt + 3, nbad);
+    if (jkey >= 0) W.flush_scaled(jkey, jcell, jacc);
+    if (qn)
+      drain_wave_queue(qr, qn, lane, p, W, G, VD, C.qsp, rlist, rcap, cnt, BS);
+  }  // species
   __syncthreads();
   if (keyed && t < 27) Bn.counts[tile * 27 + t] = bin_cnt[t];
   if (Bn.oow && t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
@@ -1557,15 +1610,14 @@ __device__ __forceinline__ void tiled_push(
   }
 }
 
-template <int GS, int MODE>
+template <int GS, int MODE, int NSP>
 __global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_advance_p_tiled(
-    vpic_particle_t *__restrict__ p, int np, const int *__restrict__ part,
-    const vpic_interpolator_t *__restrict__ f0, float *__restrict__ acc,
-    float *__restrict__ wbuf, int wstride, vpic_particle_t *__restrict__ trash,
-    vpic_particle_mover_t *__restrict__ rlist, int rcap, int *__restrict__ cnt,
-    PushCoef C, GridDev G, TileGeom T, BinArgs Bn, OowBuckets OB) {
-  tiled_push<GS, MODE>(blockIdx.x, p, np, part, f0, acc, wbuf, wstride, trash,
-                       rlist, rcap, cnt, C, G, T, Bn, OB);
+    TileSpeciesSet SS, const vpic_interpolator_t *__restrict__ f0,
+    float *__restrict__ acc, float *__restrict__ wbuf, int wstride,
+    vpic_particle_t *__restrict__ trash, GridDev G, TileGeom T, BinArgs Bn,
+    OowBuckets OB) {
+  tiled_push<GS, MODE, NSP>(blockIdx.x, SS.s, f0, acc, wbuf, wstride, trash,
+                            G, T, Bn, OB);
 }
 
 // Add the tiles' windows into the accumulator: voxel v receives the
@@ -1738,8 +1790,7 @@ __global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_ens_push_tiled(
     int nitems, int *__restrict__ queue, vpic_particle_t *__restrict__ trash,
     vpic_particle_mover_t *__restrict__ rlist, int *__restrict__ cnt) {
   __shared__ int s_item;
-  const BinArgs Bn = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                      nullptr};
+  const BinArgs Bn = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
     __syncthreads();
@@ -1748,9 +1799,9 @@ __global__ __launch_bounds__(TILE_BS, TILE_MIN_WAVES) void k_ens_push_tiled(
     if (it >= nitems) break;
     const EnsTileItem I = items[it];
     const EnsTileBox &B = boxes[__builtin_amdgcn_readfirstlane(I.box)];
-    tiled_push<16, 0>(__builtin_amdgcn_readfirstlane(I.tile), B.p, B.np,
-                      B.part, B.f0, B.acc, B.wbuf, B.wstride, trash, rlist, 0,
-                      cnt, B.C, B.G, B.T, Bn, B.OB);
+    const TileSpecies S = {B.p, B.part, rlist, cnt, nullptr, B.C, B.np, 0};
+    tiled_push<16, 0, 1>(__builtin_amdgcn_readfirstlane(I.tile), &S, B.f0,
+                         B.acc, B.wbuf, B.wstride, trash, B.G, B.T, Bn, B.OB);
     __syncthreads();
   }
 }
@@ -2076,7 +2127,11 @@ struct BinState {
 };
 
 struct PushScratch {
-  DeviceScratch counters, xlist, rlist, sortbuf, wbuf, trash, inject;
+  // per species of one call: counters, crosser list, returned movers and
+  // their ordering buffer
+  DeviceScratch counters[TILE_MAXSP], xlist[TILE_MAXSP], rlist[TILE_MAXSP],
+      sortbuf[TILE_MAXSP];
+  DeviceScratch wbuf, trash, inject;
   DeviceScratch ocnt, orec;  // out-of-window buckets (OowBuckets)
   BinState bins[8];
   int64_t bin_clock = 0;
@@ -2118,11 +2173,9 @@ BinState &bin_state(PushScratch &S, const void *p) {
   return *hit;
 }
 
-typedef void (*TiledKernel)(vpic_particle_t *, int, const int *,
-                            const vpic_interpolator_t *, float *, float *,
-                            int, vpic_particle_t *, vpic_particle_mover_t *,
-                            int, int *, PushCoef, GridDev, TileGeom, BinArgs,
-                            OowBuckets);
+typedef void (*TiledKernel)(TileSpeciesSet, const vpic_interpolator_t *,
+                            float *, float *, int, vpic_particle_t *, GridDev,
+                            TileGeom, BinArgs, OowBuckets);
 
 int tiled_kernel_id(int gs) {
   int id = 0;
@@ -2130,23 +2183,26 @@ int tiled_kernel_id(int gs) {
   return id;  // log2 of the group size (1 .. 64)
 }
 
-template <int MODE>
+template <int MODE, int NSP>
 TiledKernel tiled_kernel_mode(int gs) {
   switch (tiled_kernel_id(gs)) {
-    case 0: return k_advance_p_tiled<1, MODE>;
-    case 1: return k_advance_p_tiled<2, MODE>;
-    case 2: return k_advance_p_tiled<4, MODE>;
-    case 3: return k_advance_p_tiled<8, MODE>;
-    case 4: return k_advance_p_tiled<16, MODE>;
-    case 5: return k_advance_p_tiled<32, MODE>;
-    default: return k_advance_p_tiled<64, MODE>;
+    case 0: return k_advance_p_tiled<1, MODE, NSP>;
+    case 1: return k_advance_p_tiled<2, MODE, NSP>;
+    case 2: return k_advance_p_tiled<4, MODE, NSP>;
+    case 3: return k_advance_p_tiled<8, MODE, NSP>;
+    case 4: return k_advance_p_tiled<16, MODE, NSP>;
+    case 5: return k_advance_p_tiled<32, MODE, NSP>;
+    default: return k_advance_p_tiled<64, MODE, NSP>;
   }
 }
 
-TiledKernel tiled_kernel(int gs, int mode) {
-  return mode == 0 ? tiled_kernel_mode<0>(gs)
-                   : mode == 1 ? tiled_kernel_mode<1>(gs)
-                               : tiled_kernel_mode<2>(gs);
+// The instance for lane groups of gs, binning state `mode` (one species)
+// or nsp species in one launch (no binning).
+TiledKernel tiled_kernel(int gs, int mode, int nsp = 1) {
+  if (nsp == 2) return tiled_kernel_mode<0, 2>(gs);
+  return mode == 0 ? tiled_kernel_mode<0, 1>(gs)
+                   : mode == 1 ? tiled_kernel_mode<1, 1>(gs)
+                               : tiled_kernel_mode<2, 1>(gs);
 }
 
 // ceil(2^32 / d) for fast_div.
@@ -2368,42 +2424,81 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                      vpic_accumulator_t *a, const vpic_interpolator_t *f,
                      const vpic_grid_t *g, const int *partition, int *keys,
                      hipStream_t s) {
+  const vpic_push_species_t one = {p, np, q, m, pm, max_nm, dev_nm,
+                                   partition, keys};
+  return launch_advance_p_set(1, &one, a, f, g, s);
+}
+
+// advance_p of nsp species into one accumulator.  Species that all have a
+// sort-time partition share one tiled launch (the same windows, staged
+// coefficients and merge, the species pushed in turn by every tile);
+// otherwise each goes on its own.  The tail of each (particles past its
+// partition, listed crossers, returned movers) is its own.
+int launch_advance_p_set(int nsp, const vpic_push_species_t *sp,
+                         vpic_accumulator_t *a, const vpic_interpolator_t *f,
+                         const vpic_grid_t *g, hipStream_t s) {
+  if (nsp < 1 || nsp > TILE_MAXSP)
+    return vpic_set_error(VPIC_EINVAL, "advance_p: %d species in one call "
+                          "(1 to %d)", nsp, TILE_MAXSP);
+  if (nsp > 1) {
+    bool together = g_bin_mode == 0;
+    for (int k = 0; k < nsp; k++)
+      together = together && sp[k].partition && sp[k].np > 0;
+    if (!together) {
+      for (int k = 0; k < nsp; k++) {
+        const int rc = launch_advance_p_set(1, sp + k, a, f, g, s);
+        if (rc) return rc;
+      }
+      return VPIC_OK;
+    }
+  }
   PushScratch &S = push_scratch();
-  const int xcap = crosser_capacity(np);
+  int xcap[TILE_MAXSP], *cnt[TILE_MAXSP];
+  vpic_particle_mover_t *xlist[TILE_MAXSP], *rlist[TILE_MAXSP];
+  PushCoef C[TILE_MAXSP];
+  int64_t np_total = 0;
   int rc;
-  if ((rc = scratch_reserve(S.counters, 64)) ||
-      (rc = scratch_reserve(S.xlist, sizeof(vpic_particle_mover_t) * xcap)) ||
-      (rc = scratch_reserve(S.rlist, sizeof(vpic_particle_mover_t) * xcap)) ||
-      (rc = scratch_reserve(S.sortbuf, sizeof(int2) * pow2_at_least(xcap))))
-    return rc;
-  int *cnt = (int *)S.counters.ptr;
-  vpic_particle_mover_t *xlist = (vpic_particle_mover_t *)S.xlist.ptr;
-  vpic_particle_mover_t *rlist = (vpic_particle_mover_t *)S.rlist.ptr;
-
-  const PushCoef C = push_coef(q, m, g);
+  for (int k = 0; k < nsp; k++) {
+    xcap[k] = crosser_capacity(sp[k].np);
+    if ((rc = scratch_reserve(S.counters[k], 64)) ||
+        (rc = scratch_reserve(S.xlist[k],
+                              sizeof(vpic_particle_mover_t) * xcap[k])) ||
+        (rc = scratch_reserve(S.rlist[k],
+                              sizeof(vpic_particle_mover_t) * xcap[k])) ||
+        (rc = scratch_reserve(S.sortbuf[k],
+                              sizeof(int2) * pow2_at_least(xcap[k]))))
+      return rc;
+    cnt[k] = (int *)S.counters[k].ptr;
+    xlist[k] = (vpic_particle_mover_t *)S.xlist[k].ptr;
+    rlist[k] = (vpic_particle_mover_t *)S.rlist[k].ptr;
+    C[k] = push_coef(sp[k].q, sp[k].m, g);
+    np_total += sp[k].np;
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt[k]);
+    VPIC_LAUNCHED("k_clear_counters");
+    // a flat push moves particles without recording their tiles
+    if (!sp[k].partition) push_invalidate_bins(sp[k].p);
+  }
   const GridDev G = make_grid_dev(g);
-
-  hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, cnt);
-  VPIC_LAUNCHED("k_clear_counters");
-  // a flat push moves particles without recording their tiles
-  if (!partition) push_invalidate_bins(p);
-  if (np > 0) {
-    // the timing brackets the main push kernel only: k_advance_p_tiled
-    // when a partition is given, else k_advance_p
-    const bool timed = push_timing_on();
-    const int *flat_start = nullptr;
-    if (partition) {
-      TileGeom T = g_tile;
-      // a tile no larger than the grid (a 2D grid's tiles are one cell
-      // thick; the window then has no halo along that axis either)
-      T.tx = T.tx < g->nx ? T.tx : g->nx;
-      T.ty = T.ty < g->ny ? T.ty : g->ny;
-      T.tz = T.tz < g->nz ? T.tz : g->nz;
-      T.ntx = (g->nx + T.tx - 1) / T.tx;
-      T.nty = (g->ny + T.ty - 1) / T.ty;
-      T.ntz = (g->nz + T.tz - 1) / T.tz;
-      const unsigned sx1 = g->nx + 2, sy1 = g->ny + 2;
-      const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
+  // the timing brackets the main push kernel only: k_advance_p_tiled when
+  // a partition is given, else k_advance_p
+  const bool timed = push_timing_on();
+  const int *flat_start[TILE_MAXSP] = {};
+  if (sp[0].partition && np_total > 0) {
+    // (one species here, or several that all have partitions)
+    vpic_particle_t *p = sp[0].p;
+    const int np = sp[0].np;
+    const int *partition = sp[0].partition;
+    TileGeom T = g_tile;
+    // a tile no larger than the grid (a 2D grid's tiles are one cell
+    // thick; the window then has no halo along that axis either)
+    T.tx = T.tx < g->nx ? T.tx : g->nx;
+    T.ty = T.ty < g->ny ? T.ty : g->ny;
+    T.tz = T.tz < g->nz ? T.tz : g->nz;
+    T.ntx = (g->nx + T.tx - 1) / T.tx;
+    T.nty = (g->ny + T.ty - 1) / T.ty;
+    T.ntz = (g->nz + T.tz - 1) / T.tz;
+    const unsigned sx1 = g->nx + 2, sy1 = g->ny + 2;
+    const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
       T.mx = magic_ok ? div_magic(sx1) : 0;
       T.my = magic_ok ? div_magic(sy1) : 0;
       tile_axis_halos(T, g);
@@ -2428,13 +2523,14 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
       if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
         return rc;
       // Sub-buckets for out-of-window deposits: 27 per tile of 1/32 of a
-      // tile's average particle count each (late in a sort interval about
-      // a tenth of the electrons' deposits leave their window, mostly
-      // across the six faces), 64 to 4096 records.  Every tiled push writes
-      // all 27 counts of every tile, so they need no clearing.
+      // tile's average particle count (all species) each (late in a sort
+      // interval about a tenth of the electrons' deposits leave their
+      // window, mostly across the six faces), 64 to 4096 records.  Every
+      // tiled push writes all 27 counts of every tile, so they need no
+      // clearing.
       OowBuckets OB;
       {
-        int64_t cap = (int64_t)np / nblk / 32;
+        int64_t cap = np_total / nblk / 32;
         cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
         if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 27 * (size_t)nblk)) ||
             (rc = scratch_reserve(S.orec,
@@ -2444,20 +2540,19 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         OB.rec = (float4 *)S.orec.ptr;
         OB.cap = (int)cap;
       }
-      static bool attr_set[7] = {false};
+      static bool attr_set[TILE_MAXSP][7] = {};
       const int ak = tiled_kernel_id(g_group);
-      if (!attr_set[ak]) {
-        for (int mode = 0; mode < 3; mode++)
+      if (!attr_set[nsp - 1][ak]) {
+        for (int mode = 0; mode < (nsp == 1 ? 3 : 1); mode++)
           VPIC_HIP(hipFuncSetAttribute(
-              reinterpret_cast<const void *>(tiled_kernel(g_group, mode)),
+              reinterpret_cast<const void *>(tiled_kernel(g_group, mode, nsp)),
               hipFuncAttributeMaxDynamicSharedMemorySize, TILE_DYN_BYTES));
-        attr_set[ak] = true;
+        attr_set[nsp - 1][ak] = true;
       }
       // Tile lists from the codes of the last tiled push of this array.
-      BinArgs BA = {nullptr, nullptr, nullptr, nullptr,
-                    nullptr, nullptr, keys};
+      BinArgs BA = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
       BinState *BSt = nullptr;
-      if (g_bin_mode > 0) {
+      if (g_bin_mode > 0 && nsp == 1) {
         BinState &B = bin_state(S, p);
         const int nt = nblk;
         const int nsum = (nt + 1 + 2047) / 2048 + 1;
@@ -2515,18 +2610,21 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
         BA.oow = ctl + 4;
         BSt = &B;
       }
-      if (timed) push_timing_record(s, np, true);
+      if (timed) push_timing_record(s, np_total, true);
       // without binning only the mode-0 instance; with it all three, of
       // which the one matching the device's binning state runs
+      TileSpeciesSet SS = {};
+      for (int k = 0; k < nsp; k++)
+        SS.s[k] = {sp[k].p, sp[k].partition, rlist[k], cnt[k], sp[k].keys,
+                   C[k],    sp[k].np,        xcap[k]};
       for (int mode = 0; mode < (BSt ? 3 : 1); mode++) {
-        hipLaunchKernelGGL(tiled_kernel(g_group, mode), dim3(nblk),
-                           dim3(TILE_BS), lds, s, p, np, partition, f,
-                           (float *)a, (float *)S.wbuf.ptr, wstride,
-                           (vpic_particle_t *)S.trash.ptr, rlist, xcap, cnt,
-                           C, G, T, BA, OB);
+        hipLaunchKernelGGL(tiled_kernel(g_group, mode, nsp), dim3(nblk),
+                           dim3(TILE_BS), lds, s, SS, f, (float *)a,
+                           (float *)S.wbuf.ptr, wstride,
+                           (vpic_particle_t *)S.trash.ptr, G, T, BA, OB);
         VPIC_LAUNCHED("k_advance_p_tiled");
       }
-      if (timed) push_timing_record(s, np, false);
+      if (timed) push_timing_record(s, np_total, false);
       if (BSt) {
         hipLaunchKernelGGL(k_bin_post, dim3(1), dim3(64), 0, s,
                            (int *)BSt->ctl.ptr, partition, g->nv, np);
@@ -2538,27 +2636,32 @@ int launch_advance_p(vpic_particle_t *p, int np, float q, float m,
                          OB);
       VPIC_LAUNCHED("k_merge_windows");
       // particles past the partition (added since the sort) go flat
-      flat_start = partition + g->nv;
-    }
-    {
+      for (int k = 0; k < nsp; k++) flat_start[k] = sp[k].partition + g->nv;
+  }
+  for (int k = 0; k < nsp; k++) {
+    const int np = sp[k].np;
+    if (np > 0) {
       int64_t want = ((int64_t)np + 255) / 256;
       const int blocks = (int)(want < 16384 ? want : 16384);
-      if (timed && !partition) push_timing_record(s, np, true);
-      hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, p,
-                         flat_start, np, f, (float *)a, xlist, xcap, rlist,
-                         cnt, C, G, keys);
+      const bool flat_timed = timed && !sp[k].partition;
+      if (flat_timed) push_timing_record(s, np, true);
+      hipLaunchKernelGGL(k_advance_p, dim3(blocks), dim3(256), 0, s, sp[k].p,
+                         flat_start[k], np, f, (float *)a, xlist[k], xcap[k],
+                         rlist[k], cnt[k], C[k], G, sp[k].keys);
       VPIC_LAUNCHED("k_advance_p");
-      if (timed && !partition) push_timing_record(s, np, false);
+      if (flat_timed) push_timing_record(s, np, false);
+      int mblocks = (xcap[k] + 255) / 256;
+      if (mblocks > 8192) mblocks = 8192;
+      hipLaunchKernelGGL(k_move_p_list, dim3(mblocks), dim3(256), 0, s,
+                         sp[k].p, xlist[k], xcap[k], rlist[k], (float *)a,
+                         cnt[k], sp[k].q, G, sp[k].keys);
+      VPIC_LAUNCHED("k_move_p_list");
     }
-    int mblocks = (xcap + 255) / 256;
-    if (mblocks > 8192) mblocks = 8192;
-    hipLaunchKernelGGL(k_move_p_list, dim3(mblocks), dim3(256), 0, s, p,
-                       xlist, xcap, rlist, (float *)a, cnt, q, G, keys);
-    VPIC_LAUNCHED("k_move_p_list");
+    hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, sp[k].p,
+                       rlist[k], xcap[k], (int2 *)S.sortbuf[k].ptr, sp[k].pm,
+                       sp[k].max_nm, cnt[k], sp[k].dev_nm);
+    VPIC_LAUNCHED("k_order_movers");
   }
-  hipLaunchKernelGGL(k_order_movers, dim3(1), dim3(1024), 0, s, p, rlist,
-                     xcap, (int2 *)S.sortbuf.ptr, pm, max_nm, cnt, dev_nm);
-  VPIC_LAUNCHED("k_order_movers");
   return VPIC_OK;
 }
 

@@ -129,25 +129,48 @@ class Simulation:
         self.collisions.append(apply)
         return apply
 
-    def advance_p(self, sp):
-        # after a sort the partition drives the tiled (LDS) push
+    def _push_args(self, sp):
+        """vpic_push_species_t of one species for this step: after a sort
+        its partition drives the tiled (LDS) push, and when the next step
+        starts with its sort_p (advance.cc:25-29) the push leaves it the
+        voxel keys."""
         part = sp.partition.ptr if sp.last_sorted >= 0 else None
+        keys = None
         if sp.sort_interval > 0 and (self.step + 1) % sp.sort_interval == 0:
-            # the next step starts with this species' sort_p
-            # (advance.cc:25-29): the push leaves it the voxel keys
             if sp.keys is None or sp.keys.shape[0] < sp.max_np:
                 sp.keys = device.DeviceArray((max(sp.max_np, 1),), np.int32)
-            _lib.call("vpic_hip_advance_p_keys", sp.p.ptr, sp.np, sp.q,
-                      sp.m, sp.pm.ptr, sp.max_nm, sp.dev_nm.ptr,
-                      self.acc.ptr, self.interp.ptr, self._G(), part,
-                      sp.keys.ptr, self.stream)
+            keys = sp.keys.ptr
             sp.keys_step = self.step
         else:
-            _lib.call("vpic_hip_advance_p_async", sp.p.ptr, sp.np, sp.q,
-                      sp.m, sp.pm.ptr, sp.max_nm, sp.dev_nm.ptr,
-                      self.acc.ptr, self.interp.ptr, self._G(), part,
-                      self.stream)
             sp.keys_step = -1
+        return T.PushSpecies(sp.p.ptr, sp.np, sp.q, sp.m, sp.pm.ptr,
+                             sp.max_nm, sp.dev_nm.ptr, part, keys)
+
+    def advance_p(self, sp):
+        a = self._push_args(sp)
+        _lib.call("vpic_hip_advance_p_keys", a.p, a.np, a.q, a.m, a.pm,
+                  a.max_nm, a.dev_nm, self.acc.ptr, self.interp.ptr,
+                  self._G(), a.partition, a.keys, self.stream)
+
+    def advance_species(self):
+        """advance_p of every species into the accumulator (advance.cc:
+        56-62): species with a partition go two to a call, which pushes
+        both through the same tiles in one launch."""
+        pending = []
+        for sp in self.species:
+            if sp.last_sorted < 0 or sp.np == 0:
+                self.advance_p(sp)
+                continue
+            pending.append(sp)
+            if len(pending) == 2:
+                arr = (T.PushSpecies * 2)(*[self._push_args(x)
+                                            for x in pending])
+                _lib.call("vpic_hip_advance_p_species", 2, arr,
+                          self.acc.ptr, self.interp.ptr, self._G(),
+                          self.stream)
+                pending = []
+        for sp in pending:
+            self.advance_p(sp)
 
     def load_interpolator_array(self):
         _lib.call("vpic_hip_load_interpolator_array", self.interp.ptr,
@@ -409,8 +432,7 @@ class Simulation:
             op(self)
         with t("push"):
             self.clear_accumulator_array()
-            for sp in self.species:
-                self.advance_p(sp)
+            self.advance_species()
             # advance.cc:70-77: particles that hit a particle boundary
             if any(self.g.pbc[k] != T.BC_PERIODIC for k in range(6)):
                 self.boundary_p()

@@ -84,6 +84,15 @@ HYDRO = np.dtype([(n, np.float32) for n in HYDRO_FIELDS] +
                  [("pad0", np.float32), ("pad1", np.float32)])
 
 
+class PushSpecies(ctypes.Structure):
+    """vpic_push_species_t: one species of vpic_hip_advance_p_species."""
+    _fields_ = [("p", ctypes.c_void_p), ("np", ctypes.c_int32),
+                ("q", ctypes.c_float), ("m", ctypes.c_float),
+                ("pm", ctypes.c_void_p), ("max_nm", ctypes.c_int32),
+                ("dev_nm", ctypes.c_void_p), ("partition", ctypes.c_void_p),
+                ("keys", ctypes.c_void_p)]
+
+
 class DumpInfo(ctypes.Structure):
     """vpic_dump_info_t: step, the domain's low corner, rank / nproc."""
     _fields_ = [("step", ctypes.c_int64), ("x0", ctypes.c_float),
