@@ -2525,13 +2525,19 @@ int launch_advance_p_set(int nsp, const vpic_push_species_t *sp,
       // Sub-buckets for out-of-window deposits: 27 per tile of 1/32 of a
       // tile's average particle count (all species) each (late in a sort
       // interval about a tenth of the electrons' deposits leave their
-      // window, mostly across the six faces), 64 to 4096 records.  Every
-      // tiled push writes all 27 counts of every tile, so they need no
-      // clearing.
+      // window, mostly across the six faces), 64 to 4096 records; a tile's
+      // whole count when that takes at most 256 MiB (small grids, whose
+      // narrow or periodic-edge tiles can send most of their deposits to
+      // one neighbour).  A deposit that finds its sub-bucket full takes a
+      // float atomic.  Every tiled push writes all 27 counts of every
+      // tile, so they need no clearing.
       OowBuckets OB;
       {
-        int64_t cap = np_total / nblk / 32;
+        const int64_t per_tile = np_total / nblk + 1;
+        int64_t cap = per_tile / 32;
         cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
+        if (64 * 27 * per_tile * nblk <= ((int64_t)256 << 20) && per_tile > cap)
+          cap = per_tile;
         if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 27 * (size_t)nblk)) ||
             (rc = scratch_reserve(S.orec,
                                   64 * 27 * (size_t)cap * (size_t)nblk)))

@@ -420,6 +420,19 @@ class Simulation:
                   self._G(), ctypes.byref(self._info()), bc.ctypes.data,
                   rng.ctypes.data, nb.ctypes.data)
 
+    # -- checkpoint / restart (checkpt.h:101-110) ----------------------------
+    def checkpoint(self, path):
+        """Write the state a restarted run continues from
+        (vpic_amd.checkpoint)."""
+        from . import checkpoint
+        checkpoint.save(self, path)
+
+    @staticmethod
+    def restore(path):
+        """The Simulation saved at `path` by checkpoint()."""
+        from . import checkpoint
+        return checkpoint.load(path)
+
     # -- one time step (advance.cc:15-207) ----------------------------------
     def advance(self, timer=None):
         t = timer or (lambda name: _NullCtx())
