@@ -177,13 +177,16 @@ def cpu_baseline(s, budget_s=12.0):
     return out
 
 
-def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3,
+def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=None,
                    sort_interval=20):
     """The batched ensemble: nbox counter-streaming beam boxes drawn from
     the seed (sides from {32,64,96,128}, 16-256 ppc, drift 0.05-0.3c,
     spread 0.01c, immobile ions), every box sorted every sort_interval
-    steps, tuned, then `steps` graph-replayed steps timed; runs/s and
-    aggregate particle-steps/s."""
+    steps, tuned (each trial a whole sort interval from a sort, by
+    default, so that the drift between sorts counts), then `steps`
+    graph-replayed steps timed; runs/s and aggregate particle-steps/s."""
+    if tune_steps is None:
+        tune_steps = sort_interval if sort_interval > 0 else 3
     import torch
     from vpic_amd import ensemble as E
     cfg = E.beam_config(nbox, seed)
@@ -210,7 +213,8 @@ def ensemble_probe(nbox=256, seed=2026, steps=500, tune_steps=3,
             "tuned_shape": {"push_bs": shape[0], "push_chunk": shape[1],
                             "field_bs": shape[2], "field_chunk": shape[3],
                             "blocks_per_cu": shape[4],
-                            "tile": list(tile)},
+                            "tile": list(tile[:3]),
+                            "window_halo": tile[3]},
             "sort_interval": sort_interval,
             "config": "256 2D periodic counter-streaming electron-beam boxes, "
                       "seed %d, hipGraph-replayed step, sort_p of every box "

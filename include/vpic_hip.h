@@ -788,6 +788,14 @@ int vpic_hip_ensemble_sort_interval(vpic_ensemble_t *e, int32_t interval);
 int vpic_hip_ensemble_tile(vpic_ensemble_t *e, int32_t tx, int32_t ty,
                            int32_t tz);
 int vpic_hip_ensemble_get_tile(const vpic_ensemble_t *e, int32_t out[3]);
+/* The same with the tiles' LDS window halo (1 to 3 cells; none along an
+ * axis of one cell, so a 2D box's window grows only in-plane):
+ * vpic_hip_ensemble_tile sets halo 1.  out[3] of the getter is the halo.
+ * The tuner tries halos 1 to 3 on flat tiles. */
+int vpic_hip_ensemble_tile_window(vpic_ensemble_t *e, int32_t tx, int32_t ty,
+                                  int32_t tz, int32_t halo);
+int vpic_hip_ensemble_get_tile_window(const vpic_ensemble_t *e,
+                                      int32_t out[4]);
 /* The tuner: times each candidate shape over trial_steps graph replays from
  * the same saved state (restored afterwards), keeps the fastest and
  * returns its ms per step. */

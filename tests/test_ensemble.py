@@ -59,8 +59,9 @@ def _small_boxes(seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tile", [None, (0, 0, 0), (2, 2, 1)],
-                         ids=["tile-default", "flat", "tile-2x2x1"])
+@pytest.mark.parametrize("tile", [None, (0, 0, 0), (2, 2, 1), (3, 2, 1, 3)],
+                         ids=["tile-default", "flat", "tile-2x2x1",
+                              "tile-3x2x1-halo3"])
 def test_ensemble_boxes_match_single_box_oracle(orc, tile):
     import torch
     from oracle.oracle import host_grid

@@ -116,6 +116,8 @@ _SIGS = {
     "vpic_hip_ensemble_sort_interval": [_vp, _i32],
     "vpic_hip_ensemble_tile": [_vp, _i32, _i32, _i32],
     "vpic_hip_ensemble_get_tile": [_vp, _vp],
+    "vpic_hip_ensemble_tile_window": [_vp, _i32, _i32, _i32, _i32],
+    "vpic_hip_ensemble_get_tile_window": [_vp, _vp],
     "vpic_hip_push_timing_enable": [ctypes.c_int],
     "vpic_hip_push_timing": [_pi64, _pf64, _pi64],
 }

@@ -184,7 +184,11 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
                      const int *np, const float *q, const float *m,
                      const int64_t *voff, const int64_t *poff,
                      vpic_particle_t *P, const vpic_interpolator_t *FI,
-                     vpic_accumulator_t *A, int tx, int ty, int tz);
+                     vpic_accumulator_t *A, int tx, int ty, int tz,
+                     int halo);
+// Largest window halo of an ensemble's tiles (its boxes are often 2D, where
+// a wide halo costs little LDS).
+constexpr int ENS_MAX_HALO = 3;
 void ens_tiled_destroy(EnsTiled *t);
 int *ens_tiled_epart(EnsTiled *t);
 int *ens_tiled_counters(EnsTiled *t);

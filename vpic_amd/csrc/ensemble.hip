@@ -52,6 +52,7 @@ struct Shape {
 // flat push.
 struct TileShape {
   int tx, ty, tz;
+  int halo;  // cells of the LDS window beyond the tile (1 to ENS_MAX_HALO)
 };
 
 }  // namespace
@@ -86,7 +87,7 @@ struct vpic_ensemble {
   hipGraph_t sgraph = nullptr;
   hipGraphExec_t sexec = nullptr;
   // the tiled push (after a first sort: it reads the sort's partitions)
-  TileShape tile = {16, 8, 4};
+  TileShape tile = {16, 8, 4, 1};
   EnsTiled *tiled = nullptr;
   bool sorted_once = false;
   bool graph_tiled = false;  // the captured step uses the tiled push
@@ -159,7 +160,7 @@ int ensure_tiled(vpic_ensemble *e) {
   return ens_tiled_create(&e->tiled, e->nbox, e->grids.data(), e->np.data(),
                           e->q.data(), e->m.data(), e->voff.data(),
                           e->poff.data(), e->P, e->FI, e->A, e->tile.tx,
-                          e->tile.ty, e->tile.tz);
+                          e->tile.ty, e->tile.tz, e->tile.halo);
 }
 
 int enqueue_sort(vpic_ensemble *e, hipStream_t s) {
@@ -515,20 +516,27 @@ extern "C" int vpic_hip_ensemble_shape(vpic_ensemble_t *e, int32_t push_bs,
   return build_items(e);
 }
 
-extern "C" int vpic_hip_ensemble_tile(vpic_ensemble_t *e, int32_t tx,
-                                      int32_t ty, int32_t tz) {
+extern "C" int vpic_hip_ensemble_tile_window(vpic_ensemble_t *e, int32_t tx,
+                                             int32_t ty, int32_t tz,
+                                             int32_t halo) {
   if (!e || tx < 0 || ty < 0 || tz < 0 || (tx == 0) != (ty == 0) ||
-      (tx == 0) != (tz == 0))
+      (tx == 0) != (tz == 0) || halo < 1 || halo > ENS_MAX_HALO)
     return vpic_set_error(VPIC_EINVAL, "ensemble_tile: bad arguments");
-  if (tx == e->tile.tx && ty == e->tile.ty && tz == e->tile.tz)
+  if (tx == e->tile.tx && ty == e->tile.ty && tz == e->tile.tz &&
+      halo == e->tile.halo)
     return VPIC_OK;
   VPIC_HIP(hipStreamSynchronize(e->stream));
   if (e->tiled) ens_tiled_destroy(e->tiled);
   e->tiled = nullptr;
   e->sorted_once = false;
-  e->tile = {tx, ty, tz};
+  e->tile = {tx, ty, tz, halo};
   drop_graph(e);
   return VPIC_OK;
+}
+
+extern "C" int vpic_hip_ensemble_tile(vpic_ensemble_t *e, int32_t tx,
+                                      int32_t ty, int32_t tz) {
+  return vpic_hip_ensemble_tile_window(e, tx, ty, tz, 1);
 }
 
 extern "C" int vpic_hip_ensemble_get_tile(const vpic_ensemble_t *e,
@@ -537,6 +545,16 @@ extern "C" int vpic_hip_ensemble_get_tile(const vpic_ensemble_t *e,
   out[0] = e->tile.tx;
   out[1] = e->tile.ty;
   out[2] = e->tile.tz;
+  return VPIC_OK;
+}
+
+extern "C" int vpic_hip_ensemble_get_tile_window(const vpic_ensemble_t *e,
+                                                 int32_t out[4]) {
+  if (!e || !out) return vpic_set_error(VPIC_EINVAL, "bad arguments");
+  out[0] = e->tile.tx;
+  out[1] = e->tile.ty;
+  out[2] = e->tile.tz;
+  out[3] = e->tile.halo;
   return VPIC_OK;
 }
 
@@ -581,9 +599,14 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   };
   // (a box one cell thick along an axis has no window halo along it, so
   // the flat shapes' windows are the tile and a one-cell ring in-plane)
-  static const TileShape tiles[] = {{32, 16, 1}, {64, 8, 1}, {16, 16, 1},
-                                    {32, 8, 1},  {16, 8, 1}, {8, 8, 1},
-                                    {16, 8, 4},  {7, 8, 7}};
+  // (and wider halos, which on flat boxes keep the deposits of particles
+  // that drifted further since the sort in LDS; a shape whose window does
+  // not fit some box is skipped)
+  static const TileShape tiles[] = {
+      {32, 16, 1, 1}, {32, 16, 1, 2}, {32, 16, 1, 3}, {64, 8, 1, 1},
+      {64, 8, 1, 2},  {16, 16, 1, 1}, {16, 16, 1, 2}, {16, 16, 1, 3},
+      {32, 8, 1, 1},  {32, 8, 1, 2},  {16, 8, 1, 1},  {8, 8, 1, 1},
+      {16, 8, 4, 1},  {7, 8, 7, 1}};
   const int ncand = sizeof(cands) / sizeof(cands[0]);
   const int ntile = sizeof(tiles) / sizeof(tiles[0]);
   void *saveP = nullptr, *saveF = nullptr, *saveFI = nullptr;
@@ -616,7 +639,7 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   };
   auto set_tile = [&](const TileShape &t) -> int {
     if (t.tx == e->tile.tx && t.ty == e->tile.ty && t.tz == e->tile.tz &&
-        (e->tiled || t.tx <= 0))
+        t.halo == e->tile.halo && (e->tiled || t.tx <= 0))
       return VPIC_OK;
     (void)hipStreamSynchronize(s);
     if (e->tiled) ens_tiled_destroy(e->tiled);
@@ -650,7 +673,8 @@ extern "C" int vpic_hip_ensemble_tune(vpic_ensemble_t *e, int32_t trial_steps,
   // the tiles (the tiled push needs the sort's partitions)
   for (int c = 0; c < (interval0 > 0 ? ntile : 0) && rc == VPIC_OK; c++) {
     float ms = 0;
-    if ((rc = set_tile(tiles[c])) || (rc = trial(ms))) break;
+    if (set_tile(tiles[c]) != VPIC_OK) continue;  // does not fit a box
+    if ((rc = trial(ms))) break;
     if (ms < best_ms) {
       best_ms = ms;
       best_tile = tiles[c];

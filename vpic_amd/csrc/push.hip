@@ -2711,9 +2711,10 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
                      const int *np, const float *q, const float *m,
                      const int64_t *voff, const int64_t *poff,
                      vpic_particle_t *P, const vpic_interpolator_t *FI,
-                     vpic_accumulator_t *A, int tx, int ty, int tz) {
+                     vpic_accumulator_t *A, int tx, int ty, int tz,
+                     int halo) {
   *out = nullptr;
-  if (tx < 1 || ty < 1 || tz < 1)
+  if (tx < 1 || ty < 1 || tz < 1 || halo < 1 || halo > ENS_MAX_HALO)
     return vpic_set_error(VPIC_EINVAL, "ensemble tiles: bad tile shape");
   std::vector<EnsTileBox> hb(nbox);
   std::vector<EnsTileItem> hi;
@@ -2725,7 +2726,7 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
     T.tx = tx < g.nx ? tx : g.nx;
     T.ty = ty < g.ny ? ty : g.ny;
     T.tz = tz < g.nz ? tz : g.nz;
-    T.halo = 1;
+    T.halo = halo;
     tile_axis_halos(T, &g);
     if (T.ty * T.tz > TILE_MAXROWS || T.tx * T.ty * T.tz > TILE_MAXCELLS ||
         96 * tile_window(T) + 72 * (size_t)T.tx * T.ty * T.tz >

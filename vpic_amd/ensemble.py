@@ -109,15 +109,15 @@ class Ensemble:
         never; default 20)."""
         _lib.call("vpic_hip_ensemble_sort_interval", self.h, int(n))
 
-    def tile(self, tx=None, ty=None, tz=None):
-        """The tiled push's tile (cells per axis; 0,0,0: the flat push);
-        returns the tile in use."""
-        cur = (ctypes.c_int32 * 3)()
-        _lib.call("vpic_hip_ensemble_get_tile", self.h, cur)
+    def tile(self, tx=None, ty=None, tz=None, halo=None):
+        """The tiled push's tile (cells per axis; 0,0,0: the flat push) and
+        its window halo; returns (tx, ty, tz, halo) in use."""
+        cur = (ctypes.c_int32 * 4)()
+        _lib.call("vpic_hip_ensemble_get_tile_window", self.h, cur)
         new = [v if v is not None else cur[k] for k, v in
-               enumerate((tx, ty, tz))]
+               enumerate((tx, ty, tz, halo))]
         if new != list(cur):
-            _lib.call("vpic_hip_ensemble_tile", self.h, *new)
+            _lib.call("vpic_hip_ensemble_tile_window", self.h, *new)
         return tuple(new)
 
     def tune(self, trial_steps=3):
