@@ -412,9 +412,16 @@ extern "C" int vpic_hip_advance_p_species(int32_t nsp,
                              g);
     if (rc) return rc;
     CHECK(sp[k].dev_nm, "species %d: dev_nm is NULL", k);
-    for (int j = 0; j < k; j++)
+    for (int j = 0; j < k; j++) {
       CHECK(sp[j].p != sp[k].p || sp[k].np == 0,
             "species %d and %d share a particle array", j, k);
+      CHECK(sp[j].dev_nm != sp[k].dev_nm,
+            "species %d and %d share their dev_nm counters", j, k);
+      CHECK(sp[j].pm != sp[k].pm || sp[k].max_nm == 0,
+            "species %d and %d share a mover array", j, k);
+      CHECK(sp[j].keys != sp[k].keys || !sp[k].keys,
+            "species %d and %d share a key array", j, k);
+    }
   }
   return launch_advance_p_set(nsp, sp, a, f, g, S(stream));
 }
