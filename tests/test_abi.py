@@ -139,6 +139,34 @@ def test_bad_args_rejected():
     assert rc < 0 and b"partner" in lib.vpic_hip_last_error()
 
 
+def test_species_push_args_rejected():
+    """vpic_hip_advance_p_species checks its species before any device
+    work: a count outside 1..2, and two species sharing a per-species
+    output (dev_nm counters, movers, keys)."""
+    lib = _lib.load()
+    g = T.grid_periodic(0, 0, 0, 4, 4, 4, 4, 4, 4, 0.5)
+    v = ctypes.c_void_p
+    arr = (T.PushSpecies * 2)(
+        T.PushSpecies(16, 8, -1.0, 1.0, 4096, 8, 8192, None, None),
+        T.PushSpecies(32, 8, 1.0, 25.0, 4096 * 3, 8, 8192, None, None))
+    rc = lib.vpic_hip_advance_p_species(3, arr, v(64), v(128),
+                                        ctypes.byref(g), None)
+    assert rc < 0 and b"species in one advance_p" in lib.vpic_hip_last_error()
+    rc = lib.vpic_hip_advance_p_species(2, arr, v(64), v(128),
+                                        ctypes.byref(g), None)
+    assert rc < 0 and b"dev_nm" in lib.vpic_hip_last_error()
+    arr[1].dev_nm = 8192 + 64
+    arr[1].pm = 4096
+    rc = lib.vpic_hip_advance_p_species(2, arr, v(64), v(128),
+                                        ctypes.byref(g), None)
+    assert rc < 0 and b"mover array" in lib.vpic_hip_last_error()
+    arr[1].pm = 4096 * 3
+    arr[0].keys = arr[1].keys = 1 << 20
+    rc = lib.vpic_hip_advance_p_species(2, arr, v(64), v(128),
+                                        ctypes.byref(g), None)
+    assert rc < 0 and b"key array" in lib.vpic_hip_last_error()
+
+
 def test_neighbor_table_matches_oracle(orc):
     import numpy as np
     from oracle.oracle import neighbor_table
