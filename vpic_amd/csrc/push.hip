@@ -2499,150 +2499,150 @@ int launch_advance_p_set(int nsp, const vpic_push_species_t *sp,
     T.ntz = (g->nz + T.tz - 1) / T.tz;
     const unsigned sx1 = g->nx + 2, sy1 = g->ny + 2;
     const bool magic_ok = (uint64_t)g->nv * sx1 < ((uint64_t)1 << 32);
-      T.mx = magic_ok ? div_magic(sx1) : 0;
-      T.my = magic_ok ? div_magic(sy1) : 0;
-      tile_axis_halos(T, g);
-      const size_t wcells = tile_window(T);
-      const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
-      const size_t scells = tile_ring(T);
-      if (win_bytes + 72 * (size_t)T.tx * T.ty * T.tz > (size_t)TILE_DYN_BYTES)
-        return vpic_set_error(VPIC_EINVAL,
-                              "advance_p: push tile %dx%dx%d with halo %d "
-                              "needs %zu window cells on this grid, more "
-                              "than the LDS holds", T.tx, T.ty, T.tz, T.halo,
-                              wcells);
-      T.stage_win = win_bytes + 72 * scells <= (size_t)TILE_DYN_BYTES;
-      const size_t lds =
-          win_bytes +
-          72 * (T.stage_win ? scells : (size_t)T.tx * T.ty * T.tz);
-      const int nblk = T.ntx * T.nty * T.ntz;
-      const int wstride = (int)(12 * wcells);
-      if ((rc = scratch_reserve(S.wbuf, sizeof(float) * (size_t)wstride *
-                                            (size_t)nblk)))
+    T.mx = magic_ok ? div_magic(sx1) : 0;
+    T.my = magic_ok ? div_magic(sy1) : 0;
+    tile_axis_halos(T, g);
+    const size_t wcells = tile_window(T);
+    const size_t win_bytes = sizeof(unsigned long long) * 12 * wcells;
+    const size_t scells = tile_ring(T);
+    if (win_bytes + 72 * (size_t)T.tx * T.ty * T.tz > (size_t)TILE_DYN_BYTES)
+      return vpic_set_error(VPIC_EINVAL,
+                            "advance_p: push tile %dx%dx%d with halo %d "
+                            "needs %zu window cells on this grid, more "
+                            "than the LDS holds", T.tx, T.ty, T.tz, T.halo,
+                            wcells);
+    T.stage_win = win_bytes + 72 * scells <= (size_t)TILE_DYN_BYTES;
+    const size_t lds =
+        win_bytes +
+        72 * (T.stage_win ? scells : (size_t)T.tx * T.ty * T.tz);
+    const int nblk = T.ntx * T.nty * T.ntz;
+    const int wstride = (int)(12 * wcells);
+    if ((rc = scratch_reserve(S.wbuf, sizeof(float) * (size_t)wstride *
+                                          (size_t)nblk)))
+      return rc;
+    if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
+      return rc;
+    // Sub-buckets for out-of-window deposits: 27 per tile of 1/32 of a
+    // tile's average particle count (all species) each (late in a sort
+    // interval about a tenth of the electrons' deposits leave their
+    // window, mostly across the six faces), 64 to 4096 records; a tile's
+    // whole count when that takes at most 256 MiB (small grids, whose
+    // narrow or periodic-edge tiles can send most of their deposits to
+    // one neighbour).  A deposit that finds its sub-bucket full takes a
+    // float atomic.  Every tiled push writes all 27 counts of every
+    // tile, so they need no clearing.
+    OowBuckets OB;
+    {
+      const int64_t per_tile = np_total / nblk + 1;
+      int64_t cap = per_tile / 32;
+      cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
+      if (64 * 27 * per_tile * nblk <= ((int64_t)256 << 20) && per_tile > cap)
+        cap = per_tile;
+      if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 27 * (size_t)nblk)) ||
+          (rc = scratch_reserve(S.orec,
+                                64 * 27 * (size_t)cap * (size_t)nblk)))
         return rc;
-      if ((rc = scratch_reserve(S.trash, sizeof(vpic_particle_t) * TILE_BS)))
+      OB.cnt = (int *)S.ocnt.ptr;
+      OB.rec = (float4 *)S.orec.ptr;
+      OB.cap = (int)cap;
+    }
+    static bool attr_set[TILE_MAXSP][7] = {};
+    const int ak = tiled_kernel_id(g_group);
+    if (!attr_set[nsp - 1][ak]) {
+      for (int mode = 0; mode < (nsp == 1 ? 3 : 1); mode++)
+        VPIC_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(tiled_kernel(g_group, mode, nsp)),
+            hipFuncAttributeMaxDynamicSharedMemorySize, TILE_DYN_BYTES));
+      attr_set[nsp - 1][ak] = true;
+    }
+    // Tile lists from the codes of the last tiled push of this array.
+    BinArgs BA = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    BinState *BSt = nullptr;
+    if (g_bin_mode > 0 && nsp == 1) {
+      BinState &B = bin_state(S, p);
+      const int nt = nblk;
+      const int nsum = (nt + 1 + 2047) / 2048 + 1;
+      if ((rc = scratch_reserve(B.key, (size_t)np)) ||
+          (rc = scratch_reserve(B.counts, sizeof(int) * 27 * (size_t)nt)) ||
+          (rc = scratch_reserve(B.base, sizeof(int) * 27 * (size_t)nt)) ||
+          (rc = scratch_reserve(B.total, sizeof(int) * (size_t)(nt + 1))) ||
+          (rc = scratch_reserve(B.sums, sizeof(int) * (size_t)nsum)) ||
+          (rc = scratch_reserve(B.ctl, sizeof(int) * BIN_CTL)) ||
+          (rc = scratch_reserve(B.list[0], sizeof(int) * (size_t)np)) ||
+          (rc = scratch_reserve(B.list[1], sizeof(int) * (size_t)np)) ||
+          (rc = scratch_reserve(B.start[0], sizeof(int) * (size_t)(nt + 1))) ||
+          (rc = scratch_reserve(B.start[1], sizeof(int) * (size_t)(nt + 1))))
         return rc;
-      // Sub-buckets for out-of-window deposits: 27 per tile of 1/32 of a
-      // tile's average particle count (all species) each (late in a sort
-      // interval about a tenth of the electrons' deposits leave their
-      // window, mostly across the six faces), 64 to 4096 records; a tile's
-      // whole count when that takes at most 256 MiB (small grids, whose
-      // narrow or periodic-edge tiles can send most of their deposits to
-      // one neighbour).  A deposit that finds its sub-bucket full takes a
-      // float atomic.  Every tiled push writes all 27 counts of every
-      // tile, so they need no clearing.
-      OowBuckets OB;
-      {
-        const int64_t per_tile = np_total / nblk + 1;
-        int64_t cap = per_tile / 32;
-        cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
-        if (64 * 27 * per_tile * nblk <= ((int64_t)256 << 20) && per_tile > cap)
-          cap = per_tile;
-        if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 27 * (size_t)nblk)) ||
-            (rc = scratch_reserve(S.orec,
-                                  64 * 27 * (size_t)cap * (size_t)nblk)))
-          return rc;
-        OB.cnt = (int *)S.ocnt.ptr;
-        OB.rec = (float4 *)S.orec.ptr;
-        OB.cap = (int)cap;
+      int *ctl = (int *)B.ctl.ptr;
+      if (B.np != np || B.ntiles != nt) {
+        hipLaunchKernelGGL(k_fill_int32, dim3(1), dim3(64), 0, s, ctl,
+                           BIN_CTL, 0);
+        VPIC_LAUNCHED("k_fill_int32");
+        B.np = np;
+        B.ntiles = nt;
+        B.cur = 0;
       }
-      static bool attr_set[TILE_MAXSP][7] = {};
-      const int ak = tiled_kernel_id(g_group);
-      if (!attr_set[nsp - 1][ak]) {
-        for (int mode = 0; mode < (nsp == 1 ? 3 : 1); mode++)
-          VPIC_HIP(hipFuncSetAttribute(
-              reinterpret_cast<const void *>(tiled_kernel(g_group, mode, nsp)),
-              hipFuncAttributeMaxDynamicSharedMemorySize, TILE_DYN_BYTES));
-        attr_set[nsp - 1][ak] = true;
-      }
-      // Tile lists from the codes of the last tiled push of this array.
-      BinArgs BA = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-      BinState *BSt = nullptr;
-      if (g_bin_mode > 0 && nsp == 1) {
-        BinState &B = bin_state(S, p);
-        const int nt = nblk;
-        const int nsum = (nt + 1 + 2047) / 2048 + 1;
-        if ((rc = scratch_reserve(B.key, (size_t)np)) ||
-            (rc = scratch_reserve(B.counts, sizeof(int) * 27 * (size_t)nt)) ||
-            (rc = scratch_reserve(B.base, sizeof(int) * 27 * (size_t)nt)) ||
-            (rc = scratch_reserve(B.total, sizeof(int) * (size_t)(nt + 1))) ||
-            (rc = scratch_reserve(B.sums, sizeof(int) * (size_t)nsum)) ||
-            (rc = scratch_reserve(B.ctl, sizeof(int) * BIN_CTL)) ||
-            (rc = scratch_reserve(B.list[0], sizeof(int) * (size_t)np)) ||
-            (rc = scratch_reserve(B.list[1], sizeof(int) * (size_t)np)) ||
-            (rc = scratch_reserve(B.start[0], sizeof(int) * (size_t)(nt + 1))) ||
-            (rc = scratch_reserve(B.start[1], sizeof(int) * (size_t)(nt + 1))))
-          return rc;
-        int *ctl = (int *)B.ctl.ptr;
-        if (B.np != np || B.ntiles != nt) {
-          hipLaunchKernelGGL(k_fill_int32, dim3(1), dim3(64), 0, s, ctl,
-                             BIN_CTL, 0);
-          VPIC_LAUNCHED("k_fill_int32");
-          B.np = np;
-          B.ntiles = nt;
-          B.cur = 0;
-        }
-        const int nxt = B.cur ^ 1;
-        int *counts = (int *)B.counts.ptr, *base = (int *)B.base.ptr;
-        int *total = (int *)B.total.ptr;
-        int *snext = (int *)B.start[nxt].ptr;
-        hipLaunchKernelGGL(k_bin_decide, dim3(1), dim3(64), 0, s, ctl,
-                           partition, g->nv, np, g_bin_mode);
-        VPIC_LAUNCHED("k_bin_decide");
-        hipLaunchKernelGGL(k_bin_total, dim3((nt + 1 + 255) / 256), dim3(256),
-                           0, s, (const int *)ctl, (const int *)counts, total,
-                           T);
-        VPIC_LAUNCHED("k_bin_total");
-        if ((rc = device_exclusive_scan(total, snext, nt + 1,
-                                        (int *)B.sums.ptr, s)))
-          return rc;
-        hipLaunchKernelGGL(k_bin_base, dim3((nt + 255) / 256), dim3(256), 0,
-                           s, (const int *)ctl, (const int *)counts,
-                           (const int *)snext, base, T);
-        VPIC_LAUNCHED("k_bin_base");
-        hipLaunchKernelGGL(k_bin_scatter, dim3((nt + 3) / 4), dim3(256), 0, s,
-                           ctl, partition, np,
-                           (const int *)B.list[B.cur].ptr,
-                           (const int *)B.start[B.cur].ptr,
-                           (const unsigned char *)B.key.ptr,
-                           (const int *)counts, (const int *)base,
-                           (int *)B.list[nxt].ptr, G, T);
-        VPIC_LAUNCHED("k_bin_scatter");
-        BA.ctl = ctl;
-        BA.list = (const int *)B.list[nxt].ptr;
-        BA.lstart = snext;
-        BA.key = (unsigned char *)B.key.ptr;
-        BA.counts = counts;
-        BA.oow = ctl + 4;
-        BSt = &B;
-      }
-      if (timed) push_timing_record(s, np_total, true);
-      // without binning only the mode-0 instance; with it all three, of
-      // which the one matching the device's binning state runs
-      TileSpeciesSet SS = {};
-      for (int k = 0; k < nsp; k++)
-        SS.s[k] = {sp[k].p, sp[k].partition, rlist[k], cnt[k], sp[k].keys,
-                   C[k],    sp[k].np,        xcap[k]};
-      for (int mode = 0; mode < (BSt ? 3 : 1); mode++) {
-        hipLaunchKernelGGL(tiled_kernel(g_group, mode, nsp), dim3(nblk),
-                           dim3(TILE_BS), lds, s, SS, f, (float *)a,
-                           (float *)S.wbuf.ptr, wstride,
-                           (vpic_particle_t *)S.trash.ptr, G, T, BA, OB);
-        VPIC_LAUNCHED("k_advance_p_tiled");
-      }
-      if (timed) push_timing_record(s, np_total, false);
-      if (BSt) {
-        hipLaunchKernelGGL(k_bin_post, dim3(1), dim3(64), 0, s,
-                           (int *)BSt->ctl.ptr, partition, g->nv, np);
-        VPIC_LAUNCHED("k_bin_post");
-        BSt->cur ^= 1;
-      }
-      hipLaunchKernelGGL(k_merge_windows, dim3(nblk), dim3(256), 0, s,
-                         (float *)a, (const float *)S.wbuf.ptr, wstride, G, T,
-                         OB);
-      VPIC_LAUNCHED("k_merge_windows");
-      // particles past the partition (added since the sort) go flat
-      for (int k = 0; k < nsp; k++) flat_start[k] = sp[k].partition + g->nv;
+      const int nxt = B.cur ^ 1;
+      int *counts = (int *)B.counts.ptr, *base = (int *)B.base.ptr;
+      int *total = (int *)B.total.ptr;
+      int *snext = (int *)B.start[nxt].ptr;
+      hipLaunchKernelGGL(k_bin_decide, dim3(1), dim3(64), 0, s, ctl,
+                         partition, g->nv, np, g_bin_mode);
+      VPIC_LAUNCHED("k_bin_decide");
+      hipLaunchKernelGGL(k_bin_total, dim3((nt + 1 + 255) / 256), dim3(256),
+                         0, s, (const int *)ctl, (const int *)counts, total,
+                         T);
+      VPIC_LAUNCHED("k_bin_total");
+      if ((rc = device_exclusive_scan(total, snext, nt + 1,
+                                      (int *)B.sums.ptr, s)))
+        return rc;
+      hipLaunchKernelGGL(k_bin_base, dim3((nt + 255) / 256), dim3(256), 0,
+                         s, (const int *)ctl, (const int *)counts,
+                         (const int *)snext, base, T);
+      VPIC_LAUNCHED("k_bin_base");
+      hipLaunchKernelGGL(k_bin_scatter, dim3((nt + 3) / 4), dim3(256), 0, s,
+                         ctl, partition, np,
+                         (const int *)B.list[B.cur].ptr,
+                         (const int *)B.start[B.cur].ptr,
+                         (const unsigned char *)B.key.ptr,
+                         (const int *)counts, (const int *)base,
+                         (int *)B.list[nxt].ptr, G, T);
+      VPIC_LAUNCHED("k_bin_scatter");
+      BA.ctl = ctl;
+      BA.list = (const int *)B.list[nxt].ptr;
+      BA.lstart = snext;
+      BA.key = (unsigned char *)B.key.ptr;
+      BA.counts = counts;
+      BA.oow = ctl + 4;
+      BSt = &B;
+    }
+    if (timed) push_timing_record(s, np_total, true);
+    // without binning only the mode-0 instance; with it all three, of
+    // which the one matching the device's binning state runs
+    TileSpeciesSet SS = {};
+    for (int k = 0; k < nsp; k++)
+      SS.s[k] = {sp[k].p, sp[k].partition, rlist[k], cnt[k], sp[k].keys,
+                 C[k], sp[k].np, xcap[k]};
+    for (int mode = 0; mode < (BSt ? 3 : 1); mode++) {
+      hipLaunchKernelGGL(tiled_kernel(g_group, mode, nsp), dim3(nblk),
+                         dim3(TILE_BS), lds, s, SS, f, (float *)a,
+                         (float *)S.wbuf.ptr, wstride,
+                         (vpic_particle_t *)S.trash.ptr, G, T, BA, OB);
+      VPIC_LAUNCHED("k_advance_p_tiled");
+    }
+    if (timed) push_timing_record(s, np_total, false);
+    if (BSt) {
+      hipLaunchKernelGGL(k_bin_post, dim3(1), dim3(64), 0, s,
+                         (int *)BSt->ctl.ptr, partition, g->nv, np);
+      VPIC_LAUNCHED("k_bin_post");
+      BSt->cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_merge_windows, dim3(nblk), dim3(256), 0, s,
+                       (float *)a, (const float *)S.wbuf.ptr, wstride, G, T,
+                       OB);
+    VPIC_LAUNCHED("k_merge_windows");
+    // particles past the partition (added since the sort) go flat
+    for (int k = 0; k < nsp; k++) flat_start[k] = sp[k].partition + g->nv;
   }
   for (int k = 0; k < nsp; k++) {
     const int np = sp[k].np;
