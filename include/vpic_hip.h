@@ -477,6 +477,10 @@ int vpic_hip_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, void *stream);
 
 /* synchronize_jf: local_adjust_jf then the shared-face fold
  * (standard/remote.cc:418-507, standard/local.cc:336-368). */
+/* Waits for all the work queued on the current device (every stream):
+ * for hosts that time or hand over results of the stream-ordered calls. */
+int vpic_hip_device_synchronize(void);
+
 int vpic_hip_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
                             void *stream);
 

@@ -26,6 +26,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <chrono>
 #include <vector>
 
 extern "C" {
@@ -748,7 +749,83 @@ static void check_errors() {
   report(g_handled == 3, "errors reach the handler", "%d of 3", g_handled);
 }
 
+// The reference's step (advance.cc:15-207 for a periodic vacuum box with
+// two species) through the reference-named entry points in resident mode,
+// timed: the drop-in at scale, with no particle crossing PCIe between
+// steps.  Electrons (uth 0.1) and ions (m 25, uth 0.02), ppc each, sorted
+// every 20 steps, density 1; `warm` untimed steps, then `steps` timed.
+static int bench_resident(int nx, int ny, int nz, int ppc, int warm,
+                          int steps) {
+  Box b;
+  make_box(b, nx, ny, nz, 0);
+  Arrays A;
+  make_arrays(A, b, 7);
+  memset(A.f.data(), 0, sizeof(field_t) * A.f.size());
+  if (vpic_hip_install_field_kernels(&A.fa)) {
+    printf("FAIL install: %s\n", vpic_hip_last_error());
+    return 1;
+  }
+  Species sp[2];
+  make_species(sp[0], b, -1.0f, 1.0f, ppc, 0.1f, 71);
+  make_species(sp[1], b, 1.0f, 25.0f, ppc, 0.02f, 72);
+  // weights 1/ppc: density 1, plasma frequency 1, so dt = 0.5 resolves it
+  // (the checks' weights near 1 would give omega_pe dt = 5, unstable)
+  for (int s = 0; s < 2; s++)
+    for (int k = 0; k < sp[s].sp.np; k++) sp[s].p[k].w = 1.0f / ppc;
+  field_advance_kernels_t *k = A.fa.kernel;
+  vpic_hip_dropin_resident(1);
+  load_interpolator_array(&A.ia, &A.fa);
+  auto step = [&](int n) {
+    for (int s = 0; s < 2; s++)
+      if (n % sp[s].sp.sort_interval == 0) sort_p(&sp[s].sp);
+    clear_accumulator_array(&A.aa);
+    for (int s = 0; s < 2; s++) advance_p(&sp[s].sp, &A.aa, &A.ia);
+    reduce_accumulator_array(&A.aa);
+    k->clear_jf(&A.fa);
+    unload_accumulator_array(&A.fa, &A.aa);
+    k->synchronize_jf(&A.fa);
+    k->advance_b(&A.fa, 0.5f);
+    k->advance_e(&A.fa, 1.0f);
+    k->advance_b(&A.fa, 0.5f);
+    load_interpolator_array(&A.ia, &A.fa);
+  };
+  for (int n = 0; n < warm; n++) step(n);
+  vpic_hip_device_synchronize();
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int n = warm; n < warm + steps; n++) step(n);
+  vpic_hip_device_synchronize();
+  const double el = std::chrono::duration<double>(
+                        std::chrono::steady_clock::now() - t0).count();
+  const double np = (double)sp[0].sp.np + sp[1].sp.np;
+  int lost = 0;
+  for (int s = 0; s < 2; s++) lost += sp[s].sp.nm;
+  vpic_hip_dropin_resident(0);  // downloads the device state
+  double umax = 0, emax = 0;
+  for (int k = 0; k < sp[0].sp.np; k++) {
+    const particle_t &q = sp[0].p[k];
+    umax = fmax(umax, sqrt((double)q.ux * q.ux + (double)q.uy * q.uy +
+                           (double)q.uz * q.uz));
+  }
+  for (size_t v = 0; v < A.f.size(); v++)
+    emax = fmax(emax, fabs((double)A.f[v].ex));
+  vpic_hip_dropin_release_all();
+  printf("dropin resident: %dx%dx%d cells, %.0f particles, %d steps, "
+         "%.4e particle-steps/s, %.3f ms/step, movers %d, electron |u| max "
+         "%.3g, |ex| max %.3g\n", nx, ny, nz, np, steps, np * steps / el,
+         el * 1e3 / steps, lost, umax, emax);
+  return lost == 0 ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
+  if (argc > 1 && strcmp(argv[1], "--bench") == 0) {
+    int a[6] = {128, 64, 64, 100, 5, 40};  // nx ny nz ppc warm steps
+    for (int k = 0; k < 6 && k + 2 < argc; k++) a[k] = atoi(argv[k + 2]);
+    if (vpic_hip_init(0) != 0) {
+      printf("FAIL init: %s\n", vpic_hip_last_error());
+      return 1;
+    }
+    return bench_resident(a[0], a[1], a[2], a[3], a[4], a[5]);
+  }
   check_layout();
   if (argc > 1 && strcmp(argv[1], "--layout") == 0) return g_fail;
   if (vpic_hip_init(0) != 0) {

@@ -35,6 +35,7 @@ _SIGS = {
     "vpic_hip_advance_p_keys": [_vp, _i32, _f32, _f32, _vp, _i32, _vp, _vp,
                                 _vp, _pg, _vp, _vp, _vp],
     "vpic_hip_advance_p_species": [_i32, _vp, _vp, _vp, _pg, _vp],
+    "vpic_hip_device_synchronize": [],
     "vpic_hip_push_tile": [_i32, _i32, _i32],
     "vpic_hip_push_tile_window": [_i32, _i32, _i32, _i32],
     "vpic_hip_push_get_tile": [_vp],

@@ -400,6 +400,11 @@ extern "C" int vpic_hip_takizuka_abe(
                              mj, cvar0, interval, seed, call, g, S(stream));
 }
 
+extern "C" int vpic_hip_device_synchronize(void) {
+  VPIC_HIP(hipDeviceSynchronize());
+  return VPIC_OK;
+}
+
 extern "C" int vpic_hip_advance_p_species(int32_t nsp,
                                           const vpic_push_species_t *sp,
                                           vpic_accumulator_t *a,
