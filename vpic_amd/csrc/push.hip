@@ -1277,6 +1277,9 @@ __device__ __forceinline__ void tiled_push(
   const int snx = x1 - x0 + 1 + 2 * swx, sny = y1 - y0 + 1 + 2 * swy,
             snz = z1 - z0 + 1 + 2 * swz;
   const int sncell = snx * sny * snz;
+  // the staging box is the window itself (a halo of at most one cell
+  // staged whole): a voxel's staged cell is then its window cell
+  const bool stage_is_window = swx == T.hx && swy == T.hy && swz == T.hz;
   float4 *ic_e0 = reinterpret_cast<float4 *>(lds_win + 12 * ncell);
   float4 *ic_e1 = ic_e0 + sncell;
   float4 *ic_e2 = ic_e1 + sncell;
@@ -1473,15 +1476,25 @@ __device__ __forceinline__ void tiled_push(
         int x, y, z;
         VD.coords(G, ii, x, y, z);
         const int wx = x - W.wx0, wy = y - W.wy0, wz = z - W.wz0;
-        fcell = ((unsigned)wx < (unsigned)W.wnx &&
-                 (unsigned)wy < (unsigned)W.wny &&
-                 (unsigned)wz < (unsigned)W.wnz)
-                    ? wx + W.wnx * (wy + W.wny * wz)
+        const bool inw = (unsigned)wx < (unsigned)W.wnx &&
+                         (unsigned)wy < (unsigned)W.wny &&
+                         (unsigned)wz < (unsigned)W.wnz;
+        // (window and staging extents are far below 2^24: 24-bit products)
+        fcell = inw ? wx + (int)__umul24(W.wnx, wy + (int)__umul24(W.wny, wz))
                     : -1;
-        const int sx = x - sx0, sy = y - sy0, sz = z - sz0;
-        if ((unsigned)sx < (unsigned)snx && (unsigned)sy < (unsigned)sny &&
-            (unsigned)sz < (unsigned)snz) {
-          const int c = sx + snx * (sy + sny * sz);
+        bool staged;
+        int c;
+        if (stage_is_window) {
+          staged = inw;
+          c = fcell;
+        } else {
+          const int sx = x - sx0, sy = y - sy0, sz = z - sz0;
+          staged = (unsigned)sx < (unsigned)snx &&
+                   (unsigned)sy < (unsigned)sny &&
+                   (unsigned)sz < (unsigned)snz;
+          c = sx + (int)__umul24(snx, sy + (int)__umul24(sny, sz));
+        }
+        if (staged) {
           F.e0 = ic_e0[c];
           F.e1 = ic_e1[c];
           F.e2 = ic_e2[c];
@@ -1492,13 +1505,12 @@ __device__ __forceinline__ void tiled_push(
         }
         fkey = ii;
       }
-      // prefetch this lane's next particle (issued after the coefficients,
-      // so waiting for them leaves the prefetch in flight), then the list
-      // entry after it
+      // prefetch this lane's next particle (the compiler places the loads;
+      // pinning them right after the coefficient loads measured 0.3%
+      // slower)
       const int kn = k + PUSH_DEPTH * GS;
       pref.ok = particle_of(kn, pref.n);
       load_particle(p, pref.n, pref.ok, pref.A, pref.B);
-      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch issued here
 
       // every lane stores (an idle lane to its own trash slot), so the
       // number of memory operations per iteration does not depend on the
