@@ -109,14 +109,18 @@ __device__ __forceinline__ long long to_fixed(double y) {
 // tile in 27 sub-buckets, one per neighbouring destination tile (offset
 // code (dx+1) + 3(dy+1) + 9(dz+1), tile indices taken periodically).
 // Record k of source tile s, code d, is rec[4 * ((27 s + d) cap + k)] .. +3
-// = {voxel (bits), j0..j2}, {j3..j6}, {j7..j10}, {j11, 0, 0, 0}; cnt[27 s +
-// d] is written at the end of the push.  k_merge_windows of tile t reads,
+// = {voxel (bits), j0..j2}, {j3..j6}, {j7..j10}, {j11, 0, 0, 0}; cnt[2 (27 s
+// + d)] (the count) and cnt[2 (27 s + d) + 1] (the bits of the largest sum of
+// the 12 |j| of its records, a bound on every component, so the reader can
+// fix its fixed-point scale without a pass over the records) are written at
+// the end of the push.  k_merge_windows of tile t reads,
 // for every code d, the sub-bucket d of the tile t - offset(d): each
 // sub-bucket has exactly one reader.  Slots are taken with LDS atomics; a
 // deposit further than one tile away or into a full sub-bucket goes to the
 // accumulator with float atomics.
 struct OowBuckets {
-  int *cnt;     // 27 counts per source tile (may exceed cap)
+  int *cnt;     // 27 (count, magnitude bits) per source tile (count may
+                // exceed cap)
   float4 *rec;  // 27 * cap records per source tile
   int cap;
 };
@@ -145,6 +149,7 @@ struct WindowSink {
   int *oow;         // LDS count of deposits outside the window
   OowBuckets ob;    // where those deposits go
   int *dcnt;        // LDS: this tile's 27 sub-bucket counts
+  unsigned *dmax;   // LDS: and their largest record magnitudes (bits)
   int tx, ty, tz, ntx, nty, ntz;  // tile extent and tiles per axis
   int btx, bty, btz;              // this tile
   int self;                       // its index
@@ -241,6 +246,7 @@ struct WindowSink {
         }
       }
       if (d >= 0 && k < ob.cap) {
+        atomicMax(dmax + d, __float_as_uint(mag));  // mag >= 0, finite
         float4 *r = ob.rec + 4 * ((int64_t)(27 * self + d) * ob.cap + k);
         r[0] = make_float4(__int_as_float(voxel), j[0], j[1], j[2]);
         r[1] = make_float4(j[3], j[4], j[5], j[6]);
@@ -1192,6 +1198,7 @@ __device__ __forceinline__ void tiled_push(
   __shared__ int bin_cnt[27];
   __shared__ int oow_cnt;
   __shared__ int oow_dcnt[27];
+  __shared__ unsigned oow_dmax[27];
 
   const int t = threadIdx.x;
   const int lane = lane_id();
@@ -1230,6 +1237,7 @@ __device__ __forceinline__ void tiled_push(
   W.oow = &oow_cnt;
   W.ob = OB;
   W.dcnt = oow_dcnt;
+  W.dmax = oow_dmax;
   W.tx = T.tx;
   W.ty = T.ty;
   W.tz = T.tz;
@@ -1299,6 +1307,7 @@ __device__ __forceinline__ void tiled_push(
   if (t < 27) bin_cnt[t] = 0;
   if (t == 0) oow_cnt = 0;
   if (t < 27) oow_dcnt[t] = 0;
+  if (t < 27) oow_dmax[t] = 0u;
   __syncthreads();
   // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
   // the first particle of every row of every species.  One particle's
@@ -1610,7 +1619,10 @@ __device__ __forceinline__ void tiled_push(
   __syncthreads();
   if (keyed && t < 27) Bn.counts[tile * 27 + t] = bin_cnt[t];
   if (Bn.oow && t == 0 && oow_cnt) atomicAdd(Bn.oow, oow_cnt);
-  if (OB.cap > 0 && t < 27) OB.cnt[27 * tile + t] = oow_dcnt[t];
+  if (OB.cap > 0 && t < 27) {
+    OB.cnt[2 * (27 * tile + t)] = oow_dcnt[t];
+    OB.cnt[2 * (27 * tile + t) + 1] = (int)oow_dmax[t];
+  }
   // The window, as floats, to this tile's slot (component-major, ncell).
   float *wout = wbuf + (int64_t)tile * wstride;
   const double inv = 1.0 / W.scale;
@@ -1650,7 +1662,6 @@ __device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
                                            const TileGeom &T,
                                            const OowBuckets &OB) {
   __shared__ unsigned long long bsum[12 * TILE_MAXCELLS];
-  __shared__ float bmax[256];
   int b = tile;
   const int bx = b % T.ntx;
   b /= T.ntx;
@@ -1665,20 +1676,24 @@ __device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
   // (this tile - offset(d)), periodic in the tile indices
   __shared__ int sb_n[27];
   __shared__ int64_t sb_at[27];
+  __shared__ float sb_max[27];
   __shared__ int sb_total;
   if (t < 27) {
     int n = 0;
     int64_t at = 0;
+    float mx = 0.0f;
     if (OB.cap > 0) {
       const int ox = t % 3 - 1, oy = (t / 3) % 3 - 1, oz = t / 9 - 1;
       const int sx = (bx - ox + T.ntx) % T.ntx, sy = (by - oy + T.nty) % T.nty,
                 sz = (bz - oz + T.ntz) % T.ntz;
       const int src = sx + T.ntx * (sy + T.nty * sz);
       at = (int64_t)(27 * src + t) * OB.cap;
-      n = min(OB.cnt[27 * src + t], OB.cap);
+      n = min(OB.cnt[2 * (27 * src + t)], OB.cap);
+      mx = __uint_as_float((unsigned)OB.cnt[2 * (27 * src + t) + 1]);
     }
     sb_n[t] = n;
     sb_at[t] = at;
+    sb_max[t] = n > 0 ? mx : 0.0f;
   }
   __syncthreads();
   if (t == 0) {
@@ -1691,29 +1706,14 @@ __device__ __forceinline__ void merge_tile(int tile, float *__restrict__ acc,
   (void)tile;
   double bscale = 0.0;
   if (nrec > 0) {
+    // every component of every record is at most the largest magnitude
+    // the pushing tiles recorded
     float m = 0.0f;
-    for (int d = 0; d < 27; d++) {
-      const float4 *rec = OB.rec + 4 * sb_at[d];
-      for (int r = t; r < sb_n[d]; r += blockDim.x) {
-        const float4 a = rec[4 * r], c = rec[4 * r + 1], e = rec[4 * r + 2],
-                     f = rec[4 * r + 3];
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(a.y), fabsf(a.z)), fabsf(a.w)));
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)),
-                           fmaxf(fabsf(c.z), fabsf(c.w))));
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(e.x), fabsf(e.y)),
-                           fmaxf(fabsf(e.z), fabsf(e.w))));
-        m = fmaxf(m, fabsf(f.x));
-      }
-    }
-    bmax[t] = m;
+    for (int d = 0; d < 27; d++) m = fmaxf(m, sb_max[d]);
     for (int k = t; k < 12 * ncell; k += blockDim.x) bsum[k] = 0ull;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w /= 2) {
-      if (t < w) bmax[t] = fmaxf(bmax[t], bmax[t + w]);
-      __syncthreads();
-    }
     int ex = 0;
-    const double bound = (double)bmax[0] * (double)nrec;
+    const double bound = (double)m * (double)nrec;
     if (bound > 0.0) frexp(bound, &ex);  // every sum is below 2^ex
     bscale = ldexp(1.0, 50 - ex);
     for (int d = 0; d < 27; d++) {
@@ -2550,7 +2550,7 @@ int launch_advance_p_set(int nsp, const vpic_push_species_t *sp,
       cap = cap < 64 ? 64 : cap > 4096 ? 4096 : cap;
       if (64 * 27 * per_tile * nblk <= ((int64_t)256 << 20) && per_tile > cap)
         cap = per_tile;
-      if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 27 * (size_t)nblk)) ||
+      if ((rc = scratch_reserve(S.ocnt, sizeof(int) * 54 * (size_t)nblk)) ||
           (rc = scratch_reserve(S.orec,
                                 64 * 27 * (size_t)cap * (size_t)nblk)))
         return rc;
@@ -2781,7 +2781,7 @@ int ens_tiled_create(EnsTiled **out, int nbox, const vpic_grid_t *grids,
     B.np = np[b];
     B.part = reinterpret_cast<const int *>((intptr_t)(voff[b] + b));
     wtot += (int64_t)ntiles * B.wstride;
-    otot += 27 * (int64_t)ntiles;
+    otot += 54 * (int64_t)ntiles;
     rtot += 4 * 27 * cap * (int64_t)ntiles;
     nvtot += g.nv + 1;
     for (int t = 0; t < ntiles; t++) hi.push_back({b, t});
