@@ -1412,8 +1412,12 @@ __device__ __forceinline__ void tiled_push(
     const int steps = (per + GS - 1) / GS;          // same for every lane
     const int grp = t / GS, j = t % GS;
     const int g0 = min(grp * per, total), g1 = min(g0 + per, total);
-    const vpic_particle_mover_t *qr = wq[t >> 6];
-    vpic_particle_mover_t *q = wq[t >> 6];
+    // this wavefront's crosser queue, indexed by a wave-uniform value so
+    // that its address is a scalar (kept in a vector register, it was
+    // spilled to scratch, and every reload waited for all memory traffic)
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const vpic_particle_mover_t *qr = wq[wv];
+    vpic_particle_mover_t *q = wq[wv];
     int qn = 0;  // entries in this wavefront's queue (wavefront-uniform)
 
     float jacc[12];  // this lane's current, not yet in the window
