@@ -472,6 +472,14 @@ int vpic_hip_unload_accumulator_array(vpic_field_t *f,
 /* clear_jf (standard/sfa.cc:232-237). */
 int vpic_hip_clear_jf(vpic_field_t *f, const vpic_grid_t *g, void *stream);
 
+/* clear_jf followed by unload_accumulator_array, the pair advance.cc:107-109
+ * calls, in one pass over the field records: the same jf bit for bit (the
+ * unload's += on a cleared jf), rhof and every other field kept. */
+int vpic_hip_clear_jf_unload_accumulator_array(vpic_field_t *f,
+                                               const vpic_accumulator_t *a,
+                                               const vpic_grid_t *g,
+                                               void *stream);
+
 /* clear_rhof. */
 int vpic_hip_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, void *stream);
 

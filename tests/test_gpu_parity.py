@@ -257,6 +257,32 @@ def test_field_kernels_bitexact(dev, orc):
         assert np.allclose(np.array(en[:]), eo, rtol=1e-12, atol=0)
 
 
+def test_clear_jf_unload_fused(dev):
+    """vpic_hip_clear_jf_unload_accumulator_array is the reference's
+    clear_jf + unload_accumulator_array pair (advance.cc:107-109) in one
+    pass: every field component bit-identical to the two calls', with
+    nonzero ghost accumulators, rhof and the other fields left as they
+    were."""
+    for name, g in _field_bc_grids():
+        f = H.random_full_fields(g, 5)
+        acc = np.random.default_rng(2).standard_normal((g.nv, 12)).astype(
+            np.float32)
+        G = ctypes.byref(g)
+        d2 = dev.DeviceArray.from_host(f)
+        d1 = dev.DeviceArray.from_host(f)
+        da = dev.DeviceArray.from_host(acc)
+        say("clear_jf + unload fused,", name)
+        _lib.call("vpic_hip_clear_jf", d2.ptr, G, None)
+        _lib.call("vpic_hip_unload_accumulator_array", d2.ptr, da.ptr, G, None)
+        _lib.call("vpic_hip_clear_jf_unload_accumulator_array", d1.ptr,
+                  da.ptr, G, None)
+        want, got = d2.download(), d1.download()
+        for c in T.FIELD_FLOATS:
+            assert H.same_bits(got[c], want[c]), (name, c)
+        assert H.same_bits(got["rhof"], f["rhof"]), name
+        assert not H.same_bits(got["jfx"], f["jfx"]), name
+
+
 CLEANING = ["compute_div_e_err", "compute_rhob", "clean_div_e",
             "compute_curl_b", "compute_div_b_err", "clean_div_b",
             "synchronize_tang_e_norm_b", "rms"]

@@ -58,6 +58,7 @@ _SIGS = {
     "vpic_hip_reduce_accumulator_array": [_vp, _pg, _vp],
     "vpic_hip_unload_accumulator_array": [_vp, _vp, _pg, _vp],
     "vpic_hip_clear_jf": [_vp, _pg, _vp],
+    "vpic_hip_clear_jf_unload_accumulator_array": [_vp, _vp, _pg, _vp],
     "vpic_hip_clear_rhof": [_vp, _pg, _vp],
     "vpic_hip_synchronize_jf": [_vp, _pg, _vp],
     "vpic_hip_synchronize_rho": [_vp, _pg, _vp],

@@ -539,6 +539,15 @@ extern "C" int vpic_hip_unload_accumulator_array(vpic_field_t *f,
   return launch_unload_accumulator(f, a, g, S(stream));
 }
 
+extern "C" int vpic_hip_clear_jf_unload_accumulator_array(
+    vpic_field_t *f, const vpic_accumulator_t *a, const vpic_grid_t *g,
+    void *stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  CHECK(f && a && aligned16(f), "bad array argument");
+  return launch_clear_unload_accumulator(f, a, g, S(stream));
+}
+
 extern "C" int vpic_hip_clear_jf(vpic_field_t *f, const vpic_grid_t *g,
                                  void *stream) {
   int rc = check_grid(g);

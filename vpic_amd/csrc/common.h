@@ -298,6 +298,9 @@ int launch_clear_accumulator(vpic_accumulator_t *a, const vpic_grid_t *g,
 int launch_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
                               const vpic_grid_t *g, hipStream_t s);
 int launch_clear_jf(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
+int launch_clear_unload_accumulator(vpic_field_t *f,
+                                    const vpic_accumulator_t *a,
+                                    const vpic_grid_t *g, hipStream_t s);
 int launch_clear_rhof(vpic_field_t *f, const vpic_grid_t *g, hipStream_t s);
 int launch_synchronize_jf(vpic_field_t *f, const vpic_grid_t *g,
                           hipStream_t s);

@@ -146,19 +146,48 @@ __global__ __launch_bounds__(256) void k_clear_jf(vpic_field_t *f, int nv,
   }
 }
 
+// jf of voxel v (interior or on the upper faces) plus its quarter-face
+// currents (unload_accumulator_pipeline.cc:18-82).
+__device__ __forceinline__ float4 unload_add(
+    float4 jf, const vpic_accumulator_t *__restrict__ a, int v, int sy,
+    int sz, float cx, float cy, float cz) {
+  const vpic_accumulator_t *a0 = a + v, *ax = a + v - 1, *ay = a + v - sy,
+                           *az = a + v - sz, *ayz = a + v - sy - sz,
+                           *azx = a + v - 1 - sz, *axy = a + v - 1 - sy;
+  jf.x += cx * (a0->jx[0] + ay->jx[1] + az->jx[2] + ayz->jx[3]);
+  jf.y += cy * (a0->jy[0] + az->jy[1] + ax->jy[2] + azx->jy[3]);
+  jf.z += cz * (a0->jz[0] + ax->jz[1] + ay->jz[2] + axy->jz[3]);
+  return jf;
+}
+
 __device__ __forceinline__ void unload_accumulator_at(
     vpic_field_t *__restrict__ f, const vpic_accumulator_t *__restrict__ a,
     int x, int y, int z, int nx, int ny, float cx, float cy, float cz) {
   const int v = vox(x, y, z, nx, ny);
   const int sy = nx + 2, sz = (ny + 2) * sy;
-  const vpic_accumulator_t *a0 = a + v, *ax = a + v - 1, *ay = a + v - sy,
-                           *az = a + v - sz, *ayz = a + v - sy - sz,
-                           *azx = a + v - 1 - sz, *axy = a + v - 1 - sy;
-  float4 jf = F4(f, v)[3];
-  jf.x += cx * (a0->jx[0] + ay->jx[1] + az->jx[2] + ayz->jx[3]);
-  jf.y += cy * (a0->jy[0] + az->jy[1] + ax->jy[2] + azx->jy[3]);
-  jf.z += cz * (a0->jz[0] + ax->jz[1] + ay->jz[2] + axy->jz[3]);
-  F4(f, v)[3] = jf;
+  F4(f, v)[3] = unload_add(F4(f, v)[3], a, v, sy, sz, cx, cy, cz);
+}
+
+// clear_jf followed by unload_accumulator_array in one pass over the field
+// records: jf of every voxel set to 0, then, on voxels 1..n+1 per axis,
+// 0 + the unload's sum (the += of the unload on a cleared jf, so the two
+// calls' result bit for bit), rhof kept.
+__global__ __launch_bounds__(256) void k_clear_unload_accumulator(
+    vpic_field_t *__restrict__ f, const vpic_accumulator_t *__restrict__ a,
+    int nv, int nx, int ny, int nz, float cx, float cy, float cz) {
+  const int sy = nx + 2, sz = (ny + 2) * sy;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += gridDim.x * blockDim.x) {
+    const int x = v % sy, t = v / sy, y = t % (ny + 2), z = t / (ny + 2);
+    float4 jf = F4(f, v)[3];
+    jf.x = 0.0f;
+    jf.y = 0.0f;
+    jf.z = 0.0f;
+    if (x >= 1 && x <= nx + 1 && y >= 1 && y <= ny + 1 && z >= 1 &&
+        z <= nz + 1)
+      jf = unload_add(jf, a, v, sy, sz, cx, cy, cz);
+    F4(f, v)[3] = jf;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_unload_accumulator(
@@ -995,6 +1024,21 @@ int launch_unload_accumulator(vpic_field_t *f, const vpic_accumulator_t *a,
   hipLaunchKernelGGL(k_unload_accumulator, dim3(xcd_grid(n, 256)),
                      dim3(256), 0, s, f, a, b, g->nx, g->ny, cx, cy, cz);
   VPIC_LAUNCHED("k_unload_accumulator");
+  return VPIC_OK;
+}
+
+int launch_clear_unload_accumulator(vpic_field_t *f,
+                                    const vpic_accumulator_t *a,
+                                    const vpic_grid_t *g, hipStream_t s) {
+  // the unload's coefficients, as launch_unload_accumulator computes them
+  const float cx = 0.25 * g->rdy * g->rdz / g->dt;
+  const float cy = 0.25 * g->rdz * g->rdx / g->dt;
+  const float cz = 0.25 * g->rdx * g->rdy / g->dt;
+  int blk = blocks_for(g->nv, 256);
+  if (blk > 65536) blk = 65536;
+  hipLaunchKernelGGL(k_clear_unload_accumulator, dim3(blk), dim3(256), 0, s,
+                     f, a, g->nv, g->nx, g->ny, g->nz, cx, cy, cz);
+  VPIC_LAUNCHED("k_clear_unload_accumulator");
   return VPIC_OK;
 }
 

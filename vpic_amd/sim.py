@@ -185,9 +185,10 @@ class Simulation:
                   self._G(), self.stream)
 
     def unload_accumulator_array(self):
-        _lib.call("vpic_hip_clear_jf", self.fields.ptr, self._G(), self.stream)
-        _lib.call("vpic_hip_unload_accumulator_array", self.fields.ptr,
-                  self.acc.ptr, self._G(), self.stream)
+        # clear_jf + unload_accumulator_array (advance.cc:107-109) in one
+        # pass over the field records
+        _lib.call("vpic_hip_clear_jf_unload_accumulator_array",
+                  self.fields.ptr, self.acc.ptr, self._G(), self.stream)
         _lib.call("vpic_hip_synchronize_jf", self.fields.ptr, self._G(),
                   self.stream)
 
