@@ -105,6 +105,12 @@ __device__ inline void atomic_add_f64(double *a, double v) {
 }
 
 __device__ inline int lane_id() { return __lane_id(); }
+// Number of bits of the 64-bit mask m set below this lane (v_mbcnt: no
+// lane index or lane mask kept in registers).
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi(
+      (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
 
 // ---------------------------------------------------------------------------
 // Host-side helpers shared by the launchers (defined in api.hip)

@@ -241,7 +241,7 @@ struct WindowSink {
           int base = 0;
           if (lane == leader) base = atomicAdd(dcnt + dl, __popcll(same));
           base = __shfl(base, leader);
-          if (d == dl) k = base + __popcll(same & ((1ull << lane) - 1ull));
+          if (d == dl) k = base + lanes_below(same);
           todo &= ~same;
         }
       }
@@ -1581,7 +1581,7 @@ __device__ __forceinline__ void tiled_push(
           mv.dispy = R.dispy;
           mv.dispz = R.dispz;
           mv.i = n;
-          q[qn + __popcll(cm & ((1ull << lane) - 1ull))] = mv;
+          q[qn + lanes_below(cm)] = mv;
         }
         qn += c;
       }
