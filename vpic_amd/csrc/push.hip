@@ -1316,7 +1316,8 @@ __device__ __forceinline__ void tiled_push(
   // converts in FP64 (exact below 2^51: sums of up to 2^20 particles; a
   // larger value takes the float path), as does a particle of larger |q w|.
   // The resolution, 8|q w|max / 2^31 per deposit, is far below FP32
-  // rounding of the cell sums.
+  // rounding of the cell sums.  The same reads give the first species' rows
+  // of the sort-time partition (clamped to [0, np)).
 #pragma unroll
   for (int sp = 0; sp < NSP; sp++) {
     const TileSpecies &S = SP[sp];
@@ -1326,6 +1327,10 @@ __device__ __forceinline__ void tiled_push(
       int e = S.part[vox(x1 + 1, y, z, G.nx, G.ny)];
       a = min(max(a, 0), S.np);
       e = min(max(e, a), S.np);
+      if (sp == 0) {
+        row_start[r] = a;
+        row_pref[r + 1] = e - a;
+      }
       if (a < e) {
         const float4 *pp = reinterpret_cast<const float4 *>(S.p + a);
         const float4 A = pp[0], B = pp[1];
@@ -1355,19 +1360,22 @@ __device__ __forceinline__ void tiled_push(
     int *__restrict__ cnt = SP[sp].cnt;
     const PushCoef C = SP[sp].C;
     BS.vkeys = SP[sp].vkeys;
-    // all wavefronts are done with the last species' rows
-    if (sp > 0) __syncthreads();
-    // Rows of the tile in the sort-time partition, clamped to [0, np).
-    for (int r = t; r < nrow; r += TILE_BS) {
-      const int y = y0 + r % rows_y, z = z0 + r / rows_y;
-      int s = part[vox(x0, y, z, G.nx, G.ny)];
-      int e = part[vox(x1 + 1, y, z, G.nx, G.ny)];
-      s = min(max(s, 0), np);
-      e = min(max(e, s), np);
-      row_start[r] = s;
-      row_pref[r + 1] = e - s;
+    if (sp > 0) {
+      // all wavefronts are done with the last species' rows
+      __syncthreads();
+      // Rows of the tile in the sort-time partition, clamped to [0, np)
+      // (the first species' were read with the scale).
+      for (int r = t; r < nrow; r += TILE_BS) {
+        const int y = y0 + r % rows_y, z = z0 + r / rows_y;
+        int s = part[vox(x0, y, z, G.nx, G.ny)];
+        int e = part[vox(x1 + 1, y, z, G.nx, G.ny)];
+        s = min(max(s, 0), np);
+        e = min(max(e, s), np);
+        row_start[r] = s;
+        row_pref[r + 1] = e - s;
+      }
+      __syncthreads();
     }
-    __syncthreads();
     if (t < 64) {
       // inclusive prefix of the row lengths by the first wavefront: four rows
       // per lane (nrow <= TILE_MAXROWS = 256), then the lanes' totals
