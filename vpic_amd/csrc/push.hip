@@ -1272,6 +1272,39 @@ __device__ __forceinline__ void tiled_push(
   TB.hz = btz + 1 < T.ntz ? min(z1 + T.tz, G.nz) : z1;
 
   for (int k = t; k < 12 * ncell; k += TILE_BS) lds_win[k] = 0ull;
+  // The window's fixed-point scale (below) samples the first particle of
+  // every row of every species; those reads (the row bounds, then the
+  // particle's voxel and weight) are issued here, so that the particle
+  // reads overlap the staging reads that follow, and are used after the
+  // barrier.  The same row bounds are the first species' rows of the
+  // sort-time partition (clamped to [0, np)).  One row per lane.
+  static_assert(TILE_MAXROWS <= TILE_BS, "one row per lane");
+  int s_vox[NSP];
+  float s_w[NSP];
+  bool s_has[NSP];
+#pragma unroll
+  for (int sp = 0; sp < NSP; sp++) {
+    const TileSpecies &S = SP[sp];
+    s_has[sp] = false;
+    s_vox[sp] = 0;
+    s_w[sp] = 0.0f;
+    if (t < nrow) {
+      const int y = y0 + t % rows_y, z = z0 + t / rows_y;
+      int a = S.part[vox(x0, y, z, G.nx, G.ny)];
+      int e = S.part[vox(x1 + 1, y, z, G.nx, G.ny)];
+      a = min(max(a, 0), S.np);
+      e = min(max(e, a), S.np);
+      if (sp == 0) {
+        row_start[t] = a;
+        row_pref[t + 1] = e - a;
+      }
+      if (a < e) {
+        s_vox[sp] = S.p[a].i;
+        s_w[sp] = S.p[a].w;
+        s_has[sp] = true;
+      }
+    }
+  }
   // Interpolation coefficients staged in LDS (after the window) for the
   // cells of the staging box: the tile and a one-cell ring around it (which
   // stays within the ghost layers) when the launch made room for it, else
@@ -1310,35 +1343,18 @@ __device__ __forceinline__ void tiled_push(
   if (t < 27) oow_dmax[t] = 0u;
   __syncthreads();
   // Fixed-point scale of the window: 2^k with 2^k * 8|q w|max <= 2^31 for
-  // the first particle of every row of every species.  One particle's
-  // current is below 4.4|q w|, so its components scale below 2^31 and
-  // convert in FP32; a register sum of many of them may exceed 2^31 and
-  // converts in FP64 (exact below 2^51: sums of up to 2^20 particles; a
-  // larger value takes the float path), as does a particle of larger |q w|.
-  // The resolution, 8|q w|max / 2^31 per deposit, is far below FP32
-  // rounding of the cell sums.  The same reads give the first species' rows
-  // of the sort-time partition (clamped to [0, np)).
+  // the first particle of every row of every species (read above).  One
+  // particle's current is below 4.4|q w|, so its components scale below
+  // 2^31 and convert in FP32; a register sum of many of them may exceed
+  // 2^31 and converts in FP64 (exact below 2^51: sums of up to 2^20
+  // particles; a larger value takes the float path), as does a particle of
+  // larger |q w|.  The resolution, 8|q w|max / 2^31 per deposit, is far
+  // below FP32 rounding of the cell sums.
 #pragma unroll
-  for (int sp = 0; sp < NSP; sp++) {
-    const TileSpecies &S = SP[sp];
-    for (int r = t; r < nrow; r += TILE_BS) {
-      const int y = y0 + r % rows_y, z = z0 + r / rows_y;
-      int a = S.part[vox(x0, y, z, G.nx, G.ny)];
-      int e = S.part[vox(x1 + 1, y, z, G.nx, G.ny)];
-      a = min(max(a, 0), S.np);
-      e = min(max(e, a), S.np);
-      if (sp == 0) {
-        row_start[r] = a;
-        row_pref[r + 1] = e - a;
-      }
-      if (a < e) {
-        const float4 *pp = reinterpret_cast<const float4 *>(S.p + a);
-        const float4 A = pp[0], B = pp[1];
-        if ((unsigned)__float_as_int(A.w) < (unsigned)G.nv)
-          atomicMax(&qw_bits, __float_as_uint(fabsf(S.C.qsp * B.w)));
-      }
-    }
-  }
+  for (int sp = 0; sp < NSP; sp++)
+    if (s_has[sp] &&
+        (unsigned)s_vox[sp] < (unsigned)G.nv)
+      atomicMax(&qw_bits, __float_as_uint(fabsf(SP[sp].C.qsp * s_w[sp])));
   __syncthreads();
   {
     const float m = __uint_as_float(qw_bits);
