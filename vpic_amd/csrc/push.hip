@@ -1279,7 +1279,7 @@ __device__ __forceinline__ void tiled_push(
   // barrier.  The same row bounds are the first species' rows of the
   // sort-time partition (clamped to [0, np)).  One row per lane.
   static_assert(TILE_MAXROWS <= TILE_BS, "one row per lane");
-  int s_vox[NSP];
+  int s_vox[NSP], s_a[NSP], s_e[NSP];
   float s_w[NSP];
   bool s_has[NSP];
 #pragma unroll
@@ -1287,6 +1287,7 @@ __device__ __forceinline__ void tiled_push(
     const TileSpecies &S = SP[sp];
     s_has[sp] = false;
     s_vox[sp] = 0;
+    s_a[sp] = s_e[sp] = 0;
     s_w[sp] = 0.0f;
     if (t < nrow) {
       const int y = y0 + t % rows_y, z = z0 + t / rows_y;
@@ -1294,6 +1295,8 @@ __device__ __forceinline__ void tiled_push(
       int e = S.part[vox(x1 + 1, y, z, G.nx, G.ny)];
       a = min(max(a, 0), S.np);
       e = min(max(e, a), S.np);
+      s_a[sp] = a;
+      s_e[sp] = e;
       if (sp == 0) {
         row_start[t] = a;
         row_pref[t + 1] = e - a;
@@ -1370,7 +1373,6 @@ __device__ __forceinline__ void tiled_push(
   for (int sp = 0; sp < NSP; sp++) {
     vpic_particle_t *__restrict__ p = SP[sp].p;
     const int np = SP[sp].np;
-    const int *__restrict__ part = SP[sp].part;
     vpic_particle_mover_t *__restrict__ rlist = SP[sp].rlist;
     const int rcap = SP[sp].rcap;
     int *__restrict__ cnt = SP[sp].cnt;
@@ -1379,16 +1381,10 @@ __device__ __forceinline__ void tiled_push(
     if (sp > 0) {
       // all wavefronts are done with the last species' rows
       __syncthreads();
-      // Rows of the tile in the sort-time partition, clamped to [0, np)
-      // (the first species' were read with the scale).
-      for (int r = t; r < nrow; r += TILE_BS) {
-        const int y = y0 + r % rows_y, z = z0 + r / rows_y;
-        int s = part[vox(x0, y, z, G.nx, G.ny)];
-        int e = part[vox(x1 + 1, y, z, G.nx, G.ny)];
-        s = min(max(s, 0), np);
-        e = min(max(e, s), np);
-        row_start[r] = s;
-        row_pref[r + 1] = e - s;
+      // this species' rows, read in the prologue
+      if (t < nrow) {
+        row_start[t] = s_a[sp];
+        row_pref[t + 1] = s_e[sp] - s_a[sp];
       }
       __syncthreads();
     }
